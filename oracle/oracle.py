@@ -1,0 +1,252 @@
+"""ctypes front end of the CPU oracle — TEST INFRASTRUCTURE ONLY.
+
+Only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg import
+this module.  The product (rama_amd, librama_hip.so, rama_py) never does.
+Each function mirrors the stage entry point of the same name in
+include/rama_hip.h, so a parity test calls both with the same arguments.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+from . import build as _build
+
+_lib = None
+
+I32P = ctypes.POINTER(ctypes.c_int32)
+F32P = ctypes.POINTER(ctypes.c_float)
+I64P = ctypes.POINTER(ctypes.c_int64)
+F64P = ctypes.POINTER(ctypes.c_double)
+
+
+class Options(ctypes.Structure):
+    """Layout of rama_options_c (include/rama_hip.h)."""
+
+    _fields_ = [
+        ("max_cycle_length_lb", ctypes.c_int32),
+        ("num_dual_itr_lb", ctypes.c_int32),
+        ("max_cycle_length_primal", ctypes.c_int32),
+        ("num_dual_itr_primal", ctypes.c_int32),
+        ("num_outer_itr_dual", ctypes.c_int32),
+        ("mean_multiplier_mm", ctypes.c_float),
+        ("matching_thresh_crossover_ratio", ctypes.c_float),
+        ("tri_memory_factor", ctypes.c_float),
+        ("only_compute_lb", ctypes.c_int32),
+        ("max_time_sec", ctypes.c_int32),
+        ("dump_timeline", ctypes.c_int32),
+        ("verbose", ctypes.c_int32),
+        ("sanitize_graph", ctypes.c_int32),
+    ]
+
+
+# multicut_solver_options presets (include/multicut_solver_options.h:5-53)
+_PRESETS = {
+    "PD": {},
+    "P": dict(max_cycle_length_lb=0, num_dual_itr_lb=0, max_cycle_length_primal=0, num_dual_itr_primal=0),
+    "PD+": dict(max_cycle_length_lb=5, num_dual_itr_lb=10, max_cycle_length_primal=5, num_dual_itr_primal=10),
+    "D": dict(max_cycle_length_lb=5, num_dual_itr_lb=10, num_outer_itr_dual=5, only_compute_lb=1),
+}
+
+
+def options(preset: str = "PD", **kw) -> Options:
+    o = Options(5, 10, 3, 5, 1, 0.05, 0.05, 2.0, 0, -1, 0, 0, 0)
+    for k, v in _PRESETS[preset].items():
+        setattr(o, k, v)
+    for k, v in kw.items():
+        setattr(o, k, v)
+    return o
+
+
+VP = ctypes.c_void_p
+I32 = ctypes.c_int32
+I64 = ctypes.c_int64
+F32 = ctypes.c_float
+F64 = ctypes.c_double
+OPTP = ctypes.POINTER(Options)
+
+_SIGS = {
+    "oracle_solve": [I32P, I32P, F32P, I64, OPTP, I32P, I64, I64P, F64P, I32P],
+    "oracle_dcoo_init": [I32P, I32P, F32P, I64, I32P, I32P, F32P, I32P],
+    "oracle_contract": [I32P, I32P, F32P, I64, I32P, I64, I32P, I32P, F32P, I64P, I32P, I32P, F64P],
+    "oracle_compute_offsets": [I32P, I64, I32, I32P],
+    "oracle_compress_labels": [I32P, I64, I32, I32P],
+    "oracle_matching": [I32P, I32P, F32P, I64, I32, I32, F32, I32P, I64P, I32P],
+    "oracle_conflicted_cycles": [I32P, I32P, F32P, I64, I32, I32, I32, F32, F32, I32P, I32P, I32P, I64, I64P, I64P],
+    "oracle_message_passing": [I32P, I32P, F32P, I64, I32P, I32P, I32P, I64, I32, I32, I32P, I32P, F32P, I64, I64P,
+                               F64P],
+    "oracle_dual_solver": [I32P, I32P, F32P, I64, I32, I32, I32, I32, F32, I32, F32, I32P, I32P, F32P, I64, I64P, I32P,
+                           F64P],
+    "oracle_connected_components": [I32P, I32P, I32, I64, I32P],
+    "oracle_max_spanning_forest": [I32P, I32P, F32P, I64, I32P, I32P, F32P, I64P],
+    "oracle_woc_mapping": [I32P, I32P, F32P, I64, I32, I32, I32P, I64P, I32P],
+    "oracle_objective": [I32P, I64, I32P, I32P, F32P, I64],
+}
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        path = _build.build(force=False)
+        _lib = ctypes.CDLL(path)
+        _lib.oracle_last_error.restype = ctypes.c_char_p
+        _lib.oracle_objective.restype = ctypes.c_double
+        for name, args in _SIGS.items():
+            getattr(_lib, name).argtypes = args
+    return _lib
+
+
+def _p(a, t):
+    return a.ctypes.data_as(t)
+
+
+def _i32(a):
+    return np.ascontiguousarray(a, dtype=np.int32)
+
+
+def _f32(a):
+    return np.ascontiguousarray(a, dtype=np.float32)
+
+
+def _check(rc):
+    if rc != 0:
+        raise RuntimeError(lib().oracle_last_error().decode())
+
+
+def solve(i, j, c, opts=None):
+    """Full solve (src/rama_cuda.cu:166-193). Returns (labels, lb, iterations)."""
+    opts = opts or options()
+    i, j, c = _i32(i), _i32(j), _f32(c)
+    cap = int(max(i.max(initial=0), j.max(initial=0))) + 2
+    lab = np.zeros(cap, np.int32)
+    n = ctypes.c_int64()
+    lb = ctypes.c_double()
+    it = ctypes.c_int32()
+    _check(lib().oracle_solve(_p(i, I32P), _p(j, I32P), _p(c, F32P), len(c), ctypes.byref(opts), _p(lab, I32P),
+                              cap, ctypes.byref(n), ctypes.byref(lb), ctypes.byref(it)))
+    return lab[: n.value].copy(), lb.value, it.value
+
+
+def dcoo_init(i, j, c):
+    i, j, c = _i32(i), _i32(j), _f32(c)
+    m = len(c)
+    r, co, v = np.zeros(m, np.int32), np.zeros(m, np.int32), np.zeros(m, np.float32)
+    dims = np.zeros(2, np.int32)
+    _check(lib().oracle_dcoo_init(_p(i, I32P), _p(j, I32P), _p(c, F32P), m, _p(r, I32P), _p(co, I32P),
+                                  _p(v, F32P), _p(dims, I32P)))
+    return r, co, v, (int(dims[0]), int(dims[1]))
+
+
+def contract(row, col, val, node_map):
+    row, col, val, node_map = _i32(row), _i32(col), _f32(val), _i32(node_map)
+    m = len(val)
+    r, co, v = np.zeros(m, np.int32), np.zeros(m, np.int32), np.zeros(m, np.float32)
+    mo = ctypes.c_int64()
+    dims = np.zeros(2, np.int32)
+    bad = ctypes.c_int32()
+    ds = ctypes.c_double()
+    _check(lib().oracle_contract(_p(row, I32P), _p(col, I32P), _p(val, F32P), m, _p(node_map, I32P), len(node_map),
+                                 _p(r, I32P), _p(co, I32P), _p(v, F32P), ctypes.byref(mo), _p(dims, I32P),
+                                 ctypes.byref(bad), ctypes.byref(ds)))
+    k = mo.value
+    return r[:k].copy(), co[:k].copy(), v[:k].copy(), (int(dims[0]), int(dims[1])), bool(bad.value), ds.value
+
+
+def compute_offsets(ids, max_value):
+    ids = _i32(ids)
+    out = np.zeros(max_value + 2, np.int32)
+    _check(lib().oracle_compute_offsets(_p(ids, I32P), len(ids), max_value, _p(out, I32P)))
+    return out
+
+
+def compress_labels(data, max_label):
+    data = _i32(data)
+    out = np.zeros(len(data), np.int32)
+    _check(lib().oracle_compress_labels(_p(data, I32P), len(data), max_label, _p(out, I32P)))
+    return out
+
+
+def matching(row, col, val, rows, cols, mm):
+    row, col, val = _i32(row), _i32(col), _f32(val)
+    out = np.zeros(rows + cols + 1, np.int32)
+    n = ctypes.c_int64()
+    matched = ctypes.c_int32()
+    _check(lib().oracle_matching(_p(row, I32P), _p(col, I32P), _p(val, F32P), len(val), rows, cols,
+                                 ctypes.c_float(mm), _p(out, I32P), ctypes.byref(n), ctypes.byref(matched)))
+    return out[: n.value].copy(), matched.value
+
+
+def conflicted_cycles(row, col, val, rows, cols, max_len, factor=2.0, tol=1e-4):
+    row, col, val = _i32(row), _i32(col), _f32(val)
+    cap = int(factor * len(val)) + 16
+    t = [np.zeros(cap, np.int32) for _ in range(3)]
+    n = ctypes.c_int64()
+    raw = ctypes.c_int64()
+    _check(lib().oracle_conflicted_cycles(_p(row, I32P), _p(col, I32P), _p(val, F32P), len(val), rows, cols, max_len,
+                                          ctypes.c_float(factor), ctypes.c_float(tol), _p(t[0], I32P),
+                                          _p(t[1], I32P), _p(t[2], I32P), cap, ctypes.byref(n), ctypes.byref(raw)))
+    k = n.value
+    return t[0][:k].copy(), t[1][:k].copy(), t[2][:k].copy(), raw.value
+
+
+def message_passing(row, col, val, t1, t2, t3, pre=0, iters=0):
+    row, col, val = _i32(row), _i32(col), _f32(val)
+    t1, t2, t3 = _i32(t1), _i32(t2), _i32(t3)
+    cap = len(val) + 3 * len(t1) + 1
+    oi, oj, oc = np.zeros(cap, np.int32), np.zeros(cap, np.int32), np.zeros(cap, np.float32)
+    om = ctypes.c_int64()
+    lbs = np.zeros(iters + 1, np.float64)
+    _check(lib().oracle_message_passing(_p(row, I32P), _p(col, I32P), _p(val, F32P), len(val), _p(t1, I32P),
+                                        _p(t2, I32P), _p(t3, I32P), len(t1), pre, iters, _p(oi, I32P), _p(oj, I32P),
+                                        _p(oc, F32P), cap, ctypes.byref(om), _p(lbs, F64P)))
+    k = om.value
+    return oi[:k].copy(), oj[:k].copy(), oc[:k].copy(), lbs
+
+
+def dual_solver(row, col, val, rows, cols, max_len, num_iter, factor=2.0, num_outer=1, tol=1e-4):
+    row, col, val = _i32(row), _i32(col), _f32(val)
+    cap = len(val) * (1 + int(3 * factor) * 3 * max(1, num_outer)) + 16
+    r, c, v = np.zeros(cap, np.int32), np.zeros(cap, np.int32), np.zeros(cap, np.float32)
+    om = ctypes.c_int64()
+    dims = np.zeros(2, np.int32)
+    lb = ctypes.c_double()
+    _check(lib().oracle_dual_solver(_p(row, I32P), _p(col, I32P), _p(val, F32P), len(val), rows, cols, max_len,
+                                    num_iter, ctypes.c_float(factor), num_outer, ctypes.c_float(tol), _p(r, I32P),
+                                    _p(c, I32P), _p(v, F32P), cap, ctypes.byref(om), _p(dims, I32P),
+                                    ctypes.byref(lb)))
+    k = om.value
+    return r[:k].copy(), c[:k].copy(), v[:k].copy(), (int(dims[0]), int(dims[1])), lb.value
+
+
+def connected_components(offsets, cols):
+    offsets, cols = _i32(offsets), _i32(cols)
+    n = len(offsets) - 1
+    out = np.zeros(n, np.int32)
+    _check(lib().oracle_connected_components(_p(offsets, I32P), _p(cols, I32P), n, len(cols), _p(out, I32P)))
+    return out
+
+
+def max_spanning_forest(row, col, val):
+    row, col, val = _i32(row), _i32(col), _f32(val)
+    m = len(val)
+    r, c, v = np.zeros(m, np.int32), np.zeros(m, np.int32), np.zeros(m, np.float32)
+    om = ctypes.c_int64()
+    _check(lib().oracle_max_spanning_forest(_p(row, I32P), _p(col, I32P), _p(val, F32P), m, _p(r, I32P),
+                                            _p(c, I32P), _p(v, F32P), ctypes.byref(om)))
+    k = om.value
+    return r[:k].copy(), c[:k].copy(), v[:k].copy()
+
+
+def woc_mapping(row, col, val, rows, cols):
+    row, col, val = _i32(row), _i32(col), _f32(val)
+    out = np.zeros(max(rows, cols) + 1, np.int32)
+    n = ctypes.c_int64()
+    cnt = ctypes.c_int32()
+    _check(lib().oracle_woc_mapping(_p(row, I32P), _p(col, I32P), _p(val, F32P), len(val), rows, cols,
+                                    _p(out, I32P), ctypes.byref(n), ctypes.byref(cnt)))
+    return out[: n.value].copy(), cnt.value
+
+
+def objective(labels, i, j, c):
+    labels, i, j, c = _i32(labels), _i32(i), _i32(j), _f32(c)
+    return lib().oracle_objective(_p(labels, I32P), len(labels), _p(i, I32P), _p(j, I32P), _p(c, F32P), len(c))
