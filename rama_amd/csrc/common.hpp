@@ -1,0 +1,239 @@
+// common.hpp — runtime plumbing shared by every translation unit of
+// librama_hip.so: HIP error handling, a caching device allocator, typed
+// device buffers and the per-solve context (device, stream, pinned scalars).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <cstring>
+#include <cstdio>
+#include <map>
+#include <mutex>
+#include <stdexcept>
+#include <string>
+#include <unordered_map>
+#include <utility>
+#include <vector>
+
+#include "../../include/rama_hip.h"
+
+namespace rama {
+
+using i64 = int64_t;
+using u64 = uint64_t;
+using u32 = uint32_t;
+
+// Error carried to the C ABI as (status, message).
+struct Error : std::runtime_error {
+    int status;
+    Error(int s, const std::string& m) : std::runtime_error(m), status(s) {}
+};
+
+#define RAMA_HIP(expr)                                                                          \
+    do {                                                                                        \
+        hipError_t _e = (expr);                                                                 \
+        if (_e != hipSuccess)                                                                   \
+            throw ::rama::Error(_e == hipErrorOutOfMemory ? RAMA_ERR_OOM : RAMA_ERR_HIP,        \
+                                std::string(#expr " failed: ") + hipGetErrorString(_e) + " at " \
+                                + __FILE__ + ":" + std::to_string(__LINE__));                   \
+    } while (0)
+
+#define RAMA_REQUIRE(cond, msg)                                       \
+    do {                                                              \
+        if (!(cond)) throw ::rama::Error(RAMA_ERR_INVALID, (msg));    \
+    } while (0)
+
+// Launch-error check after a kernel launch.
+#define RAMA_LAUNCHED() RAMA_HIP(hipGetLastError())
+
+inline unsigned grid_for(i64 n, int block = 256) {
+    i64 g = (n + block - 1) / block;
+    if (g < 1) g = 1;
+    return (unsigned)g;
+}
+
+// Caching device allocator.  Blocks are rounded to a size class (1/8 steps
+// between powers of two) and recycled on the owning stream's order: a block
+// released while kernels that use it are queued is only handed to later work
+// on the same stream, so no synchronisation is needed.
+class Arena {
+   public:
+    explicit Arena(int device) : device_(device) {}
+    ~Arena() { trim(); }
+    Arena(const Arena&) = delete;
+    Arena& operator=(const Arena&) = delete;
+
+    void* alloc(size_t bytes) {
+        if (bytes == 0) bytes = 1;
+        const size_t cls = size_class(bytes);
+        auto it = free_.find(cls);
+        void* p = nullptr;
+        if (it != free_.end()) {
+            p = it->second;
+            free_.erase(it);
+        } else {
+            hipError_t e = hipMalloc(&p, cls);
+            if (e != hipSuccess) {
+                (void)hipGetLastError();
+                trim();
+                e = hipMalloc(&p, cls);
+            }
+            if (e != hipSuccess) {
+                (void)hipGetLastError();
+                throw Error(RAMA_ERR_OOM, "device allocation of " + std::to_string(cls) + " bytes failed");
+            }
+            reserved_ += cls;
+        }
+        used_[p] = cls;
+        live_ += cls;
+        if (live_ > peak_) peak_ = live_;
+        return p;
+    }
+
+    void release(void* p) {
+        if (!p) return;
+        auto it = used_.find(p);
+        if (it == used_.end()) return;
+        free_.emplace(it->second, p);
+        live_ -= it->second;
+        used_.erase(it);
+    }
+
+    // Return cached blocks to the driver.
+    void trim() {
+        for (auto& kv : free_) {
+            (void)hipFree(kv.second);
+            reserved_ -= kv.first;
+        }
+        free_.clear();
+    }
+
+    size_t peak() const { return peak_; }
+    size_t live() const { return live_; }
+    size_t reserved() const { return reserved_; }
+    void reset_peak() { peak_ = live_; }
+
+    static size_t size_class(size_t b) {
+        if (b <= 4096) return 4096;
+        size_t p = 1;
+        while (p < b) p <<= 1;
+        const size_t step = p >> 3;
+        return ((b + step - 1) / step) * step;
+    }
+
+   private:
+    int device_;
+    std::multimap<size_t, void*> free_;
+    std::unordered_map<void*, size_t> used_;
+    size_t live_ = 0, peak_ = 0, reserved_ = 0;
+};
+
+// Typed device buffer owned by an Arena (move-only).
+template <class T>
+class DBuf {
+   public:
+    DBuf() = default;
+    DBuf(Arena& a, i64 n) : a_(&a), n_(n) { p_ = n > 0 ? static_cast<T*>(a.alloc(sizeof(T) * (size_t)n)) : nullptr; }
+    ~DBuf() { reset(); }
+    DBuf(DBuf&& o) noexcept : a_(o.a_), p_(o.p_), n_(o.n_) { o.p_ = nullptr; o.n_ = 0; }
+    DBuf& operator=(DBuf&& o) noexcept {
+        if (this != &o) {
+            reset();
+            a_ = o.a_; p_ = o.p_; n_ = o.n_;
+            o.p_ = nullptr; o.n_ = 0;
+        }
+        return *this;
+    }
+    DBuf(const DBuf&) = delete;
+    DBuf& operator=(const DBuf&) = delete;
+
+    void reset() {
+        if (p_ && a_) a_->release(p_);
+        p_ = nullptr;
+        n_ = 0;
+    }
+    // Logical shrink: keeps the allocation, changes the element count.
+    void set_size(i64 n) { n_ = n; }
+    T* get() const { return p_; }
+    i64 size() const { return n_; }
+    bool empty() const { return n_ == 0; }
+    operator T*() const { return p_; }
+
+   private:
+    Arena* a_ = nullptr;
+    T* p_ = nullptr;
+    i64 n_ = 0;
+};
+
+// Per-solve execution context.
+struct Ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    bool own_stream = false;
+    Arena arena;
+    bool verbose = false;
+    void* pinned = nullptr;  // small pinned host area for scalar read-back
+
+    Ctx(int dev, hipStream_t s, bool verb) : device(dev), arena(dev), verbose(verb) {
+        RAMA_HIP(hipSetDevice(dev));
+        if (s) {
+            stream = s;
+        } else {
+            RAMA_HIP(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
+            own_stream = true;
+        }
+        RAMA_HIP(hipHostMalloc(&pinned, 4096, hipHostMallocDefault));
+    }
+    ~Ctx() {
+        if (stream) (void)hipStreamSynchronize(stream);
+        arena.trim();
+        if (pinned) (void)hipHostFree(pinned);
+        if (own_stream) (void)hipStreamDestroy(stream);
+    }
+    Ctx(const Ctx&) = delete;
+    Ctx& operator=(const Ctx&) = delete;
+
+    template <class T>
+    DBuf<T> buf(i64 n) { return DBuf<T>(arena, n); }
+
+    void sync() { RAMA_HIP(hipStreamSynchronize(stream)); }
+
+    // Read `n` device scalars (contiguous) into host memory (synchronises).
+    template <class T>
+    void read(const T* d, T* h, int n = 1) {
+        RAMA_HIP(hipMemcpyAsync(pinned, d, sizeof(T) * n, hipMemcpyDeviceToHost, stream));
+        sync();
+        std::memcpy(h, pinned, sizeof(T) * n);
+    }
+    template <class T>
+    T read1(const T* d) {
+        T v;
+        read(d, &v, 1);
+        return v;
+    }
+    template <class T>
+    void write1(T* d, T v) {
+        RAMA_HIP(hipMemcpyAsync(d, &v, sizeof(T), hipMemcpyHostToDevice, stream));
+        sync();
+    }
+    template <class T>
+    void h2d(T* d, const T* h, i64 n) {
+        if (n > 0) RAMA_HIP(hipMemcpyAsync(d, h, sizeof(T) * (size_t)n, hipMemcpyHostToDevice, stream));
+    }
+    template <class T>
+    void d2h(T* h, const T* d, i64 n) {
+        if (n > 0) RAMA_HIP(hipMemcpyAsync(h, d, sizeof(T) * (size_t)n, hipMemcpyDeviceToHost, stream));
+        sync();
+    }
+    template <class T>
+    void d2d(T* dst, const T* src, i64 n) {
+        if (n > 0) RAMA_HIP(hipMemcpyAsync(dst, src, sizeof(T) * (size_t)n, hipMemcpyDeviceToDevice, stream));
+    }
+    template <class T>
+    void zero(T* d, i64 n) {
+        if (n > 0) RAMA_HIP(hipMemsetAsync(d, 0, sizeof(T) * (size_t)n, stream));
+    }
+};
+
+}  // namespace rama

@@ -1,0 +1,341 @@
+// cycles.hip — conflicted 3-, 4- and 5-cycle search and triangulation
+// (src/conflicted_cycles_cuda.cu:13-343, include/rama_utils.h:62-84,296-363).
+//
+// A conflicted cycle is one repulsive edge (v1, v2) closed by a path of
+// attractive edges.  Each stage enumerates "expansions" and intersects two
+// sorted attractive neighbour lists per expansion:
+//   3-cycles: expansion = rep edge r;                 N(v1) ∩ N(v2)
+//   4-cycles: expansion = (r, n1 ∈ N(v1));            N(n1) ∩ N(v2)
+//   5-cycles: expansion = (r, n1 ∈ N(v1), n2 ∈ N(v2)); N(n1) ∩ N(n2), n1≠n2
+// and writes 1, 2 or 3 triangles per common neighbour `mid`.
+//
+// Deterministic output order (the reference's atomic slot counter is
+// replaced): a count pass stores each expansion's number of mids (uint8,
+// saturating) and exact per-tile totals; a scan of the tile totals gives
+// every tile its write offset; an emit pass revisits only expansions with
+// mids and writes at enumeration position (rep edge, expansion, mid).  The
+// triangle budget therefore keeps exactly the first `budget` writes.
+#include "rama_internal.hpp"
+
+namespace rama {
+
+constexpr int kCycTile = kBlock;  // expansions per block
+
+struct CycArgs {
+    const int* off;      // attractive CSR offsets (n+1)
+    const int* adj;      // attractive CSR neighbours
+    const int* rv1;      // repulsive edge rows (v1 < v2)
+    const int* rv2;
+    const i64* xoff;     // per-rep exclusive expansion offsets (R+1)
+    const int* tile_rep; // first rep edge of each tile
+    i64 R;
+    i64 X;               // number of expansions
+};
+
+// Locate the rep edge owning expansion x: xoff[r] <= x < xoff[r+1], searching
+// from the tile's first rep edge.
+__device__ __forceinline__ i64 find_rep(const CycArgs& a, i64 x, i64 r0) {
+    i64 lo = r0, hi = a.R - 1;
+    // galloping upper bound then binary search
+    i64 step = 1;
+    while (lo + step <= hi && a.xoff[lo + step] <= x) {
+        lo += step;
+        step <<= 1;
+    }
+    hi = lo + step < a.R - 1 ? lo + step : a.R - 1;
+    while (lo < hi) {
+        const i64 mid = (lo + hi + 1) >> 1;
+        if (a.xoff[mid] <= x) lo = mid; else hi = mid - 1;
+    }
+    return lo;
+}
+
+// Resolve expansion x to the two vertices whose neighbour lists intersect and
+// the triangle writers' (v1, v2, n1, n2).  Returns false if the expansion is
+// skipped (5-cycles with n1 == n2).
+template <int S>
+__device__ __forceinline__ bool resolve(const CycArgs& a, i64 x, i64 r0, int& v1, int& v2, int& n1, int& n2,
+                                        int& qa, int& qb) {
+    const i64 r = find_rep(a, x, r0);
+    v1 = a.rv1[r];
+    v2 = a.rv2[r];
+    const i64 loc = x - a.xoff[r];
+    if (S == 3) {
+        qa = v1;
+        qb = v2;
+        n1 = n2 = -1;
+        return true;
+    } else if (S == 4) {
+        n1 = a.adj[a.off[v1] + loc];
+        n2 = -1;
+        qa = n1;
+        qb = v2;
+        return true;
+    } else {
+        const int d1 = a.off[v1 + 1] - a.off[v1];
+        const int l1 = (int)(loc % d1), l2 = (int)(loc / d1);
+        n1 = a.adj[a.off[v1] + l1];
+        n2 = a.adj[a.off[v2] + l2];
+        qa = n1;
+        qb = n2;
+        return n1 != n2;
+    }
+}
+
+// Number of common neighbours of qa and qb (merge walk over sorted lists).
+__device__ __forceinline__ int count_common(const int* __restrict__ off, const int* __restrict__ adj, int qa, int qb) {
+    int i = off[qa], ie = off[qa + 1], j = off[qb], je = off[qb + 1];
+    int c = 0;
+    while (i < ie && j < je) {
+        const int x = adj[i], y = adj[j];
+        c += (x == y);
+        i += (x <= y);
+        j += (y <= x);
+    }
+    return c;
+}
+
+template <int S>
+__global__ __launch_bounds__(kBlock) void k_cycle_count(CycArgs a, unsigned char* __restrict__ cnt8,
+                                                        i64* __restrict__ tile_tot) {
+    __shared__ i64 ws[kWaves];
+    const i64 x = (i64)blockIdx.x * kCycTile + threadIdx.x;
+    int c = 0;
+    if (x < a.X) {
+        int v1, v2, n1, n2, qa, qb;
+        if (resolve<S>(a, x, a.tile_rep[blockIdx.x], v1, v2, n1, n2, qa, qb)) c = count_common(a.off, a.adj, qa, qb);
+        cnt8[x] = (unsigned char)(c < 255 ? c : 255);
+    }
+    const i64 tot = block_reduce((i64)c * (S - 2), OpSum(), ws);
+    if (threadIdx.x == 0) tile_tot[blockIdx.x] = tot;
+}
+
+__device__ __forceinline__ void put_tri(int* t1, int* t2, int* t3, i64 g, i64 budget, int a, int b, int c) {
+    if (g >= budget) return;
+    const int lo = min(a, min(b, c));
+    const int hi = max(a, max(b, c));
+    const int md = max(min(a, b), min(max(a, b), c));
+    t1[g] = lo;
+    t2[g] = md;
+    t3[g] = hi;
+}
+
+template <int S>
+__global__ __launch_bounds__(kBlock) void k_cycle_emit(CycArgs a, const unsigned char* __restrict__ cnt8,
+                                                       const i64* __restrict__ tile_off, i64 base, i64 budget,
+                                                       int* __restrict__ t1, int* __restrict__ t2,
+                                                       int* __restrict__ t3) {
+    __shared__ i64 ws[kWaves];
+    const i64 tstart = base + tile_off[blockIdx.x];
+    if (tstart >= budget) return;  // uniform per block
+    const i64 x = (i64)blockIdx.x * kCycTile + threadIdx.x;
+    int c = 0;
+    int v1 = 0, v2 = 0, n1 = 0, n2 = 0, qa = 0, qb = 0;
+    if (x < a.X) {
+        c = cnt8[x];
+        if (c > 0) {
+            resolve<S>(a, x, a.tile_rep[blockIdx.x], v1, v2, n1, n2, qa, qb);
+            if (c == 255) c = count_common(a.off, a.adj, qa, qb);
+        }
+    }
+    i64 tot;
+    const i64 mypos = tstart + block_excl_scan((i64)c * (S - 2), ws, &tot);
+    if (c == 0 || mypos >= budget) return;
+    int i = a.off[qa], ie = a.off[qa + 1], j = a.off[qb], je = a.off[qb + 1];
+    i64 g = mypos;
+    while (i < ie && j < je && g < budget) {
+        const int p = a.adj[i], q = a.adj[j];
+        if (p == q) {
+            const int mid = p;
+            if (S == 3) {
+                put_tri(t1, t2, t3, g++, budget, v1, v2, mid);
+            } else if (S == 4) {
+                put_tri(t1, t2, t3, g++, budget, v1, v2, n1);
+                put_tri(t1, t2, t3, g++, budget, v2, n1, mid);
+            } else {
+                put_tri(t1, t2, t3, g++, budget, v1, v2, n1);
+                put_tri(t1, t2, t3, g++, budget, v2, n1, mid);
+                put_tri(t1, t2, t3, g++, budget, v2, mid, n2);
+            }
+        }
+        i += (p <= q);
+        j += (q <= p);
+    }
+}
+
+// tile_rep[t] = rep edge containing the first expansion of tile t.
+__global__ void k_tile_rep(const i64* __restrict__ xoff, i64 R, int* __restrict__ tile_rep) {
+    const i64 r = (i64)blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= R) return;
+    const i64 b = xoff[r], e = xoff[r + 1];
+    if (e <= b) return;
+    for (i64 t = (b + kCycTile - 1) / kCycTile; t * kCycTile < e; ++t) tile_rep[t] = (int)r;
+}
+
+// Runs one stage over all expansions; returns the number of attempted writes.
+template <int S>
+static i64 run_stage(Ctx& ctx, const Csr& G, const int* rv1, const int* rv2, i64 R, i64 base, i64 budget,
+                     Triangles& T) {
+    const int* OFF = G.off.get();
+    DBuf<i64> xoff = ctx.buf<i64>(R + 1);
+    if (S == 3) {
+        iota(ctx, xoff.get(), R + 1);
+    } else if (S == 4) {
+        scan_exclusive<i64>(ctx, [=] __device__(i64 r) { const int v = rv1[r]; return (i64)(OFF[v + 1] - OFF[v]); },
+                            xoff.get(), R);
+    } else {
+        scan_exclusive<i64>(
+            ctx,
+            [=] __device__(i64 r) {
+                const int v = rv1[r], u = rv2[r];
+                return (i64)(OFF[v + 1] - OFF[v]) * (i64)(OFF[u + 1] - OFF[u]);
+            },
+            xoff.get(), R);
+    }
+    const i64 X = ctx.read1(xoff.get() + R);
+    if (ctx.verbose && S > 3) std::printf("%d-cycles: number of expansions: %lld\n", S, (long long)X);
+    if (X == 0) return 0;
+    const i64 ntiles = (X + kCycTile - 1) / kCycTile;
+    DBuf<int> tile_rep = ctx.buf<int>(ntiles);
+    k_tile_rep<<<grid_for(R), kBlock, 0, ctx.stream>>>(xoff.get(), R, tile_rep.get());
+    RAMA_LAUNCHED();
+    CycArgs a{OFF, G.adj.get(), rv1, rv2, xoff.get(), tile_rep.get(), R, X};
+    DBuf<unsigned char> cnt8 = ctx.buf<unsigned char>(X);
+    DBuf<i64> tile = ctx.buf<i64>(ntiles + 1);
+    RAMA_REQUIRE(ntiles < ((i64)1 << 31), "too many cycle expansions");
+    k_cycle_count<S><<<(unsigned)ntiles, kBlock, 0, ctx.stream>>>(a, cnt8.get(), tile.get());
+    RAMA_LAUNCHED();
+    scan_exclusive_arr<i64>(ctx, tile.get(), tile.get(), ntiles);
+    const i64 W = ctx.read1(tile.get() + ntiles);
+    if (W > 0 && base < budget) {
+        k_cycle_emit<S><<<(unsigned)ntiles, kBlock, 0, ctx.stream>>>(a, cnt8.get(), tile.get(), base, budget,
+                                                                     T.t1.get(), T.t2.get(), T.t3.get());
+        RAMA_LAUNCHED();
+    }
+    return W;
+}
+
+// rearrange_triangles / normalize_triangles (include/rama_utils.h:325-363)
+// on the first `valid` slots: drop triangles with a repeated vertex, sort
+// lexicographically, remove duplicates.  Returns the new count.
+static i64 rearrange(Ctx& ctx, Triangles& T, i64 valid, int n_nodes) {
+    if (valid <= 0) return 0;
+    int* A = T.t1.get();
+    int* B = T.t2.get();
+    int* C = T.t3.get();
+    DBuf<int> pos;
+    const i64 k = select_positions(
+        ctx, [=] __device__(i64 t) { return (int)(A[t] != B[t] && A[t] != C[t] && B[t] != C[t]); }, valid, pos);
+    if (k == 0) return 0;
+    const int kb = bits_for(n_nodes > 1 ? n_nodes - 1 : 1);
+    DBuf<u64> keys = ctx.buf<u64>(k);
+    DBuf<u32> vals = ctx.buf<u32>(k);
+    u64* K = keys.get();
+    u32* V = vals.get();
+    const int* P = pos.get();
+    // order the three ids (write_triangle already does; kept for foreign input)
+    DBuf<int> s1 = ctx.buf<int>(k), s2 = ctx.buf<int>(k), s3 = ctx.buf<int>(k);
+    int* S1 = s1.get();
+    int* S2 = s2.get();
+    int* S3 = s3.get();
+    for_each(ctx, valid, [=] __device__(i64 t) {
+        const int a = A[t], b = B[t], c = C[t];
+        if (a == b || a == c || b == c) return;
+        const i64 q = P[t];
+        S1[q] = min(a, min(b, c));
+        S2[q] = max(min(a, b), min(max(a, b), c));
+        S3[q] = max(a, max(b, c));
+    });
+    if (3 * kb <= 64) {
+        for_each(ctx, k, [=] __device__(i64 q) {
+            K[q] = ((u64)(u32)S1[q] << (2 * kb)) | ((u64)(u32)S2[q] << kb) | (u64)(u32)S3[q];
+            V[q] = (u32)q;
+        });
+        radix_sort_pairs(ctx, keys, vals, k, 3 * kb);
+    } else {
+        // two stable passes: by third id, then by (first, second)
+        for_each(ctx, k, [=] __device__(i64 q) {
+            K[q] = (u64)(u32)S3[q];
+            V[q] = (u32)q;
+        });
+        radix_sort_pairs(ctx, keys, vals, k, kb);
+        for_each(ctx, k, [=] __device__(i64 q) {
+            const u32 o = V[q];
+            K[q] = ((u64)(u32)S1[o] << kb) | (u64)(u32)S2[o];
+        });
+        radix_sort_pairs(ctx, keys, vals, k, 2 * kb);
+    }
+    DBuf<int> upos;
+    const i64 u = select_positions(
+        ctx,
+        [=] __device__(i64 q) {
+            if (q == 0) return 1;
+            const u32 a = V[q], b = V[q - 1];
+            return (int)(S1[a] != S1[b] || S2[a] != S2[b] || S3[a] != S3[b]);
+        },
+        k, upos);
+    const int* UP = upos.get();
+    for_each(ctx, k, [=] __device__(i64 q) {
+        const u32 a = V[q];
+        if (q == 0 || S1[a] != S1[V[q - 1]] || S2[a] != S2[V[q - 1]] || S3[a] != S3[V[q - 1]]) {
+            const i64 o = UP[q];
+            A[o] = S1[a];
+            B[o] = S2[a];
+            C[o] = S3[a];
+        }
+    });
+    return u;
+}
+
+void normalize_triangles(Ctx& ctx, Triangles& T) { T.n = rearrange(ctx, T, T.n, 1 << 30); }
+
+// conflicted_cycles_cuda (src/conflicted_cycles_cuda.cu:222-343), followed by
+// the normalisation the message-passing constructor applies.
+Triangles conflicted_cycles(Ctx& ctx, const DCoo& A, int max_len, float factor, float tol) {
+    Triangles T;
+    if (max_len > 5) throw Error(RAMA_ERR_INVALID, "max_cycle_length should be <= 5. Received: " + std::to_string(max_len));
+    if (max_len < 3 || A.nnz == 0) return T;
+    const float* W = A.val.get();
+    const float thr = dcoo_min(ctx, W, A.nnz) * tol;  // determine_threshold (:170-174)
+    const int N = A.max_dim();
+    // attractive: c > -thr (is_positive_edge); repulsive: c < thr (is_neg_edge)
+    Csr G = symmetric_csr(ctx, A.row.get(), A.col.get(), W, A.nnz, N, true, -1.0f * thr);
+    DBuf<int> rpos;
+    const i64 R = select_positions(ctx, [=] __device__(i64 e) { return (int)(W[e] < thr); }, A.nnz, rpos);
+    if (G.nnz == 0 || R == 0) return T;
+    DBuf<int> rv1 = ctx.buf<int>(R), rv2 = ctx.buf<int>(R);
+    {
+        int* X1 = rv1.get();
+        int* X2 = rv2.get();
+        const int* P = rpos.get();
+        const int* AR = A.row.get();
+        const int* AC = A.col.get();
+        for_each(ctx, A.nnz, [=] __device__(i64 e) {
+            if (W[e] < thr) {
+                X1[P[e]] = AR[e];
+                X2[P[e]] = AC[e];
+            }
+        });
+    }
+    const i64 budget = (i64)(int)(factor * (float)R);
+    T.t1 = ctx.buf<int>(budget > 0 ? budget : 1);
+    T.t2 = ctx.buf<int>(budget > 0 ? budget : 1);
+    T.t3 = ctx.buf<int>(budget > 0 ? budget : 1);
+    i64 count = run_stage<3>(ctx, G, rv1.get(), rv2.get(), R, 0, budget, T);
+    if (ctx.verbose) std::printf("3-cycles: found # of triangles: %lld, budget: %lld\n", (long long)count, (long long)budget);
+    if (max_len >= 4 && count < budget) {
+        count = rearrange(ctx, T, count, N);
+        count += run_stage<4>(ctx, G, rv1.get(), rv2.get(), R, count, budget, T);
+        if (ctx.verbose) std::printf("4-cycles: found # of triangles: %lld, budget: %lld\n", (long long)count, (long long)budget);
+    }
+    if (max_len >= 5 && count < budget) {
+        count = rearrange(ctx, T, count, N);
+        count += run_stage<5>(ctx, G, rv1.get(), rv2.get(), R, count, budget, T);
+        if (ctx.verbose) std::printf("5-cycles: found # of triangles: %lld, budget: %lld\n", (long long)count, (long long)budget);
+    }
+    T.raw = count < budget ? count : budget;
+    T.n = rearrange(ctx, T, T.raw, N);
+    return T;
+}
+
+}  // namespace rama

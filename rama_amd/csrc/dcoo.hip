@@ -1,0 +1,385 @@
+// dcoo.hip — the dCOO edge store: construction, contraction (relabel, sort,
+// segmented reduce-by-key, diagonal check and removal), symmetric CSR,
+// offsets, label compression and on-device sanitising.
+//
+// Reference: include/dCOO.h, src/dCOO.cu, include/rama_utils.h.
+//
+// Layout in HBM: SoA int32 row / int32 col / fp32 val, sorted by (row, col).
+// Sorting packs (row, col) into one u64 key of 2*ceil(log2 N) bits and
+// carries the fp32 cost bit pattern as the u32 payload, so a contraction
+// moves 12 B per edge per radix pass.
+#include "rama_internal.hpp"
+
+namespace rama {
+
+__device__ __forceinline__ u32 f2u(float f) { return __float_as_uint(f); }
+__device__ __forceinline__ float u2f(u32 u) { return __uint_as_float(u); }
+
+int max_id(Ctx& ctx, const int* a, const int* b, i64 n) {
+    if (n <= 0) return -1;
+    return reduce<int>(
+        ctx, [=] __device__(i64 e) { return a[e] > b[e] ? a[e] : b[e]; }, n, -1, OpMax());
+}
+
+float dcoo_min(Ctx& ctx, const float* val, i64 n) {
+    return reduce<float>(ctx, [=] __device__(i64 e) { return val[e]; }, n, 3.402823466e38f, OpMin());
+}
+
+// get_lb (include/rama_utils.h:179-190): sum of negative parts in double.
+double dcoo_lb(Ctx& ctx, const float* val, i64 n) {
+    if (n <= 0) return 0.0;
+    return reduce<double>(
+        ctx, [=] __device__(i64 e) { const float x = val[e]; return x < 0.0f ? (double)x : 0.0; }, n, 0.0,
+        OpSum());
+}
+
+// Writes rows = last row + 1 and cols = max col + 1 (src/dCOO.cu:46-51).
+static void derive_dims(Ctx& ctx, DCoo& A) {
+    if (A.nnz == 0) {
+        A.rows = A.cols = 0;
+        return;
+    }
+    const int* col = A.col.get();
+    DBuf<int> tmp = ctx.buf<int>(2);
+    reduce_to<int>(ctx, [=] __device__(i64 e) { return col[e]; }, A.nnz, -1, OpMax(), tmp.get());
+    ctx.d2d(tmp.get() + 1, A.row.get() + A.nnz - 1, 1);
+    int h[2];
+    ctx.read(tmp.get(), h, 2);
+    A.cols = h[0] + 1;
+    A.rows = h[1] + 1;
+}
+
+// dCOO(col, row, data, directed=true) + init (src/dCOO.cu:28-55): orient
+// each edge (min, max) and stable-sort by (row, col).  No deduplication.
+DCoo dcoo_from_edges(Ctx& ctx, const int* row_in, const int* col_in, const float* val_in, i64 m) {
+    DCoo A;
+    A.nnz = m;
+    A.row = ctx.buf<int>(m);
+    A.col = ctx.buf<int>(m);
+    A.val = ctx.buf<float>(m);
+    if (m == 0) return A;
+    const int mx = max_id(ctx, row_in, col_in, m);
+    RAMA_REQUIRE(mx >= 0, "node ids must be non-negative");
+    const int kb = bits_for(mx);
+    DBuf<u64> keys = ctx.buf<u64>(m);
+    DBuf<u32> vals = ctx.buf<u32>(m);
+    u64* K = keys.get();
+    u32* V = vals.get();
+    DBuf<int> bad = ctx.buf<int>(1);
+    ctx.zero(bad.get(), 1);
+    int* badp = bad.get();
+    for_each(ctx, m, [=] __device__(i64 e) {
+        const int a = row_in[e], b = col_in[e];
+        if (a < 0 || b < 0) badp[0] = 1;
+        const u32 lo = (u32)(a < b ? a : b), hi = (u32)(a < b ? b : a);
+        K[e] = ((u64)lo << kb) | hi;
+        V[e] = f2u(val_in[e]);
+    });
+    RAMA_REQUIRE(ctx.read1(badp) == 0, "node ids must be non-negative");
+    radix_sort_pairs(ctx, keys, vals, m, 2 * kb);
+    int* R = A.row.get();
+    int* C = A.col.get();
+    float* W = A.val.get();
+    const u64 mask = ((u64)1 << kb) - 1;
+    for_each(ctx, m, [=] __device__(i64 e) {
+        const u64 k = K[e];
+        R[e] = (int)(k >> kb);
+        C[e] = (int)(k & mask);
+        W[e] = u2f(V[e]);
+    });
+    derive_dims(ctx, A);
+    return A;
+}
+
+// dCOO(j, i, costs, directed, is_sorted=true) (src/dual_solver.cu:32-33).
+DCoo dcoo_from_sorted(Ctx& ctx, DBuf<int>&& row, DBuf<int>&& col, DBuf<float>&& val, i64 m) {
+    DCoo A;
+    A.row = std::move(row);
+    A.col = std::move(col);
+    A.val = std::move(val);
+    A.nnz = m;
+    derive_dims(ctx, A);
+    return A;
+}
+
+DCoo dcoo_copy(Ctx& ctx, const DCoo& A) {
+    DCoo B;
+    B.nnz = A.nnz;
+    B.rows = A.rows;
+    B.cols = A.cols;
+    B.row = ctx.buf<int>(A.nnz);
+    B.col = ctx.buf<int>(A.nnz);
+    B.val = ctx.buf<float>(A.nnz);
+    ctx.d2d(B.row.get(), A.row.get(), A.nnz);
+    ctx.d2d(B.col.get(), A.col.get(), A.nnz);
+    ctx.d2d(B.val.get(), A.val.get(), A.nnz);
+    return B;
+}
+
+// Per-segment reduction of the sorted contraction keys: one thread per
+// (row, col) segment sums its costs in double in sorted order and rounds
+// once; self-loops are flagged (negative -> bad contraction) and dropped.
+__global__ void k_contract_reduce(const u64* __restrict__ K, const u32* __restrict__ V, const int* __restrict__ head_pos,
+                                  const int* __restrict__ seg_start, i64 S, i64 n, int kb, u64* __restrict__ okey,
+                                  float* __restrict__ oval, int* __restrict__ keep, int* bad, double* diag) {
+    const i64 s = (i64)blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= S) return;
+    const i64 b = seg_start[s];
+    const i64 e = s + 1 < S ? seg_start[s + 1] : n;
+    double acc = 0.0;
+    for (i64 k = b; k < e; ++k) acc += (double)__uint_as_float(V[k]);
+    const u64 key = K[b];
+    const u64 mask = ((u64)1 << kb) - 1;
+    const int r = (int)(key >> kb), c = (int)(key & mask);
+    const float f = (float)acc;
+    okey[s] = key;
+    oval[s] = f;
+    keep[s] = r != c;
+    if (r == c) {
+        if (f < 0.0f) *bad = 1;
+        atomicAdd(diag, (double)f);
+    }
+}
+
+// dCOO::contract_cuda + has_bad_contractions + remove_diagonal
+// (src/dCOO.cu:57-95,97-142; src/rama_cuda.cu:18-30,136-140).
+DCoo contract(Ctx& ctx, const DCoo& A, const int* node_map, int max_label, bool* bad, double* diag_sum) {
+    const i64 n = A.nnz;
+    DCoo B;
+    if (n == 0) {
+        *bad = false;
+        if (diag_sum) *diag_sum = 0.0;
+        return B;
+    }
+    const int kb = bits_for(max_label < 1 ? 1 : max_label);
+    DBuf<u64> keys = ctx.buf<u64>(n);
+    DBuf<u32> vals = ctx.buf<u32>(n);
+    u64* K = keys.get();
+    u32* V = vals.get();
+    const int* R = A.row.get();
+    const int* C = A.col.get();
+    const float* W = A.val.get();
+    for_each(ctx, n, [=] __device__(i64 e) {
+        const int a = node_map[R[e]], b = node_map[C[e]];
+        const u32 lo = (u32)(a < b ? a : b), hi = (u32)(a < b ? b : a);
+        K[e] = ((u64)lo << kb) | hi;
+        V[e] = f2u(W[e]);
+    });
+    radix_sort_pairs(ctx, keys, vals, n, 2 * kb);
+
+    DBuf<int> head_pos;
+    const i64 S = select_positions(
+        ctx, [=] __device__(i64 e) { return (int)(e == 0 || K[e] != K[e - 1]); }, n, head_pos);
+    DBuf<int> seg_start = ctx.buf<int>(S + 1);
+    int* SS = seg_start.get();
+    const int* HP = head_pos.get();
+    for_each(ctx, n, [=] __device__(i64 e) {
+        if (e == 0 || K[e] != K[e - 1]) SS[HP[e]] = (int)e;
+    });
+    DBuf<u64> okey = ctx.buf<u64>(S);
+    DBuf<float> oval = ctx.buf<float>(S);
+    DBuf<int> keep = ctx.buf<int>(S);
+    DBuf<double> scal = ctx.buf<double>(2);
+    ctx.zero(scal.get(), 2);
+    int* badp = reinterpret_cast<int*>(scal.get() + 1);
+    k_contract_reduce<<<grid_for(S), kBlock, 0, ctx.stream>>>(K, V, HP, SS, S, n, kb, okey.get(), oval.get(),
+                                                                keep.get(), badp, scal.get());
+    RAMA_LAUNCHED();
+    // dims before removing the diagonal (src/dCOO.cu:91-92)
+    const u64 mask = ((u64)1 << kb) - 1;
+    const u64* OK = okey.get();
+    DBuf<int> dims = ctx.buf<int>(2);
+    reduce_to<int>(ctx, [=] __device__(i64 s) { return (int)(OK[s] & mask); }, S, -1, OpMax(), dims.get());
+    int* dp = dims.get();
+    for_each(ctx, 1, [=] __device__(i64) { dp[1] = (int)(OK[S - 1] >> kb); });
+
+    DBuf<int> kpos;
+    const int* KP = keep.get();
+    const i64 S2 = select_positions(ctx, [=] __device__(i64 s) { return KP[s]; }, S, kpos);
+    B.nnz = S2;
+    B.row = ctx.buf<int>(S2);
+    B.col = ctx.buf<int>(S2);
+    B.val = ctx.buf<float>(S2);
+    int* BR = B.row.get();
+    int* BC = B.col.get();
+    float* BW = B.val.get();
+    const int* P = kpos.get();
+    const float* OV = oval.get();
+    for_each(ctx, S, [=] __device__(i64 s) {
+        if (KP[s]) {
+            const int q = P[s];
+            BR[q] = (int)(OK[s] >> kb);
+            BC[q] = (int)(OK[s] & mask);
+            BW[q] = OV[s];
+        }
+    });
+    int h[2];
+    ctx.read(dims.get(), h, 2);
+    B.cols = h[0] + 1;
+    B.rows = h[1] + 1;
+    double sc[2];
+    ctx.read(scal.get(), sc, 2);
+    int badv;
+    std::memcpy(&badv, &sc[1], sizeof(int));
+    *bad = badv != 0;
+    if (diag_sum) *diag_sum = sc[0];
+    return B;
+}
+
+// compute_offsets (include/rama_utils.h:403-413): out[v] = #{ids < v} for
+// v in [0, max_value+1]; one thread per boundary of the sorted ids.
+template <class RowOf>
+__global__ void k_offsets(RowOf rowof, i64 n, int end_value, int* out) {
+    const i64 i = (i64)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i > n) return;
+    const int prev = i > 0 ? rowof(i - 1) : -1;
+    const int cur = i < n ? rowof(i) : end_value;
+    for (int r = prev + 1; r <= cur && r <= end_value; ++r) out[r] = (int)i;
+}
+
+void compute_offsets(Ctx& ctx, const int* sorted_ids, i64 n, int max_value, int* out) {
+    auto f = [=] __device__(i64 i) { return sorted_ids[i]; };
+    k_offsets<<<grid_for(n + 1), kBlock, 0, ctx.stream>>>(f, n, max_value + 1, out);
+    RAMA_LAUNCHED();
+}
+
+// to_undirected + dCOO(symmetric) + compute_row_offsets
+// (include/rama_utils.h:107-131, src/dCOO.cu:144-147,164-172,
+//  src/conflicted_cycles_cuda.cu:208-217).  With `filter`, only edges with
+// cost > keep_above enter (is_positive_edge).
+Csr symmetric_csr(Ctx& ctx, const int* row, const int* col, const float* val, i64 m, int n_nodes, bool filter,
+                  float keep_above) {
+    Csr G;
+    G.n = n_nodes;
+    DBuf<int> pos;
+    i64 P = m;
+    if (filter) {
+        P = select_positions(ctx, [=] __device__(i64 e) { return (int)(val[e] > keep_above); }, m, pos);
+    }
+    const i64 N2 = 2 * P;
+    G.nnz = N2;
+    G.off = ctx.buf<int>((i64)n_nodes + 1);
+    G.adj = ctx.buf<int>(N2);
+    G.w = ctx.buf<float>(N2);
+    if (N2 == 0) {
+        ctx.zero(G.off.get(), (i64)n_nodes + 1);
+        return G;
+    }
+    const int kb = bits_for(n_nodes > 1 ? n_nodes - 1 : 1);
+    DBuf<u64> keys = ctx.buf<u64>(N2);
+    DBuf<u32> vals = ctx.buf<u32>(N2);
+    u64* K = keys.get();
+    u32* V = vals.get();
+    const int* PP = filter ? pos.get() : nullptr;
+    for_each(ctx, m, [=] __device__(i64 e) {
+        if (PP && !(val[e] > keep_above)) return;
+        const i64 q = PP ? PP[e] : e;
+        const u32 a = (u32)row[e], b = (u32)col[e];
+        const u32 v = f2u(val[e]);
+        K[q] = ((u64)a << kb) | b;
+        V[q] = v;
+        K[P + q] = ((u64)b << kb) | a;
+        V[P + q] = v;
+    });
+    radix_sort_pairs(ctx, keys, vals, N2, 2 * kb);
+    int* ADJ = G.adj.get();
+    float* WW = G.w.get();
+    const u64 mask = ((u64)1 << kb) - 1;
+    for_each(ctx, N2, [=] __device__(i64 e) {
+        ADJ[e] = (int)(K[e] & mask);
+        WW[e] = u2f(V[e]);
+    });
+    auto rowof = [=] __device__(i64 i) { return (int)(K[i] >> kb); };
+    k_offsets<<<grid_for(N2 + 1), kBlock, 0, ctx.stream>>>(rowof, N2, n_nodes, G.off.get());
+    RAMA_LAUNCHED();
+    return G;
+}
+
+// compress_label_sequence (include/rama_utils.h:160-177); returns the number
+// of distinct labels.
+int compress_labels(Ctx& ctx, const int* data, i64 n, int max_label, int* out) {
+    DBuf<int> mask = ctx.buf<int>((i64)max_label + 2);
+    ctx.zero(mask.get(), (i64)max_label + 2);
+    int* M = mask.get();
+    for_each(ctx, n, [=] __device__(i64 k) { M[data[k]] = 1; });
+    scan_exclusive_arr<int>(ctx, M, M, (i64)max_label + 1);
+    for_each(ctx, n, [=] __device__(i64 k) { out[k] = M[data[k]]; });
+    return ctx.read1(M + max_label + 1);
+}
+
+// map_node_labels (src/rama_cuda.cu:32-52)
+void map_node_labels(Ctx& ctx, const int* cur, i64 cur_size, int* orig, i64 orig_size) {
+    for_each(ctx, orig_size, [=] __device__(i64 n) {
+        const int m = orig[n];
+        if (m < cur_size) orig[n] = cur[m];
+    });
+}
+
+// compute_sanitized_graph (include/rama_utils.h:429-466): orient, sort,
+// drop repeated edges (first kept), renumber used node ids consecutively.
+Sanitized sanitize_graph(Ctx& ctx, DBuf<int>& i, DBuf<int>& j, DBuf<float>& c, i64& m) {
+    Sanitized s;
+    if (m == 0) return s;
+    const int mx = max_id(ctx, i.get(), j.get(), m);
+    const int kb = bits_for(mx);
+    DBuf<u64> keys = ctx.buf<u64>(m);
+    DBuf<u32> vals = ctx.buf<u32>(m);
+    u64* K = keys.get();
+    u32* V = vals.get();
+    const int* I = i.get();
+    const int* J = j.get();
+    const float* Cc = c.get();
+    for_each(ctx, m, [=] __device__(i64 e) {
+        const u32 a = (u32)I[e], b = (u32)J[e];
+        K[e] = ((u64)(a < b ? a : b) << kb) | (a < b ? b : a);
+        V[e] = f2u(Cc[e]);
+    });
+    radix_sort_pairs(ctx, keys, vals, m, 2 * kb);
+    DBuf<int> pos;
+    const i64 m2 = select_positions(
+        ctx, [=] __device__(i64 e) { return (int)(e == 0 || K[e] != K[e - 1]); }, m, pos);
+    DBuf<int> ni = ctx.buf<int>(m2), nj = ctx.buf<int>(m2);
+    DBuf<float> nc = ctx.buf<float>(m2);
+    s.size = (i64)mx + 1;
+    s.ids = ctx.buf<int>((i64)mx + 2);
+    ctx.zero(s.ids.get(), (i64)mx + 2);
+    int* NI = ni.get();
+    int* NJ = nj.get();
+    float* NC = nc.get();
+    int* IDS = s.ids.get();
+    const int* PP = pos.get();
+    const u64 mask = ((u64)1 << kb) - 1;
+    for_each(ctx, m, [=] __device__(i64 e) {
+        if (e == 0 || K[e] != K[e - 1]) {
+            const int q = PP[e];
+            const int a = (int)(K[e] >> kb), b = (int)(K[e] & mask);
+            NI[q] = a;
+            NJ[q] = b;
+            NC[q] = u2f(V[e]);
+            IDS[a] = 1;
+            IDS[b] = 1;
+        }
+    });
+    scan_exclusive_arr<int>(ctx, IDS, IDS, (i64)mx + 1);
+    for_each(ctx, m2, [=] __device__(i64 e) {
+        NI[e] = IDS[NI[e]];
+        NJ[e] = IDS[NJ[e]];
+    });
+    i = std::move(ni);
+    j = std::move(nj);
+    c = std::move(nc);
+    m = m2;
+    return s;
+}
+
+// desanitize_node_labels (include/rama_utils.h:468-493); out has s.size entries.
+void desanitize_labels(Ctx& ctx, const Sanitized& s, const int* labels, int* out) {
+    const int* IDS = s.ids.get();
+    const i64 sz = s.size;
+    for_each(ctx, sz, [=] __device__(i64 n) {
+        const int cur = IDS[n];
+        out[n] = (n == sz - 1 || cur != IDS[n + 1]) ? labels[cur] : -1;
+    });
+}
+
+}  // namespace rama

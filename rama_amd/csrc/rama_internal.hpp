@@ -1,0 +1,126 @@
+// rama_internal.hpp — device-resident data structures and the stage
+// functions that make up the solver.  All buffers live in HBM on the
+// context's device; host code only sees sizes and a handful of scalars.
+#pragma once
+
+#include "prims.cuh"
+
+namespace rama {
+
+// Directed multicut graph in COO form, sorted by (row, col) with row < col —
+// the reference's dCOO (include/dCOO.h:13-71).  rows/cols follow the
+// reference's rows_/cols_ bookkeeping exactly because the driver's decisions
+// (matching->MST crossover, label-map sizes) depend on them.
+struct DCoo {
+    DBuf<int> row, col;
+    DBuf<float> val;
+    i64 nnz = 0;
+    int rows = 0, cols = 0;
+    int max_dim() const { return rows > cols ? rows : cols; }
+};
+
+// Symmetric adjacency in CSR form (both directions, sorted by (row, col)).
+struct Csr {
+    DBuf<int> off;  // n + 1
+    DBuf<int> adj;  // nnz
+    DBuf<float> w;  // nnz
+    int n = 0;
+    i64 nnz = 0;
+};
+
+// Solver options as seen by the device code.
+struct Options {
+    int max_cycle_length_lb = 5;
+    int num_dual_itr_lb = 10;
+    int max_cycle_length_primal = 3;
+    int num_dual_itr_primal = 5;
+    int num_outer_itr_dual = 1;
+    float mean_multiplier_mm = 0.05f;
+    float matching_thresh_crossover_ratio = 0.05f;
+    float tri_memory_factor = 2.0f;
+    bool only_compute_lb = false;
+    int max_time_sec = -1;
+    bool dump_timeline = false;
+    bool verbose = true;
+    bool sanitize_graph = false;
+};
+Options to_options(const rama_options_c& o);
+
+// ---- dcoo.hip
+DCoo dcoo_from_edges(Ctx& ctx, const int* row_in, const int* col_in, const float* val_in, i64 m);
+DCoo dcoo_from_sorted(Ctx& ctx, DBuf<int>&& row, DBuf<int>&& col, DBuf<float>&& val, i64 m);
+DCoo dcoo_copy(Ctx& ctx, const DCoo& A);
+DCoo contract(Ctx& ctx, const DCoo& A, const int* node_map, int max_label, bool* bad, double* diag_sum);
+Csr symmetric_csr(Ctx& ctx, const int* row, const int* col, const float* val, i64 m, int n_nodes,
+                  bool filter, float keep_above);
+void compute_offsets(Ctx& ctx, const int* sorted_ids, i64 n, int max_value, int* out);
+int compress_labels(Ctx& ctx, const int* data, i64 n, int max_label, int* out);
+void map_node_labels(Ctx& ctx, const int* cur, i64 cur_size, int* orig, i64 orig_size);
+double dcoo_lb(Ctx& ctx, const float* val, i64 n);
+float dcoo_min(Ctx& ctx, const float* val, i64 n);
+int max_id(Ctx& ctx, const int* a, const int* b, i64 n);
+struct Sanitized {
+    DBuf<int> ids;  // original id -> compact id (exclusive scan of "used")
+    i64 size = 0;
+};
+Sanitized sanitize_graph(Ctx& ctx, DBuf<int>& i, DBuf<int>& j, DBuf<float>& c, i64& m);
+void desanitize_labels(Ctx& ctx, const Sanitized& s, const int* labels, int* out);
+
+// ---- matching.hip
+// export_undirected + vertex matching; returns the (uncompressed) mapping of
+// size U.n and the number of matched vertices.
+int matching_vertex_based(Ctx& ctx, const Csr& U, float mean_multiplier, DBuf<int>& map);
+
+// ---- cycles.hip
+struct Triangles {
+    DBuf<int> t1, t2, t3;
+    i64 n = 0;    // after normalisation
+    i64 raw = 0;  // slots filled by the search (reference triangles_v1.size())
+};
+Triangles conflicted_cycles(Ctx& ctx, const DCoo& A, int max_cycle_length, float tri_memory_factor, float tol_ratio);
+void normalize_triangles(Ctx& ctx, Triangles& T);
+
+// ---- mp.hip
+struct MessagePassing;  // opaque
+struct MPState {
+    DBuf<int> ei, ej;            // message-passing edges (sorted, unique)
+    DBuf<float> ec;              // edge costs
+    DBuf<int> ecnt;              // #triangles per edge
+    DBuf<int> soff;              // edge -> slot list offsets (E+1)
+    DBuf<int> slots;             // slot ids 3t+k in (triangle, slot) order
+    DBuf<int> c12, c13, c23;     // triangle -> edge index
+    DBuf<float> t12, t13, t23;   // triangle costs
+    DBuf<float> delta;           // per-slot messages (3T)
+    DBuf<double> lbparts;        // partial lower bounds
+    i64 E = 0, T = 0;
+};
+void mp_build(Ctx& ctx, const DCoo& A, Triangles& tri, MPState& S);
+double mp_lower_bound(Ctx& ctx, MPState& S);
+double mp_fused_lower_bound(Ctx& ctx, MPState& S);
+void mp_send_to_triplets(Ctx& ctx, MPState& S);
+void mp_iteration(Ctx& ctx, MPState& S);
+DCoo mp_reparametrized(Ctx& ctx, MPState& S);
+
+// ---- dual.hip
+double dual_solver(Ctx& ctx, DCoo& A, int max_cycle_length, int num_iter, float tri_memory_factor, int num_outer,
+                   float tol_ratio);
+
+// ---- forest.hip
+void connected_components(Ctx& ctx, const int* off, const int* adj, int n, int* labels);
+// Forest edge indices (into the given edge list) of the maximum spanning
+// forest under (cost desc, index asc); returns the count, ascending order.
+i64 max_spanning_forest(Ctx& ctx, const int* r, const int* c, const float* w, i64 m, int n, DBuf<int>& ids);
+// edge_contractions_woc(A).find_contraction_mapping(); returns #forest entries.
+int woc_mapping(Ctx& ctx, const DCoo& A, DBuf<int>& map, i64& map_size);
+
+// ---- solver.hip
+struct SolveResult {
+    DBuf<int> labels;
+    i64 num_labels = 0;
+    double lb = 0.0;
+    int iterations = 0;
+    std::vector<std::vector<int>> timeline;
+};
+SolveResult solve(Ctx& ctx, DCoo& A, const Options& o);
+
+}  // namespace rama

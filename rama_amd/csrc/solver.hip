@@ -1,0 +1,98 @@
+// solver.hip — the primal-dual driver (src/rama_cuda.cu:54-164): initial
+// dual pass for the lower bound, then contraction rounds (re-solve a small
+// dual, pick contractions by matching or by the conflict-free spanning
+// forest, contract, compose labels) until nothing is left to contract.
+#include <chrono>
+
+#include "rama_internal.hpp"
+
+namespace rama {
+
+Options to_options(const rama_options_c& c) {
+    Options o;
+    o.max_cycle_length_lb = c.max_cycle_length_lb;
+    o.num_dual_itr_lb = c.num_dual_itr_lb;
+    o.max_cycle_length_primal = c.max_cycle_length_primal;
+    o.num_dual_itr_primal = c.num_dual_itr_primal;
+    o.num_outer_itr_dual = c.num_outer_itr_dual;
+    o.mean_multiplier_mm = c.mean_multiplier_mm;
+    o.matching_thresh_crossover_ratio = c.matching_thresh_crossover_ratio;
+    o.tri_memory_factor = c.tri_memory_factor;
+    o.only_compute_lb = c.only_compute_lb != 0;
+    o.max_time_sec = c.max_time_sec;
+    o.dump_timeline = c.dump_timeline != 0;
+    o.verbose = c.verbose != 0;
+    o.sanitize_graph = c.sanitize_graph != 0;
+    return o;
+}
+
+SolveResult solve(Ctx& ctx, DCoo& A, const Options& o) {
+    SolveResult out;
+    const auto t0 = std::chrono::steady_clock::now();
+    out.lb = dual_solver(ctx, A, o.max_cycle_length_lb, o.num_dual_itr_lb, o.tri_memory_factor, o.num_outer_itr_dual,
+                         1e-4f);
+    const int N = A.max_dim();
+    out.labels = ctx.buf<int>(N);
+    iota(ctx, out.labels.get(), N);
+    out.num_labels = N;
+    if (o.only_compute_lb) {
+        out.num_labels = 0;
+        return out;
+    }
+    bool use_mm = !(o.matching_thresh_crossover_ratio > 1.0f);
+    for (int iter = 0; A.nnz > 0; ++iter) {
+        if (iter > 0)
+            dual_solver(ctx, A, o.max_cycle_length_primal, o.num_dual_itr_primal, 1.0f, 1, 1e-4f);
+        DBuf<int> cur;
+        i64 cur_size = 0;
+        int n_contract = 0;
+        int max_label = 0;
+        if (use_mm) {
+            // export_undirected: node count = largest endpoint + 1
+            const int* C = A.col.get();
+            const int Un = reduce<int>(ctx, [=] __device__(i64 e) { return C[e]; }, A.nnz, -1, OpMax()) + 1;
+            Csr U = symmetric_csr(ctx, A.row.get(), A.col.get(), A.val.get(), A.nnz, Un, false, 0.0f);
+            DBuf<int> m;
+            n_contract = matching_vertex_based(ctx, U, o.mean_multiplier_mm, m);
+            cur = ctx.buf<int>(Un);
+            max_label = compress_labels(ctx, m.get(), Un, Un - 1, cur.get()) - 1;
+            cur_size = Un;
+            if ((float)n_contract < (float)A.rows * o.matching_thresh_crossover_ratio) {
+                if (o.verbose)
+                    std::printf("# edges to contract = %d, # vertices = %d\nswitching to MST based contraction edge selection\n",
+                                n_contract, A.rows);
+                use_mm = false;
+            }
+        } else {
+            n_contract = woc_mapping(ctx, A, cur, cur_size);
+            max_label = (int)cur_size - 1;
+        }
+        if (n_contract == 0) {
+            if (o.verbose) std::printf("# iterations = %d\n", iter);
+            break;
+        }
+        bool bad = false;
+        double diag = 0.0;
+        DCoo B = contract(ctx, A, cur.get(), max_label, &bad, &diag);
+        if (o.verbose) {
+            std::printf("original A size %dx%d\ncontracted A size %dx%d\nenergy reduction %g\n", A.cols, A.rows, B.cols,
+                        B.rows, diag);
+        }
+        if (bad) throw Error(RAMA_ERR_BAD_CONTRACTION, "Found bad contractions");
+        A = std::move(B);
+        map_node_labels(ctx, cur.get(), cur_size, out.labels.get(), N);
+        out.iterations = iter + 1;
+        if (o.dump_timeline) {
+            std::vector<int> snap(N);
+            ctx.d2h(snap.data(), out.labels.get(), N);
+            out.timeline.push_back(std::move(snap));
+        }
+        if (o.max_time_sec >= 0) {
+            const auto t = std::chrono::duration_cast<std::chrono::seconds>(std::chrono::steady_clock::now() - t0).count();
+            if (t > o.max_time_sec) break;
+        }
+    }
+    return out;
+}
+
+}  // namespace rama
