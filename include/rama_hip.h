@@ -193,6 +193,18 @@ int rama_stage_woc_mapping(const int32_t* row, const int32_t* col, const float* 
                            int32_t rows, int32_t cols,
                            int32_t* out_map, int64_t* out_map_size, int32_t* out_count);
 
+/* ------------------------------------------------------------------------
+ * Kernel timing (no counterpart in the reference; its cumulative wall-clock
+ * macros, include/time_measure_util.h:11-70, are the closest).  When enabled,
+ * instrumented launches are bracketed by HIP events on the solver's stream
+ * and aggregated per kernel name with the algorithmic bytes each launch must
+ * move.  Entries stay valid until the next reset.
+ * ------------------------------------------------------------------------ */
+void rama_profile_enable(int on);
+void rama_profile_reset(void);
+int64_t rama_profile_count(void);
+int rama_profile_entry(int64_t k, const char** name, int64_t* launches, double* total_ms, double* total_bytes);
+
 #ifdef __cplusplus
 }
 #endif

@@ -73,6 +73,10 @@ SIGNATURES = {
     "rama_stage_connected_components": ([I32P, I32P, I32, I64, I32P], I32),
     "rama_stage_max_spanning_forest": ([I32P, I32P, F32P, I64, I32P, I32P, F32P, I64P], I32),
     "rama_stage_woc_mapping": ([I32P, I32P, F32P, I64, I32, I32, I32P, I64P, I32P], I32),
+    "rama_profile_enable": ([I32], None),
+    "rama_profile_reset": ([], None),
+    "rama_profile_count": ([], I64),
+    "rama_profile_entry": ([I64, ctypes.POINTER(ctypes.c_char_p), I64P, F64P, F64P], I32),
 }
 
 _lib = None

@@ -203,11 +203,15 @@ static i64 run_stage(Ctx& ctx, const Csr& G, const int* rv1, const int* rv2, i64
     DBuf<unsigned char> cnt8 = ctx.buf<unsigned char>(X);
     DBuf<i64> tile = ctx.buf<i64>(ntiles + 1);
     RAMA_REQUIRE(ntiles < ((i64)1 << 31), "too many cycle expansions");
-    k_cycle_count<S><<<(unsigned)ntiles, kBlock, 0, ctx.stream>>>(a, cnt8.get(), tile.get());
-    RAMA_LAUNCHED();
+    {
+        prof::Scope ps(ctx, S == 3 ? "cycles3_count" : (S == 4 ? "cycles4_count" : "cycles5_count"), 0.0);
+        k_cycle_count<S><<<(unsigned)ntiles, kBlock, 0, ctx.stream>>>(a, cnt8.get(), tile.get());
+        RAMA_LAUNCHED();
+    }
     scan_exclusive_arr<i64>(ctx, tile.get(), tile.get(), ntiles);
     const i64 W = ctx.read1(tile.get() + ntiles);
     if (W > 0 && base < budget) {
+        prof::Scope ps(ctx, S == 3 ? "cycles3_emit" : (S == 4 ? "cycles4_emit" : "cycles5_emit"), 0.0);
         k_cycle_emit<S><<<(unsigned)ntiles, kBlock, 0, ctx.stream>>>(a, cnt8.get(), tile.get(), base, budget,
                                                                      T.t1.get(), T.t2.get(), T.t3.get());
         RAMA_LAUNCHED();

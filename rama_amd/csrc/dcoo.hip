@@ -182,9 +182,13 @@ DCoo contract(Ctx& ctx, const DCoo& A, const int* node_map, int max_label, bool*
     DBuf<double> scal = ctx.buf<double>(2);
     ctx.zero(scal.get(), 2);
     int* badp = reinterpret_cast<int*>(scal.get() + 1);
-    k_contract_reduce<<<grid_for(S), kBlock, 0, ctx.stream>>>(K, V, HP, SS, S, n, kb, okey.get(), oval.get(),
-                                                                keep.get(), badp, scal.get());
-    RAMA_LAUNCHED();
+    {
+        // 12 B in per sorted edge; seg_start in + key, cost, keep flag out per segment
+        prof::Scope ps(ctx, "contract_reduce", 12.0 * n + 20.0 * S);
+        k_contract_reduce<<<grid_for(S), kBlock, 0, ctx.stream>>>(K, V, HP, SS, S, n, kb, okey.get(), oval.get(),
+                                                                    keep.get(), badp, scal.get());
+        RAMA_LAUNCHED();
+    }
     // dims before removing the diagonal (src/dCOO.cu:91-92)
     const u64 mask = ((u64)1 << kb) - 1;
     const u64* OK = okey.get();

@@ -104,9 +104,13 @@ int matching_vertex_based(Ctx& ctx, const Csr& U, float mean_multiplier, DBuf<in
     for (int t = 0; t < 10; ++t) {
         ctx.zero(scal.get() + 1, 1);
         const i64 threads = (i64)n * kGroup;
-        k_pick_best<<<grid_for(threads), kBlock, 0, ctx.stream>>>(n, thr, U.off.get(), U.adj.get(), U.w.get(),
-                                                                  matched.get(), best.get());
-        RAMA_LAUNCHED();
+        {
+            // offsets + (neighbour, cost, neighbour's matched flag) per CSR entry + best per vertex
+            prof::Scope ps(ctx, "match_pick_best", 12.0 * U.nnz + 12.0 * n);
+            k_pick_best<<<grid_for(threads), kBlock, 0, ctx.stream>>>(n, thr, U.off.get(), U.adj.get(), U.w.get(),
+                                                                      matched.get(), best.get());
+            RAMA_LAUNCHED();
+        }
         k_match<<<grid_for(n), kBlock, 0, ctx.stream>>>(n, best.get(), matched.get(), map.get(), scal.get() + 1);
         RAMA_LAUNCHED();
         reduce_to<int>(ctx, [=] __device__(i64 v) { return M[v]; }, n, 0, OpSum(), scal.get());

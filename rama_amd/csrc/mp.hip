@@ -240,6 +240,9 @@ static void launch_round(Ctx& ctx, MPState& S, bool sweep) {
     const i64 T = S.T, E = S.E;
     const unsigned gt = grid_for(T), ge = grid_for(E);
     double* P = S.lbparts.get();
+    // per triangle: 3 edge ids, 3 x (cost, count) gathers, 3 costs in and
+    // out, 3 messages out = 72 B
+    prof::Scope ps1(ctx, sweep ? "mp_triangles" : "mp_triangles_absorb", 72.0 * T);
     if (sweep)
         k_mp_triangles<true><<<gt, kBlock, 0, ctx.stream>>>(T, S.c12.get(), S.c13.get(), S.c23.get(), S.ec.get(),
                                                              S.ecnt.get(), S.t12.get(), S.t13.get(), S.t23.get(),
@@ -249,6 +252,8 @@ static void launch_round(Ctx& ctx, MPState& S, bool sweep) {
                                                               S.ecnt.get(), S.t12.get(), S.t13.get(), S.t23.get(),
                                                               S.delta.get(), P);
     RAMA_LAUNCHED();
+    // per edge: count, offsets, cost in/out; per slot: slot id + message
+    prof::Scope ps2(ctx, sweep ? "mp_edges" : "mp_edges_zero", 16.0 * E + 24.0 * T);
     if (sweep)
         k_mp_edges<true><<<ge, kBlock, 0, ctx.stream>>>(E, S.ecnt.get(), S.soff.get(), S.slots.get(), S.delta.get(),
                                                          S.ec.get(), P + gt);

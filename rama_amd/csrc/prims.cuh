@@ -7,6 +7,7 @@
 #include <cstring>
 
 #include "common.hpp"
+#include "profiler.hpp"
 
 namespace rama {
 

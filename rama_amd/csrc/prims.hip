@@ -118,11 +118,18 @@ void radix_sort_pairs(Ctx& ctx, u64* keys, u32* vals, u64* tk, u32* tv, i64 n, i
     const int passes = (bits + 7) / 8;
     for (int p = 0; p < passes; ++p) {
         const int shift = 8 * p;
-        k_radix_hist<<<ntiles, kBlock, 0, ctx.stream>>>(ka, n, shift, hist.get(), ntiles);
-        RAMA_LAUNCHED();
+        {
+            prof::Scope ps(ctx, "radix_hist", 8.0 * n);
+            k_radix_hist<<<ntiles, kBlock, 0, ctx.stream>>>(ka, n, shift, hist.get(), ntiles);
+            RAMA_LAUNCHED();
+        }
         scan_exclusive_arr<int>(ctx, hist.get(), hist.get(), (i64)256 * ntiles);
-        k_radix_scatter<<<ntiles, kBlock, 0, ctx.stream>>>(ka, va, kb, vb, n, shift, hist.get(), ntiles);
-        RAMA_LAUNCHED();
+        {
+            // reads and writes one (u64 key, u32 value) pair per element
+            prof::Scope ps(ctx, "radix_scatter", 24.0 * n);
+            k_radix_scatter<<<ntiles, kBlock, 0, ctx.stream>>>(ka, va, kb, vb, n, shift, hist.get(), ntiles);
+            RAMA_LAUNCHED();
+        }
         std::swap(ka, kb);
         std::swap(va, vb);
     }

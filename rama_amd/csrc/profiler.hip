@@ -1,0 +1,109 @@
+// profiler.hip — event bookkeeping behind profiler.hpp and its C ABI:
+//   rama_profile_enable(on), rama_profile_reset(),
+//   rama_profile_count(), rama_profile_entry(k, name, launches, ms, bytes).
+#include <atomic>
+#include <mutex>
+
+#include "profiler.hpp"
+
+namespace rama {
+namespace prof {
+
+namespace {
+std::atomic<bool> g_on{false};
+std::mutex g_mu;
+struct Pending {
+    const char* name;
+    hipEvent_t a, b;
+    double bytes;
+};
+std::vector<Pending> g_pending;
+std::vector<hipEvent_t> g_free;
+struct Agg {
+    std::string name;
+    int64_t launches = 0;
+    double ms = 0.0;
+    double bytes = 0.0;
+};
+std::vector<Agg> g_agg;
+
+void resolve_locked() {
+    for (auto& p : g_pending) {
+        float ms = 0.0f;
+        if (hipEventSynchronize(p.b) == hipSuccess && hipEventElapsedTime(&ms, p.a, p.b) == hipSuccess) {
+            Agg* a = nullptr;
+            for (auto& x : g_agg)
+                if (x.name == p.name) a = &x;
+            if (!a) {
+                g_agg.push_back(Agg{p.name});
+                a = &g_agg.back();
+            }
+            a->launches += 1;
+            a->ms += ms;
+            a->bytes += p.bytes;
+        }
+        (void)hipGetLastError();
+        g_free.push_back(p.a);
+        g_free.push_back(p.b);
+    }
+    g_pending.clear();
+}
+}  // namespace
+
+bool enabled() { return g_on.load(std::memory_order_relaxed); }
+
+hipEvent_t take_event() {
+    std::lock_guard<std::mutex> l(g_mu);
+    if (!g_free.empty()) {
+        hipEvent_t e = g_free.back();
+        g_free.pop_back();
+        return e;
+    }
+    hipEvent_t e = nullptr;
+    RAMA_HIP(hipEventCreate(&e));
+    return e;
+}
+
+void push(const char* name, hipEvent_t a, hipEvent_t b, double bytes) {
+    std::lock_guard<std::mutex> l(g_mu);
+    g_pending.push_back(Pending{name, a, b, bytes});
+    if (g_pending.size() > 4096) resolve_locked();
+}
+
+}  // namespace prof
+}  // namespace rama
+
+using namespace rama::prof;
+
+extern "C" {
+
+void rama_profile_enable(int on) {
+    std::lock_guard<std::mutex> l(g_mu);
+    resolve_locked();
+    g_on.store(on != 0);
+}
+
+void rama_profile_reset(void) {
+    std::lock_guard<std::mutex> l(g_mu);
+    resolve_locked();
+    g_agg.clear();
+}
+
+int64_t rama_profile_count(void) {
+    std::lock_guard<std::mutex> l(g_mu);
+    resolve_locked();
+    return (int64_t)g_agg.size();
+}
+
+int rama_profile_entry(int64_t k, const char** name, int64_t* launches, double* ms, double* bytes) {
+    std::lock_guard<std::mutex> l(g_mu);
+    resolve_locked();
+    if (k < 0 || k >= (int64_t)g_agg.size()) return RAMA_ERR_INVALID;
+    *name = g_agg[k].name.c_str();
+    *launches = g_agg[k].launches;
+    *ms = g_agg[k].ms;
+    *bytes = g_agg[k].bytes;
+    return 0;
+}
+
+}  // extern "C"

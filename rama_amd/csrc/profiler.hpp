@@ -1,0 +1,43 @@
+// profiler.hpp — optional per-kernel timing with HIP events recorded on the
+// launching stream.  Each instrumented launch carries the algorithmic bytes
+// it must move (a roofline numerator), so achieved bandwidth = bytes / time.
+// Disabled by default; enabling costs two event records per launch.
+#pragma once
+
+#include "common.hpp"
+
+namespace rama {
+namespace prof {
+
+bool enabled();
+void push(const char* name, hipEvent_t a, hipEvent_t b, double bytes);
+hipEvent_t take_event();
+
+// Brackets the launches in its scope with events on ctx.stream.
+class Scope {
+   public:
+    Scope(Ctx& ctx, const char* name, double bytes) : ctx_(ctx), name_(name), bytes_(bytes) {
+        if (enabled()) {
+            a_ = take_event();
+            b_ = take_event();
+            RAMA_HIP(hipEventRecord(a_, ctx_.stream));
+        }
+    }
+    ~Scope() {
+        if (a_) {
+            (void)hipEventRecord(b_, ctx_.stream);
+            push(name_, a_, b_, bytes_);
+        }
+    }
+    Scope(const Scope&) = delete;
+    Scope& operator=(const Scope&) = delete;
+
+   private:
+    Ctx& ctx_;
+    const char* name_;
+    double bytes_;
+    hipEvent_t a_ = nullptr, b_ = nullptr;
+};
+
+}  // namespace prof
+}  // namespace rama

@@ -1,0 +1,50 @@
+"""Per-kernel timing from the library's HIP-event profiler (rama_profile_*).
+
+Events are recorded on the stream the solver launches on, so durations are
+those of the kernels themselves; bytes are each kernel's algorithmic HBM
+traffic (what an ideal implementation must move), giving achieved GB/s.
+"""
+import ctypes
+
+from . import _capi
+
+
+def enable(on=True):
+    L = _capi.lib()
+    if on:
+        L.rama_profile_reset()
+    L.rama_profile_enable(1 if on else 0)
+
+
+def read():
+    L = _capi.lib()
+    out = []
+    for k in range(L.rama_profile_count()):
+        name = ctypes.c_char_p()
+        n = ctypes.c_int64()
+        ms = ctypes.c_double()
+        by = ctypes.c_double()
+        if L.rama_profile_entry(k, ctypes.byref(name), ctypes.byref(n), ctypes.byref(ms), ctypes.byref(by)) == 0:
+            out.append({"name": name.value.decode(), "launches": n.value, "ms": ms.value, "bytes": by.value})
+    out.sort(key=lambda d: -d["ms"])
+    return out
+
+
+def top(stats, k=8):
+    return [{"name": d["name"], "launches": d["launches"], "ms": round(d["ms"], 3),
+             "GBps": round(d["bytes"] / (d["ms"] * 1e6), 1) if d["bytes"] > 0 and d["ms"] > 0 else None}
+            for d in stats[:k]]
+
+
+def roofline(stats, peak_gbs, name=None):
+    """Roofline entry of the dominant kernel that has a byte model (or `name`)."""
+    cands = [d for d in stats if d["bytes"] > 0 and d["ms"] > 0 and (name is None or d["name"] == name)]
+    if not cands:
+        return None
+    d = max(cands, key=lambda x: x["ms"])
+    per_launch_bytes = d["bytes"] / d["launches"]
+    per_launch_s = d["ms"] * 1e-3 / d["launches"]
+    achieved = per_launch_bytes / per_launch_s / 1e9
+    return {"kernel": d["name"], "bound": "hbm", "achieved": round(achieved, 1), "peak": peak_gbs, "unit": "GB/s",
+            "frac": round(achieved / peak_gbs, 4), "traffic": None, "launches": d["launches"],
+            "avg_launch_us": round(per_launch_s * 1e6, 2), "algorithmic_bytes_per_launch": per_launch_bytes}

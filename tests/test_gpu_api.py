@@ -1,0 +1,152 @@
+"""GPU tests of the drop-in surface: rama_py (as the reference's
+test/test_pytorch.py and test/test_pytorch_pointers.py use it), the device
+entry, sanitising, timelines, determinism, error paths, and size-independent
+properties at the full 12.47M-edge size."""
+import numpy as np
+import pytest
+
+from tests.instances_small import named, random_graph
+
+pytestmark = pytest.mark.gpu
+
+
+def _same_partition(a, b):
+    _, ia = np.unique(np.asarray(a), return_inverse=True)
+    _, ib = np.unique(np.asarray(b), return_inverse=True)
+    return len(ia) == len(ib) and len(set(zip(ia.tolist(), ib.tolist()))) == len(set(ia.tolist())) == len(set(ib.tolist()))
+
+
+def test_rama_py_triangle_host_torch_pointers(gpu):
+    import torch
+
+    from rama_amd import rama_py
+
+    opts = rama_py.multicut_solver_options("P")
+    opts.verbose = False
+    res = rama_py.rama_cuda([0, 1, 2], [1, 2, 0], [1.1, -2, 3], opts)
+    expected = res[0]
+    assert expected == [0, 1, 0]
+    i = torch.tensor([0, 1, 2], dtype=torch.int32, device="cuda")
+    j = torch.tensor([1, 2, 0], dtype=torch.int32, device="cuda")
+    costs = torch.tensor([1.1, -2, 3], dtype=torch.float32, device="cuda")
+    node_mapping, lb = rama_py.rama_torch(i, j, costs, opts)
+    assert torch.all(node_mapping.cpu() == torch.tensor(expected, dtype=torch.int32))
+    num_nodes = int(max(i.max(), j.max())) + 1
+    node_labels = torch.ones(num_nodes, device=i.device, dtype=torch.int32)
+    opts.dump_timeline = True
+    timeline = rama_py.rama_cuda_gpu_pointers(i.data_ptr(), j.data_ptr(), costs.data_ptr(), node_labels.data_ptr(),
+                                              num_nodes, i.numel(), i.device.index, opts)
+    torch.cuda.synchronize()
+    assert torch.all(node_labels.cpu() == torch.tensor(expected, dtype=torch.int32))
+    assert len(timeline) >= 1 and timeline[-1] == expected
+
+
+def test_rama_torch_rejects_bad_input(gpu):
+    import torch
+
+    from rama_amd import rama_py
+
+    opts = rama_py.multicut_solver_options()
+    opts.verbose = False
+    i = torch.tensor([0, 1], dtype=torch.int32)  # host tensor
+    with pytest.raises(RuntimeError):
+        rama_py.rama_torch(i, i, torch.tensor([1.0, 2.0]), opts)
+    d = torch.tensor([0, 1], dtype=torch.int32, device="cuda")
+    with pytest.raises(RuntimeError):
+        rama_py.rama_torch(d, d[:1], torch.tensor([1.0, 2.0], device="cuda"), opts)
+
+
+@pytest.mark.parametrize("name", ["grid64", "er3000"])
+def test_device_entry_equals_host_entry(gpu, name):
+    import torch
+
+    i, j, c = named(name)
+    hl, hlb, _, _ = gpu.solve(i, j, c, gpu.options("PD", verbose=0))
+    di, dj, dc = (torch.from_numpy(x).cuda() for x in (i, j, c))
+    lab = torch.full((len(hl) + 4,), -7, dtype=torch.int32, device="cuda")
+    n, dlb, _, _ = gpu.solve_device(di.data_ptr(), dj.data_ptr(), dc.data_ptr(), len(c), lab.data_ptr(), lab.numel(),
+                                    gpu.options("PD", verbose=0), stream=torch.cuda.current_stream().cuda_stream)
+    torch.cuda.synchronize()
+    assert n == len(hl) and dlb == hlb
+    assert np.array_equal(lab[:n].cpu().numpy(), hl)
+    # inputs are not modified
+    assert np.array_equal(di.cpu().numpy(), i) and np.array_equal(dc.cpu().numpy(), c)
+
+
+def test_sanitize_graph_matches_oracle(gpu, oracle):
+    # ids with gaps (isolated nodes) and repeated edges
+    i, j, c = random_graph(400, 1500, seed=5, dup=True)
+    i = (i * 3 + 2).astype(np.int32)
+    j = (j * 3 + 2).astype(np.int32)
+    g, glb, _, _ = gpu.solve(i, j, c, gpu.options("PD", verbose=0, sanitize_graph=1))
+    o, olb, _ = oracle.solve(i, j, c, oracle.options("PD", sanitize_graph=1))
+    assert len(g) == len(o) == int(max(i.max(), j.max())) + 1
+    assert np.array_equal(g == -1, o == -1)
+    assert np.all(g[[0, 1]] == -1)
+    assert abs(glb - olb) <= 1e-6 * max(1.0, abs(olb))
+    keep = g >= 0
+    assert _same_partition(g[keep], o[keep])
+
+
+def test_timeline_ends_at_the_final_labels(gpu):
+    i, j, c = named("grid64")
+    lab, _, _, tl = gpu.solve(i, j, c, gpu.options("PD", verbose=0, dump_timeline=1), want_timeline=True)
+    assert len(tl) >= 2
+    assert np.array_equal(tl[-1], lab)
+    # contraction only merges: cluster counts never increase
+    counts = [len(np.unique(t)) for t in tl]
+    assert all(a >= b for a, b in zip(counts, counts[1:]))
+
+
+def test_deterministic(gpu):
+    i, j, c = named("grid96x128_lr")
+    a = gpu.solve(i, j, c, gpu.options("PD", verbose=0))
+    b = gpu.solve(i, j, c, gpu.options("PD", verbose=0))
+    assert np.array_equal(a[0], b[0]) and a[1] == b[1]
+
+
+def test_error_paths(gpu):
+    i, j, c = named("grid64")
+    r, co, v, (rows, cols) = gpu.dcoo_init(i, j, c)
+    with pytest.raises(gpu.RamaError) as e:
+        gpu.conflicted_cycles(r, co, v, rows, cols, 6)
+    assert "max_cycle_length" in str(e.value)
+    # a map merging the two ends of a repulsive edge creates a negative self-loop
+    k = int(np.argmin(v))
+    mp = np.arange(max(rows, cols), dtype=np.int32)
+    mp[co[k]] = r[k]
+    *_, bad, _ = gpu.contract(r, co, v, mp)
+    assert bad
+    with pytest.raises(gpu.RamaError):
+        gpu.solve(np.zeros(0, np.int32), np.zeros(0, np.int32), np.zeros(0, np.float32))
+    with pytest.raises(gpu.RamaError):
+        gpu.solve([0, -1], [1, 2], [1.0, 1.0])
+
+
+@pytest.fixture(scope="module")
+def full_grid():
+    from rama_amd import instances
+
+    return instances.grid2d_voronoi(H=1024, W=2048, seed=0)
+
+
+@pytest.mark.parametrize("preset", ["PD", "D"])
+def test_full_size_properties(gpu, full_grid, preset):
+    """12,469,248 edges: bound below objective, labels cover every node,
+    clusters near the planted 2000, determinism across runs."""
+    i, j, c, planted = full_grid
+    assert len(c) == 12469248
+    lab, lb, ms, _ = gpu.solve(i, j, c, gpu.options(preset, verbose=0))
+    trivial = float(np.minimum(c, 0).astype(np.float64).sum())
+    assert lb > trivial
+    if preset == "D":
+        assert len(lab) == 0
+        return
+    assert len(lab) == 1024 * 2048
+    obj = gpu.objective(lab, i, j, c)
+    assert lb <= obj
+    assert (obj - lb) / abs(lb) < 0.02
+    k = len(np.unique(lab))
+    assert 1000 < k < 8000
+    lab2, lb2, _, _ = gpu.solve(i, j, c, gpu.options(preset, verbose=0))
+    assert lb2 == lb and np.array_equal(lab, lab2)
