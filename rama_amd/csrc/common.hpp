@@ -197,7 +197,38 @@ struct Ctx {
     template <class T>
     DBuf<T> buf(i64 n) { return DBuf<T>(arena, n); }
 
-    void sync() { RAMA_HIP(hipStreamSynchronize(stream)); }
+    int64_t syncs = 0;  // host round trips (diagnostic)
+
+    // Single-pass scan state: per-tile status words tagged with an epoch so
+    // no launch needs the buffer cleared; a tile counter that the last tile
+    // resets; a device error flag raised by a look-back that gave up.
+    DBuf<uint64_t> scan_status;
+    DBuf<int> scan_counter;  // [0] tile counter, [1] error flag
+    uint32_t scan_epoch = 0;
+    // Returns the status buffer (at least `tiles` words) and the epoch to use.
+    uint64_t* scan_begin(int64_t tiles, uint32_t* epoch) {
+        if (scan_counter.empty()) {
+            scan_counter = DBuf<int>(arena, 2);
+            zero(scan_counter.get(), 2);
+        }
+        if (scan_status.size() < tiles) {
+            int64_t cap = 1024;
+            while (cap < tiles) cap <<= 1;
+            scan_status = DBuf<uint64_t>(arena, cap);
+            zero(scan_status.get(), cap);
+            scan_epoch = 0;
+        }
+        if (++scan_epoch >= 65535u) {
+            zero(scan_status.get(), scan_status.size());
+            scan_epoch = 1;
+        }
+        *epoch = scan_epoch;
+        return scan_status.get();
+    }
+    void sync() {
+        ++syncs;
+        RAMA_HIP(hipStreamSynchronize(stream));
+    }
 
     // Read `n` device scalars (contiguous) into host memory (synchronises).
     template <class T>

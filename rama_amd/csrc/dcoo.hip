@@ -119,25 +119,38 @@ DCoo dcoo_copy(Ctx& ctx, const DCoo& A) {
 // Per-segment reduction of the sorted contraction keys: one thread per
 // (row, col) segment sums its costs in double in sorted order and rounds
 // once; self-loops are flagged (negative -> bad contraction) and dropped.
-__global__ void k_contract_reduce(const u64* __restrict__ K, const u32* __restrict__ V, const int* __restrict__ head_pos,
-                                  const int* __restrict__ seg_start, i64 S, i64 n, int kb, u64* __restrict__ okey,
-                                  float* __restrict__ oval, int* __restrict__ keep, int* bad, double* diag) {
+__global__ __launch_bounds__(kBlock) void k_contract_reduce(const u64* __restrict__ K, const u32* __restrict__ V,
+                                                           const int* __restrict__ seg_start, i64 S, i64 n, int kb,
+                                                           u64* __restrict__ okey, float* __restrict__ oval,
+                                                           int* __restrict__ keep, int* bad, double* diag) {
+    __shared__ double ws[kWaves];
+    __shared__ int wbad[kWaves];
     const i64 s = (i64)blockIdx.x * blockDim.x + threadIdx.x;
-    if (s >= S) return;
-    const i64 b = seg_start[s];
-    const i64 e = s + 1 < S ? seg_start[s + 1] : n;
-    double acc = 0.0;
-    for (i64 k = b; k < e; ++k) acc += (double)__uint_as_float(V[k]);
-    const u64 key = K[b];
-    const u64 mask = ((u64)1 << kb) - 1;
-    const int r = (int)(key >> kb), c = (int)(key & mask);
-    const float f = (float)acc;
-    okey[s] = key;
-    oval[s] = f;
-    keep[s] = r != c;
-    if (r == c) {
-        if (f < 0.0f) *bad = 1;
-        atomicAdd(diag, (double)f);
+    double dsum = 0.0;
+    int isbad = 0;
+    if (s < S) {
+        const i64 b = seg_start[s];
+        const i64 e = s + 1 < S ? seg_start[s + 1] : n;
+        double acc = 0.0;
+        for (i64 k = b; k < e; ++k) acc += (double)__uint_as_float(V[k]);
+        const u64 key = K[b];
+        const u64 mask = ((u64)1 << kb) - 1;
+        const int r = (int)(key >> kb), c = (int)(key & mask);
+        const float f = (float)acc;
+        okey[s] = key;
+        oval[s] = f;
+        keep[s] = r != c;
+        if (r == c) {
+            isbad = f < 0.0f;
+            dsum = (double)f;
+        }
+    }
+    // one atomic per block for the self-loop total and the bad flag
+    const double bsum = block_reduce(dsum, OpSum(), ws);
+    const int bbad = block_reduce(isbad, OpMax(), wbad);
+    if (threadIdx.x == 0) {
+        if (bsum != 0.0) atomicAdd(diag, bsum);
+        if (bbad) *bad = 1;
     }
 }
 
@@ -185,7 +198,7 @@ DCoo contract(Ctx& ctx, const DCoo& A, const int* node_map, int max_label, bool*
     {
         // 12 B in per sorted edge; seg_start in + key, cost, keep flag out per segment
         prof::Scope ps(ctx, "contract_reduce", 12.0 * n + 20.0 * S);
-        k_contract_reduce<<<grid_for(S), kBlock, 0, ctx.stream>>>(K, V, HP, SS, S, n, kb, okey.get(), oval.get(),
+        k_contract_reduce<<<grid_for(S), kBlock, 0, ctx.stream>>>(K, V, SS, S, n, kb, okey.get(), oval.get(),
                                                                     keep.get(), badp, scal.get());
         RAMA_LAUNCHED();
     }
@@ -251,14 +264,45 @@ void compute_offsets(Ctx& ctx, const int* sorted_ids, i64 n, int max_value, int*
 // (include/rama_utils.h:107-131, src/dCOO.cu:144-147,164-172,
 //  src/conflicted_cycles_cuda.cu:208-217).  With `filter`, only edges with
 // cost > keep_above enter (is_positive_edge).
-Csr symmetric_csr(Ctx& ctx, const int* row, const int* col, const float* val, i64 m, int n_nodes, bool filter,
-                  float keep_above) {
+//
+// Input: a (row, col)-sorted directed edge list with row < col.  Row r of the
+// symmetric matrix is [r' of edges (r', r), ascending] ++ [c of edges (r, c),
+// ascending], because every r' < r < c.  The first part is the transpose: a
+// stable sort of the edges by col alone (equal cols keep the input's
+// ascending rows), on 8-byte (u32 col, u32 edge) pairs.  The second part is
+// the edge list itself.  Duplicated edges keep their input order, as the
+// reference's stable sort of the doubled list does.
+Csr symmetric_csr(Ctx& ctx, const int* row_in, const int* col_in, const float* val_in, i64 m, int n_nodes,
+                  bool filter, float keep_above) {
     Csr G;
     G.n = n_nodes;
-    DBuf<int> pos;
+    const int* row = row_in;
+    const int* col = col_in;
+    const float* val = val_in;
+    DBuf<int> fr, fc;
+    DBuf<float> fv;
     i64 P = m;
     if (filter) {
-        P = select_positions(ctx, [=] __device__(i64 e) { return (int)(val[e] > keep_above); }, m, pos);
+        DBuf<int> pos;
+        P = select_positions(ctx, [=] __device__(i64 e) { return (int)(val_in[e] > keep_above); }, m, pos);
+        fr = ctx.buf<int>(P);
+        fc = ctx.buf<int>(P);
+        fv = ctx.buf<float>(P);
+        int* FR = fr.get();
+        int* FC = fc.get();
+        float* FV = fv.get();
+        const int* PP = pos.get();
+        for_each(ctx, m, [=] __device__(i64 e) {
+            if (val_in[e] > keep_above) {
+                const int q = PP[e];
+                FR[q] = row_in[e];
+                FC[q] = col_in[e];
+                FV[q] = val_in[e];
+            }
+        });
+        row = FR;
+        col = FC;
+        val = FV;
     }
     const i64 N2 = 2 * P;
     G.nnz = N2;
@@ -269,33 +313,43 @@ Csr symmetric_csr(Ctx& ctx, const int* row, const int* col, const float* val, i6
         ctx.zero(G.off.get(), (i64)n_nodes + 1);
         return G;
     }
-    const int kb = bits_for(n_nodes > 1 ? n_nodes - 1 : 1);
-    DBuf<u64> keys = ctx.buf<u64>(N2);
-    DBuf<u32> vals = ctx.buf<u32>(N2);
-    u64* K = keys.get();
-    u32* V = vals.get();
-    const int* PP = filter ? pos.get() : nullptr;
-    for_each(ctx, m, [=] __device__(i64 e) {
-        if (PP && !(val[e] > keep_above)) return;
-        const i64 q = PP ? PP[e] : e;
-        const u32 a = (u32)row[e], b = (u32)col[e];
-        const u32 v = f2u(val[e]);
-        K[q] = ((u64)a << kb) | b;
-        V[q] = v;
-        K[P + q] = ((u64)b << kb) | a;
-        V[P + q] = v;
+    // transpose: stable sort of (col, edge) by col
+    DBuf<u32> tk = ctx.buf<u32>(P), tv = ctx.buf<u32>(P);
+    u32* TK = tk.get();
+    u32* TV = tv.get();
+    for_each(ctx, P, [=] __device__(i64 e) {
+        TK[e] = (u32)col[e];
+        TV[e] = (u32)e;
     });
-    radix_sort_pairs(ctx, keys, vals, N2, 2 * kb);
+    radix_sort_pairs<u32>(ctx, tk, tv, P, bits_for(n_nodes > 1 ? n_nodes - 1 : 1));
+    DBuf<int> offF = ctx.buf<int>((i64)n_nodes + 1), offT = ctx.buf<int>((i64)n_nodes + 1);
+    compute_offsets(ctx, row, P, n_nodes - 1, offF.get());
+    {
+        auto colof = [=] __device__(i64 i) { return (int)TK[i]; };
+        k_offsets<<<grid_for(P + 1), kBlock, 0, ctx.stream>>>(colof, P, n_nodes, offT.get());
+        RAMA_LAUNCHED();
+    }
+    int* OFF = G.off.get();
+    const int* OF = offF.get();
+    const int* OT = offT.get();
+    for_each(ctx, (i64)n_nodes + 1, [=] __device__(i64 r) { OFF[r] = OF[r] + OT[r]; });
     int* ADJ = G.adj.get();
     float* WW = G.w.get();
-    const u64 mask = ((u64)1 << kb) - 1;
-    for_each(ctx, N2, [=] __device__(i64 e) {
-        ADJ[e] = (int)(K[e] & mask);
-        WW[e] = u2f(V[e]);
+    // backward part of row r: transposed entries, in order
+    for_each(ctx, P, [=] __device__(i64 k) {
+        const int r = (int)TK[k];
+        const u32 e = TV[k];
+        const i64 o = (i64)OFF[r] + (k - OT[r]);
+        ADJ[o] = row[e];
+        WW[o] = val[e];
     });
-    auto rowof = [=] __device__(i64 i) { return (int)(K[i] >> kb); };
-    k_offsets<<<grid_for(N2 + 1), kBlock, 0, ctx.stream>>>(rowof, N2, n_nodes, G.off.get());
-    RAMA_LAUNCHED();
+    // forward part of row r: the edge list's row segment
+    for_each(ctx, P, [=] __device__(i64 e) {
+        const int r = row[e];
+        const i64 o = (i64)OFF[r] + (OT[r + 1] - OT[r]) + (e - OF[r]);
+        ADJ[o] = col[e];
+        WW[o] = val[e];
+    });
     return G;
 }
 

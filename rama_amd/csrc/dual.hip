@@ -16,14 +16,20 @@ static double dual_update_cycle_length(Ctx& ctx, DCoo& A, int c, int steps, floa
     MPState S;
     mp_build(ctx, A, tri, S);
     tri = Triangles();
-    double lb = mp_lower_bound(ctx, S);
-    double prev = 0.0;
-    for (int it = 0; it < steps; ++it) {
-        if (ctx.verbose) std::printf("dual updates cycle length: %d, iteration: %d, lower bound: %.6f\n", c, it, lb);
-        if (it > 0 && (lb - prev) < 1e-3) break;
-        mp_iteration(ctx, S);
-        prev = lb;
-        lb = mp_fused_lower_bound(ctx, S);
+    double lb;
+    if (ctx.verbose) {
+        // host-driven loop so every iteration's bound can be printed
+        lb = mp_lower_bound(ctx, S);
+        double prev = 0.0;
+        for (int it = 0; it < steps; ++it) {
+            std::printf("dual updates cycle length: %d, iteration: %d, lower bound: %.6f\n", c, it, lb);
+            if (it > 0 && (lb - prev) < 1e-3) break;
+            mp_iteration(ctx, S);
+            prev = lb;
+            lb = mp_fused_lower_bound(ctx, S);
+        }
+    } else {
+        lb = mp_run(ctx, S, steps);
     }
     if (ctx.verbose) std::printf("dual updates cycle length: %d, final lower bound: %.6f\n", c, lb);
     A = mp_reparametrized(ctx, S);

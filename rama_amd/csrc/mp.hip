@@ -1,11 +1,9 @@
 // mp.hip — triangle/edge min-marginal message passing
 // (src/multicut_message_passing.cu:15-360).
 //
-// Construction merges the 3T triangle edges with the graph's edges in one
-// stable radix sort (graph edges first within a key), so each distinct edge
-// is a run of the sorted sequence: the run head gives the edge id, an
-// included graph edge gives its cost (new chords cost 0), and the run's
-// triangle items are the edge's slot list in (triangle, slot) order.
+// Construction sorts the 3T triangle edges and merges them into the graph's
+// sorted edge list: graph edges keep their cost, new chords cost 0, and each
+// edge's run of triangle items is its slot list in (triangle, slot) order.
 //
 // One iteration is two kernels:
 //   k_mp_triangles  per triangle: gather the three edge costs / counts, add
@@ -21,101 +19,159 @@
 
 namespace rama {
 
-constexpr u32 kGraphItem = 0x80000000u;
-
-void mp_build(Ctx& ctx, const DCoo& A, Triangles& tri, MPState& S) {
-    const i64 E = A.nnz, T = tri.n;
-    S.T = T;
-    const i64 N = E + 3 * T;
-    int maxv = A.max_dim() - 1;
-    if (T > 0) maxv = std::max(maxv, max_id(ctx, tri.t3.get(), tri.t3.get(), T));
-    const int kb = bits_for(maxv < 1 ? 1 : maxv);
-    DBuf<u64> keys = ctx.buf<u64>(N);
-    DBuf<u32> vals = ctx.buf<u32>(N);
-    u64* K = keys.get();
-    u32* V = vals.get();
-    {
-        const int* R = A.row.get();
-        const int* C = A.col.get();
-        for_each(ctx, E, [=] __device__(i64 e) {
-            K[e] = ((u64)(u32)R[e] << kb) | (u32)C[e];
-            V[e] = kGraphItem | (u32)e;
-        });
-        const int* T1 = tri.t1.get();
-        const int* T2 = tri.t2.get();
-        const int* T3 = tri.t3.get();
-        for_each(ctx, T, [=] __device__(i64 t) {
-            const u64 a = (u32)T1[t], b = (u32)T2[t], c = (u32)T3[t];
-            K[E + 3 * t + 0] = (a << kb) | b;
-            K[E + 3 * t + 1] = (a << kb) | c;
-            K[E + 3 * t + 2] = (b << kb) | c;
-            V[E + 3 * t + 0] = (u32)(3 * t + 0);
-            V[E + 3 * t + 1] = (u32)(3 * t + 1);
-            V[E + 3 * t + 2] = (u32)(3 * t + 2);
-        });
+template <class K>
+__device__ __forceinline__ i64 lower_bound_dev(const K* a, i64 n, K x) {
+    i64 lo = 0, hi = n;
+    while (lo < hi) {
+        const i64 mid = (lo + hi) >> 1;
+        if (a[mid] < x) lo = mid + 1; else hi = mid;
     }
-    if (T > 0) radix_sort_pairs(ctx, keys, vals, N, 2 * kb);  // graph part is already sorted when T == 0
-    DBuf<int> uid;
-    const i64 Em = select_positions(
-        ctx, [=] __device__(i64 q) { return (int)(q == 0 || K[q] != K[q - 1]); }, N, uid);
-    DBuf<int> tpos;
-    select_positions(ctx, [=] __device__(i64 q) { return (int)((V[q] & kGraphItem) == 0); }, N, tpos);
+    return lo;
+}
+
+// Builds the message-passing problem.  The graph's edges are already sorted
+// and unique; only the 3T triangle-edge items (key, slot 3t+k) are sorted.
+// Triangle keys absent from the graph are the chords.  In the merged edge
+// list a key's position is (#graph keys below it) + (#chord keys below it);
+// an edge's slots are the sorted triangle items carrying its key, which are
+// contiguous and already in (triangle, slot) order (the sort is stable).
+template <class K>
+static void mp_build_impl(Ctx& ctx, const DCoo& A, Triangles& tri, MPState& S, int kb) {
+    const i64 E = A.nnz, T = tri.n, N3 = 3 * T;
+    S.T = T;
+    const int* AR = A.row.get();
+    const int* AC = A.col.get();
+    const float* AV = A.val.get();
+    DBuf<K> ka = ctx.buf<K>(E);
+    K* KA = ka.get();
+    for_each(ctx, E, [=] __device__(i64 e) { KA[e] = ((K)(u32)AR[e] << kb) | (K)(u32)AC[e]; });
+    DBuf<K> kt = ctx.buf<K>(N3);
+    DBuf<u32> vt = ctx.buf<u32>(N3);
+    K* KT = kt.get();
+    u32* VT = vt.get();
+    const int* T1 = tri.t1.get();
+    const int* T2 = tri.t2.get();
+    const int* T3 = tri.t3.get();
+    for_each(ctx, T, [=] __device__(i64 t) {
+        const K a = (u32)T1[t], b = (u32)T2[t], c = (u32)T3[t];
+        KT[3 * t + 0] = (a << kb) | b;
+        KT[3 * t + 1] = (a << kb) | c;
+        KT[3 * t + 2] = (b << kb) | c;
+        VT[3 * t + 0] = (u32)(3 * t + 0);
+        VT[3 * t + 1] = (u32)(3 * t + 1);
+        VT[3 * t + 2] = (u32)(3 * t + 2);
+    });
+    radix_sort_pairs<K>(ctx, kt, vt, N3, 2 * kb);
+    // chords: first item of a key run whose key is not a graph edge
+    auto is_chord = [=] __device__(i64 q) {
+        if (q > 0 && KT[q] == KT[q - 1]) return 0;
+        const i64 p = lower_bound_dev(KA, E, KT[q]);
+        return (int)(p == E || KA[p] != KT[q]);
+    };
+    DBuf<int> cpos;
+    const i64 Nc = select_positions(ctx, is_chord, N3, cpos);
+    DBuf<K> ck = ctx.buf<K>(Nc);
+    DBuf<int> chead = ctx.buf<int>(Nc);
+    K* CK = ck.get();
+    int* CH = chead.get();
+    const int* CP = cpos.get();
+    for_each(ctx, N3, [=] __device__(i64 q) {
+        if (is_chord(q)) {
+            CK[CP[q]] = KT[q];
+            CH[CP[q]] = (int)q;
+        }
+    });
+    const i64 Em = E + Nc;
     S.E = Em;
     S.ei = ctx.buf<int>(Em);
     S.ej = ctx.buf<int>(Em);
     S.ec = ctx.buf<float>(Em);
     S.ecnt = ctx.buf<int>(Em);
     S.soff = ctx.buf<int>(Em + 1);
-    S.slots = ctx.buf<int>(3 * T > 0 ? 3 * T : 1);
     S.c12 = ctx.buf<int>(T);
     S.c13 = ctx.buf<int>(T);
     S.c23 = ctx.buf<int>(T);
     S.t12 = ctx.buf<float>(T);
     S.t13 = ctx.buf<float>(T);
     S.t23 = ctx.buf<float>(T);
-    S.delta = ctx.buf<float>(3 * T > 0 ? 3 * T : 1);
+    S.delta = ctx.buf<float>(N3 > 0 ? N3 : 1);
     ctx.zero(S.t12.get(), T);
     ctx.zero(S.t13.get(), T);
     ctx.zero(S.t23.get(), T);
-    ctx.zero(S.ec.get(), Em);
-    ctx.zero(S.ecnt.get(), Em);
     int* EI = S.ei.get();
     int* EJ = S.ej.get();
     float* EC = S.ec.get();
     int* ECNT = S.ecnt.get();
     int* SOFF = S.soff.get();
-    int* SL = S.slots.get();
-    int* CR[3] = {S.c12.get(), S.c13.get(), S.c23.get()};
-    int* C12 = CR[0];
-    int* C13 = CR[1];
-    int* C23 = CR[2];
-    const int* U = uid.get();
-    const int* TP = tpos.get();
-    const float* AV = A.val.get();
     const u64 mask = ((u64)1 << kb) - 1;
-    for_each(ctx, N, [=] __device__(i64 q) {
-        const u32 v = V[q];
-        const bool head = q == 0 || K[q] != K[q - 1];
-        const int u = U[q] - (head ? 0 : 1);  // uid holds the exclusive count of heads
-        if (head) {
-            EI[u] = (int)(K[q] >> kb);
-            EJ[u] = (int)(K[q] & mask);
-            SOFF[u] = TP[q];
-        }
-        if (v & kGraphItem) {
-            EC[u] = AV[v & ~kGraphItem];
-        } else {
-            SL[TP[q]] = (int)v;
-            const u32 t = v / 3, k = v % 3;
-            (k == 0 ? C12 : (k == 1 ? C13 : C23))[t] = u;
-            atomicAdd(&ECNT[u], 1);
-        }
+    // graph edges
+    for_each(ctx, E, [=] __device__(i64 e) {
+        const i64 u = e + lower_bound_dev(CK, Nc, KA[e]);
+        EI[u] = AR[e];
+        EJ[u] = AC[e];
+        EC[u] = AV[e];
+        SOFF[u] = (int)lower_bound_dev(KT, N3, KA[e]);
     });
-    const int* TPe = tpos.get();
-    int* SO = S.soff.get();
-    for_each(ctx, 1, [=] __device__(i64) { SO[Em] = TPe[N]; });
-    const i64 nb = (std::max(T, Em) + kBlock - 1) / kBlock + 2;
+    // chords (cost 0)
+    for_each(ctx, Nc, [=] __device__(i64 c) {
+        const i64 u = c + lower_bound_dev(KA, E, CK[c]);
+        EI[u] = (int)((u64)CK[c] >> kb);
+        EJ[u] = (int)((u64)CK[c] & mask);
+        EC[u] = 0.0f;
+        SOFF[u] = CH[c];
+    });
+    for_each(ctx, 1, [=] __device__(i64) { SOFF[Em] = (int)N3; });
+    for_each(ctx, Em, [=] __device__(i64 u) { ECNT[u] = SOFF[u + 1] - SOFF[u]; });
+    // triangle -> edge correspondences
+    int* C12 = S.c12.get();
+    int* C13 = S.c13.get();
+    int* C23 = S.c23.get();
+    for_each(ctx, N3, [=] __device__(i64 q) {
+        const K k = KT[q];
+        const int u = (int)(lower_bound_dev(KA, E, k) + lower_bound_dev(CK, Nc, k));
+        const u32 v = VT[q], t = v / 3, s = v % 3;
+        (s == 0 ? C12 : (s == 1 ? C13 : C23))[t] = u;
+    });
+    // the sorted slot ids are the edges' slot lists
+    S.slots = ctx.buf<int>(N3 > 0 ? N3 : 1);
+    ctx.d2d(S.slots.get(), reinterpret_cast<const int*>(VT), N3);
+}
+
+void mp_build(Ctx& ctx, const DCoo& A, Triangles& tri, MPState& S) {
+    int maxv = A.max_dim() - 1;
+    if (tri.n > 0) maxv = std::max(maxv, max_id(ctx, tri.t3.get(), tri.t3.get(), tri.n));
+    const int kb = bits_for(maxv < 1 ? 1 : maxv);
+    if (2 * kb <= 32)
+        mp_build_impl<u32>(ctx, A, tri, S, kb);
+    else
+        mp_build_impl<u64>(ctx, A, tri, S, kb);
+    const i64 nb = (std::max(S.T, S.E) + kBlock - 1) / kBlock + 2;
     S.lbparts = ctx.buf<double>(2 * nb + 2);
+}
+
+// Device-side state of the dual loop (src/dual_solver.cu:17-26): the
+// current and previous lower bound and whether iterations still run.
+struct MPControl {
+    double lb;
+    double prev;
+    int active;
+    int iterations;
+};
+
+// After an iteration: sum the fused lower-bound partials (fixed order) and
+// apply the stall test `lb - prev < 1e-3` that stops the loop.
+__global__ __launch_bounds__(kBlock) void k_mp_finalize(const double* __restrict__ parts, i64 n, MPControl* ctl) {
+    __shared__ double ws[kWaves];
+    if (!ctl->active) return;
+    double s = 0.0;
+    for (i64 k = threadIdx.x; k < n; k += kBlock) s += parts[k];
+    const double lb = block_reduce(s, OpSum(), ws);
+    if (threadIdx.x == 0) {
+        ctl->prev = ctl->lb;
+        ctl->lb = lb;
+        ctl->iterations += 1;
+        if (lb - ctl->prev < 1e-3) ctl->active = 0;
+    }
 }
 
 // Triangle lower-bound term (triangle_lb_func, src/multicut_message_passing.cu:186-199).
@@ -145,8 +201,10 @@ __global__ __launch_bounds__(kBlock) void k_mp_triangles(i64 T, const int* __res
                                                          const int* __restrict__ c23, const float* __restrict__ ec,
                                                          const int* __restrict__ ecnt, float* __restrict__ t12,
                                                          float* __restrict__ t13, float* __restrict__ t23,
-                                                         float* __restrict__ delta, double* __restrict__ lbpart) {
+                                                         float* __restrict__ delta, double* __restrict__ lbpart,
+                                                         const MPControl* __restrict__ ctl) {
     __shared__ double ws[kWaves];
+    if (ctl && !ctl->active) return;  // uniform: the loop has stopped
     const i64 t = (i64)blockIdx.x * kBlock + threadIdx.x;
     double lb = 0.0;
     if (t < T) {
@@ -207,8 +265,10 @@ __global__ __launch_bounds__(kBlock) void k_mp_triangles(i64 T, const int* __res
 template <bool kSweep>
 __global__ __launch_bounds__(kBlock) void k_mp_edges(i64 E, const int* __restrict__ ecnt, const int* __restrict__ soff,
                                                      const int* __restrict__ slots, const float* __restrict__ delta,
-                                                     float* __restrict__ ec, double* __restrict__ lbpart) {
+                                                     float* __restrict__ ec, double* __restrict__ lbpart,
+                                                     const MPControl* __restrict__ ctl) {
     __shared__ double ws[kWaves];
+    if (ctl && !ctl->active) return;
     const i64 e = (i64)blockIdx.x * kBlock + threadIdx.x;
     double lb = 0.0;
     if (e < E) {
@@ -236,7 +296,7 @@ static double sum_parts(Ctx& ctx, MPState& S, i64 n) {
     return ctx.read1(P + slot);
 }
 
-static void launch_round(Ctx& ctx, MPState& S, bool sweep) {
+static void launch_round(Ctx& ctx, MPState& S, bool sweep, const MPControl* ctl = nullptr) {
     const i64 T = S.T, E = S.E;
     const unsigned gt = grid_for(T), ge = grid_for(E);
     double* P = S.lbparts.get();
@@ -246,20 +306,20 @@ static void launch_round(Ctx& ctx, MPState& S, bool sweep) {
     if (sweep)
         k_mp_triangles<true><<<gt, kBlock, 0, ctx.stream>>>(T, S.c12.get(), S.c13.get(), S.c23.get(), S.ec.get(),
                                                              S.ecnt.get(), S.t12.get(), S.t13.get(), S.t23.get(),
-                                                             S.delta.get(), P);
+                                                             S.delta.get(), P, ctl);
     else
         k_mp_triangles<false><<<gt, kBlock, 0, ctx.stream>>>(T, S.c12.get(), S.c13.get(), S.c23.get(), S.ec.get(),
                                                               S.ecnt.get(), S.t12.get(), S.t13.get(), S.t23.get(),
-                                                              S.delta.get(), P);
+                                                              S.delta.get(), P, ctl);
     RAMA_LAUNCHED();
     // per edge: count, offsets, cost in/out; per slot: slot id + message
     prof::Scope ps2(ctx, sweep ? "mp_edges" : "mp_edges_zero", 16.0 * E + 24.0 * T);
     if (sweep)
         k_mp_edges<true><<<ge, kBlock, 0, ctx.stream>>>(E, S.ecnt.get(), S.soff.get(), S.slots.get(), S.delta.get(),
-                                                         S.ec.get(), P + gt);
+                                                         S.ec.get(), P + gt, ctl);
     else
         k_mp_edges<false><<<ge, kBlock, 0, ctx.stream>>>(E, S.ecnt.get(), S.soff.get(), S.slots.get(), S.delta.get(),
-                                                          S.ec.get(), P + gt);
+                                                          S.ec.get(), P + gt, ctl);
     RAMA_LAUNCHED();
 }
 
@@ -281,6 +341,39 @@ double mp_lower_bound(Ctx& ctx, MPState& S) {
 }
 
 void mp_send_to_triplets(Ctx& ctx, MPState& S) { launch_round(ctx, S, false); }
+
+// The whole iteration loop of dual_update_cycle_length (src/dual_solver.cu:17-29)
+// queued without host round trips: the stall test runs on the device and
+// turns the remaining iterations into no-ops.  Returns the final lower bound.
+double mp_run(Ctx& ctx, MPState& S, int steps) {
+    DBuf<MPControl> ctl = ctx.buf<MPControl>(1);
+    double* P = S.lbparts.get();
+    const float* EC = S.ec.get();
+    const float* A = S.t12.get();
+    const float* B = S.t13.get();
+    const float* D = S.t23.get();
+    const i64 E = S.E;
+    reduce_to<double>(ctx, [=] __device__(i64 e) { const float c = EC[e]; return c < 0.0f ? (double)c : 0.0; }, E, 0.0,
+                      OpSum(), P);
+    reduce_to<double>(ctx, [=] __device__(i64 t) { return (double)tri_lb(A[t], B[t], D[t]); }, S.T, 0.0, OpSum(),
+                      P + 1);
+    MPControl* C = ctl.get();
+    for_each(ctx, 1, [=] __device__(i64) {
+        C->lb = P[0] + P[1];
+        C->prev = 0.0;
+        C->active = 1;
+        C->iterations = 0;
+    });
+    const i64 nparts = (i64)grid_for(S.T) + (i64)grid_for(S.E);
+    for (int it = 0; it < steps; ++it) {
+        launch_round(ctx, S, true, C);
+        k_mp_finalize<<<1, kBlock, 0, ctx.stream>>>(P, nparts, C);
+        RAMA_LAUNCHED();
+    }
+    MPControl h;
+    ctx.read(C, &h, 1);
+    return h.lb;
+}
 
 void mp_iteration(Ctx& ctx, MPState& S) { launch_round(ctx, S, true); }
 

@@ -98,28 +98,35 @@ struct OpMax {
 };
 
 // ------------------------------------------------------------------ scan
+// Single-pass exclusive scan of non-negative integer sums with decoupled
+// look-back.  Each block takes the next tile index from an atomic counter,
+// so every smaller tile belongs to a block that is already running; it
+// publishes its aggregate, walks back over predecessors until one carries an
+// inclusive prefix, and publishes its own inclusive prefix.  Status words
+// pack (epoch:16 | flag:2 | value:46); a word from another epoch reads as
+// "not ready".  A look-back that spins too long raises ctx's error flag
+// (checked by scan_check) instead of hanging.
 constexpr int kScanItems = 8;
 constexpr int kScanTile = kBlock * kScanItems;
+constexpr u64 kScanValMask = (1ull << 46) - 1;
+constexpr long kScanSpinLimit = 1l << 24;
 
-template <class T, class F>
-__global__ __launch_bounds__(kBlock) void k_scan_partials(F f, i64 n, T* partials) {
-    __shared__ T ws[kWaves];
-    const i64 base = (i64)blockIdx.x * kScanTile;
-    T s = T(0);
-#pragma unroll
-    for (int k = 0; k < kScanItems; ++k) {
-        const i64 i = base + (i64)k * kBlock + threadIdx.x;
-        if (i < n) s += (T)f(i);
-    }
-    T r = block_reduce(s, OpSum(), ws);
-    if (threadIdx.x == 0) partials[blockIdx.x] = r;
+__device__ __forceinline__ u64 scan_word(u32 epoch, u32 flag, u64 v) {
+    return ((u64)epoch << 48) | ((u64)flag << 46) | (v & kScanValMask);
 }
 
-// out[i] = sum_{k<i} f(k) for i in [0, n]; blocked per thread.
-template <class T, class F>
-__global__ __launch_bounds__(kBlock) void k_scan_final(F f, i64 n, const T* offs, T* out) {
+// out(i, exclusive_prefix, value) is called for every i < n; total (if not
+// null) receives the sum of all values.
+template <class T, class F, class Out>
+__global__ __launch_bounds__(kBlock) void k_scan_single(F f, i64 n, Out out, T* total, u64* status, int* ctl,
+                                                        u32 epoch) {
     __shared__ T ws[kWaves];
-    const i64 base = (i64)blockIdx.x * kScanTile + (i64)threadIdx.x * kScanItems;
+    __shared__ int tile_s;
+    __shared__ T prefix_s;
+    if (threadIdx.x == 0) tile_s = atomicAdd(&ctl[0], 1);
+    __syncthreads();
+    const int tile = tile_s;
+    const i64 base = (i64)tile * kScanTile + (i64)threadIdx.x * kScanItems;
     T v[kScanItems];
     T s = T(0);
 #pragma unroll
@@ -128,16 +135,70 @@ __global__ __launch_bounds__(kBlock) void k_scan_final(F f, i64 n, const T* offs
         v[k] = i < n ? (T)f(i) : T(0);
         s += v[k];
     }
-    T tot;
-    const T boff = offs ? offs[blockIdx.x] : T(0);
-    T run = block_excl_scan(s, ws, &tot) + boff;
+    T agg;
+    const T excl = block_excl_scan(s, ws, &agg);
+    if (threadIdx.x < 64) {
+        // wave 0: publish the aggregate, then look back 64 predecessors per
+        // step; lane l reads tile p - l.  The nearest inclusive prefix ends
+        // the walk once every tile before it in the window is ready.
+        const int lane = threadIdx.x;
+        T prefix = T(0);
+        if (tile == 0) {
+            if (lane == 0)
+                __hip_atomic_store(&status[0], scan_word(epoch, 2, (u64)agg), __ATOMIC_RELAXED,
+                                   __HIP_MEMORY_SCOPE_AGENT);
+        } else {
+            if (lane == 0)
+                __hip_atomic_store(&status[tile], scan_word(epoch, 1, (u64)agg), __ATOMIC_RELAXED,
+                                   __HIP_MEMORY_SCOPE_AGENT);
+            int p = tile - 1;
+            long spins = 0;
+            while (true) {
+                const int q = p - lane;
+                u64 w = q >= 0 ? __hip_atomic_load(&status[q], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)
+                               : scan_word(epoch, 2, 0);
+                const u32 fl = (u32)(w >> 46) & 3u;
+                const bool ready = (u32)(w >> 48) == epoch && fl != 0;
+                const unsigned long long incl = __ballot(ready && fl == 2);
+                const unsigned long long notready = __ballot(!ready);
+                const int k = incl ? __ffsll((long long)incl) - 1 : 63;
+                const unsigned long long need = k == 63 ? ~0ull : ((2ull << k) - 1ull);
+                if (notready & need) {
+                    if (++spins > kScanSpinLimit) {
+                        if (lane == 0) ctl[1] = 1;
+                        break;
+                    }
+                    __builtin_amdgcn_s_sleep(1);
+                    continue;
+                }
+                T x = lane <= k ? (T)(w & kScanValMask) : T(0);
+#pragma unroll
+                for (int d = 32; d >= 1; d >>= 1) x += shfl_down64(x, d);
+                x = __shfl(x, 0, 64);
+                prefix += x;
+                if (incl) break;
+                p -= 64;
+            }
+            if (lane == 0)
+                __hip_atomic_store(&status[tile], scan_word(epoch, 2, (u64)(prefix + agg)), __ATOMIC_RELAXED,
+                                   __HIP_MEMORY_SCOPE_AGENT);
+        }
+        if (lane == 0) {
+            prefix_s = prefix;
+            if (tile == (int)gridDim.x - 1) {
+                if (total) *total = prefix + agg;
+                ctl[0] = 0;  // every block has drawn its tile: reset for the next launch
+            }
+        }
+    }
+    __syncthreads();
+    T run = prefix_s + excl;
 #pragma unroll
     for (int k = 0; k < kScanItems; ++k) {
         const i64 i = base + k;
-        if (i < n) out[i] = run;
+        if (i < n) out(i, run, v[k]);
         run += v[k];
     }
-    if (blockIdx.x == gridDim.x - 1 && threadIdx.x == kBlock - 1) out[n] = boff + tot;
 }
 
 template <class T>
@@ -146,31 +207,50 @@ struct LoadFn {
     __device__ __forceinline__ T operator()(i64 i) const { return p[i]; }
 };
 
+template <class T>
+struct StoreExcl {
+    T* p;
+    __device__ __forceinline__ void operator()(i64 i, T run, T) const { p[i] = run; }
+};
+
+// Generic single-pass scan: out(i, prefix, value) for all i < n; *total on device.
+template <class T, class F, class Out>
+void scan_apply(Ctx& ctx, F f, i64 n, Out out, T* total) {
+    if (n <= 0) {
+        if (total) ctx.zero(total, 1);
+        return;
+    }
+    const i64 nt = (n + kScanTile - 1) / kScanTile;
+    RAMA_REQUIRE(nt < ((i64)1 << 31), "scan too large");
+    u32 epoch;
+    u64* st = ctx.scan_begin(nt, &epoch);
+    k_scan_single<T, F, Out><<<(unsigned)nt, kBlock, 0, ctx.stream>>>(f, n, out, total, st, ctx.scan_counter.get(),
+                                                                      epoch);
+    RAMA_LAUNCHED();
+}
+
 // Exclusive scan of f(0..n-1) into out[0..n]; out[n] = total.  `out` may
-// alias the storage f reads (each tile is read completely before written).
+// alias the storage f reads (each tile reads all its items before writing).
 template <class T, class F>
 void scan_exclusive(Ctx& ctx, F f, T* out, i64 n) {
     if (n <= 0) {
         ctx.zero(out, 1);
         return;
     }
-    const i64 nb = (n + kScanTile - 1) / kScanTile;
-    if (nb == 1) {
-        k_scan_final<T, F><<<1, kBlock, 0, ctx.stream>>>(f, n, (const T*)nullptr, out);
-        RAMA_LAUNCHED();
-        return;
-    }
-    DBuf<T> part = ctx.buf<T>(nb + 1);
-    k_scan_partials<T, F><<<(unsigned)nb, kBlock, 0, ctx.stream>>>(f, n, part.get());
-    RAMA_LAUNCHED();
-    scan_exclusive<T>(ctx, LoadFn<T>{part.get()}, part.get(), nb);
-    k_scan_final<T, F><<<(unsigned)nb, kBlock, 0, ctx.stream>>>(f, n, part.get(), out);
-    RAMA_LAUNCHED();
+    scan_apply<T>(ctx, f, n, StoreExcl<T>{out}, out + n);
 }
 
 template <class T>
 void scan_exclusive_arr(Ctx& ctx, const T* in, T* out, i64 n) {
     scan_exclusive<T>(ctx, LoadFn<T>{in}, out, n);
+}
+
+// Raises if any look-back gave up (never expected; guards against a hang).
+inline void scan_check(Ctx& ctx) {
+    if (ctx.scan_counter.empty()) return;
+    int h[2];
+    ctx.read(ctx.scan_counter.get(), h, 2);
+    if (h[1]) throw Error(RAMA_ERR_INTERNAL, "single-pass scan look-back timed out");
 }
 
 // ------------------------------------------------------------- reduction
@@ -244,30 +324,49 @@ void for_each(Ctx& ctx, i64 n, F f) {
     RAMA_LAUNCHED();
 }
 
+template <class P>
+struct FlagFn {
+    P pred;
+    __device__ __forceinline__ int operator()(i64 i) const { return pred(i) ? 1 : 0; }
+};
+
 // Stable compaction helper: pos[i] = #{k < i : pred(k)}, returns the count.
 template <class P>
 i64 select_positions(Ctx& ctx, P pred, i64 n, DBuf<int>& pos) {
     pos = ctx.buf<int>(n + 1);
-    scan_exclusive<int>(ctx, pred, pos.get(), n);
+    scan_exclusive<int>(ctx, FlagFn<P>{pred}, pos.get(), n);
     return ctx.read1(pos.get() + n);
+}
+
+template <class Emit>
+struct SelectOut {
+    Emit emit;
+    __device__ __forceinline__ void operator()(i64 i, int pos, int flag) const {
+        if (flag) emit(i, pos);
+    }
+};
+
+// Stable compaction in one launch: emit(i, position) for every i with
+// pred(i); the number selected is left on the device in *d_count.
+template <class P, class Emit>
+void select_into(Ctx& ctx, i64 n, P pred, Emit emit, int* d_count) {
+    scan_apply<int>(ctx, FlagFn<P>{pred}, n, SelectOut<Emit>{emit}, d_count);
 }
 
 // ------------------------------------------------------------ radix sort
 constexpr int kRsItems = 16;
 constexpr int kRsTile = kBlock * kRsItems;  // 4096 keys per tile
 
-__global__ __launch_bounds__(kBlock) void k_radix_hist(const u64* __restrict__ keys, i64 n, int shift, int* hist,
-                                                       int ntiles);
-__global__ __launch_bounds__(kBlock) void k_radix_scatter(const u64* __restrict__ kin, const u32* __restrict__ vin,
-                                                          u64* __restrict__ kout, u32* __restrict__ vout, i64 n,
-                                                          int shift, const int* __restrict__ offs, int ntiles);
-
-// Stable LSD sort of (keys, vals) by the low `bits` bits of the key.
-// `keys`/`vals` hold the result; `tk`/`tv` are scratch of the same length.
-void radix_sort_pairs(Ctx& ctx, u64* keys, u32* vals, u64* tk, u32* tv, i64 n, int bits);
+// Stable LSD sort of (keys, vals) by the low `bits` bits of the key; K is
+// uint32_t or uint64_t.  `keys`/`vals` hold the result; `tk`/`tv` are scratch
+// of the same length.  Digits are balanced: ceil(bits/8) passes of at most
+// 8 bits each.
+template <class K>
+void radix_sort_pairs(Ctx& ctx, K* keys, u32* vals, K* tk, u32* tv, i64 n, int bits);
 
 // Convenience: allocates the scratch itself.
-void radix_sort_pairs(Ctx& ctx, DBuf<u64>& keys, DBuf<u32>& vals, i64 n, int bits);
+template <class K>
+void radix_sort_pairs(Ctx& ctx, DBuf<K>& keys, DBuf<u32>& vals, i64 n, int bits);
 
 inline int bits_for(i64 max_value) {
     int b = 1;

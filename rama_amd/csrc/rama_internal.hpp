@@ -99,6 +99,7 @@ double mp_lower_bound(Ctx& ctx, MPState& S);
 double mp_fused_lower_bound(Ctx& ctx, MPState& S);
 void mp_send_to_triplets(Ctx& ctx, MPState& S);
 void mp_iteration(Ctx& ctx, MPState& S);
+double mp_run(Ctx& ctx, MPState& S, int steps);
 DCoo mp_reparametrized(Ctx& ctx, MPState& S);
 
 // ---- dual.hip

@@ -3,10 +3,38 @@
 // dual, pick contractions by matching or by the conflict-free spanning
 // forest, contract, compose labels) until nothing is left to contract.
 #include <chrono>
+#include <cstdlib>
+#include <map>
 
 #include "rama_internal.hpp"
 
 namespace rama {
+
+namespace {
+// Wall time per driver stage (stream synchronised at each boundary), printed
+// when RAMA_STAGE_TIMES is set.  Diagnostic only.
+struct StageTimes {
+    bool on = std::getenv("RAMA_STAGE_TIMES") != nullptr;
+    std::map<std::string, std::pair<double, int>> acc;
+    std::chrono::steady_clock::time_point t0;
+    void begin(Ctx& ctx) {
+        if (!on) return;
+        ctx.sync();
+        t0 = std::chrono::steady_clock::now();
+    }
+    void end(Ctx& ctx, const char* name) {
+        if (!on) return;
+        ctx.sync();
+        auto& a = acc[name];
+        a.first += std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+        a.second += 1;
+    }
+    ~StageTimes() {
+        if (!on) return;
+        for (auto& kv : acc) std::fprintf(stderr, "stage %-16s %9.3f ms  %d calls\n", kv.first.c_str(), kv.second.first, kv.second.second);
+    }
+};
+}  // namespace
 
 Options to_options(const rama_options_c& c) {
     Options o;
@@ -28,25 +56,34 @@ Options to_options(const rama_options_c& c) {
 
 SolveResult solve(Ctx& ctx, DCoo& A, const Options& o) {
     SolveResult out;
+    StageTimes st;
     const auto t0 = std::chrono::steady_clock::now();
+    st.begin(ctx);
     out.lb = dual_solver(ctx, A, o.max_cycle_length_lb, o.num_dual_itr_lb, o.tri_memory_factor, o.num_outer_itr_dual,
                          1e-4f);
+    st.end(ctx, "dual_initial");
     const int N = A.max_dim();
     out.labels = ctx.buf<int>(N);
     iota(ctx, out.labels.get(), N);
     out.num_labels = N;
     if (o.only_compute_lb) {
         out.num_labels = 0;
+        scan_check(ctx);
         return out;
     }
     bool use_mm = !(o.matching_thresh_crossover_ratio > 1.0f);
     for (int iter = 0; A.nnz > 0; ++iter) {
+        st.begin(ctx);
         if (iter > 0)
             dual_solver(ctx, A, o.max_cycle_length_primal, o.num_dual_itr_primal, 1.0f, 1, 1e-4f);
+        st.end(ctx, "dual_primal");
+        st.begin(ctx);
         DBuf<int> cur;
         i64 cur_size = 0;
         int n_contract = 0;
         int max_label = 0;
+        const bool mm_branch = use_mm;
+        if (st.on) std::fprintf(stderr, "round %d: %lld edges, %s\n", iter, (long long)A.nnz, use_mm ? "matching" : "forest");
         if (use_mm) {
             // export_undirected: node count = largest endpoint + 1
             const int* C = A.col.get();
@@ -67,6 +104,8 @@ SolveResult solve(Ctx& ctx, DCoo& A, const Options& o) {
             n_contract = woc_mapping(ctx, A, cur, cur_size);
             max_label = (int)cur_size - 1;
         }
+        st.end(ctx, mm_branch ? "matching" : "mst_mapping");
+        st.begin(ctx);
         if (n_contract == 0) {
             if (o.verbose) std::printf("# iterations = %d\n", iter);
             break;
@@ -81,6 +120,7 @@ SolveResult solve(Ctx& ctx, DCoo& A, const Options& o) {
         if (bad) throw Error(RAMA_ERR_BAD_CONTRACTION, "Found bad contractions");
         A = std::move(B);
         map_node_labels(ctx, cur.get(), cur_size, out.labels.get(), N);
+        st.end(ctx, "contract");
         out.iterations = iter + 1;
         if (o.dump_timeline) {
             std::vector<int> snap(N);
@@ -92,6 +132,8 @@ SolveResult solve(Ctx& ctx, DCoo& A, const Options& o) {
             if (t > o.max_time_sec) break;
         }
     }
+    scan_check(ctx);
+    if (st.on) std::fprintf(stderr, "solve: %d rounds, %lld host synchronisations\n", out.iterations, (long long)ctx.syncs);
     return out;
 }
 
