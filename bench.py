@@ -12,7 +12,9 @@ value = E / seconds per solve (PD edges/s).
 Also reported on the same JSON line:
   roofline      the dominant kernel's algorithmic HBM bytes per launch over
                 its average launch time, timed with HIP events on the
-                solver's stream over the timed steps;
+                solver's stream over the timed steps (the warm-up steps time
+                every kernel to pick the dominant one and fill `kernels`;
+                the timed steps record events around that kernel only);
   cpu_baseline  the CPU restatement (oracle, 1 thread) timed on a bounded
                 256x1024 crop of the same generator.
 """
@@ -39,6 +41,14 @@ def parse():
     ap.add_argument("--H", type=int, default=1024)
     ap.add_argument("--W", type=int, default=2048)
     ap.add_argument("--preset", default="PD")
+    ap.add_argument("--mode", choices=["plain", "batch", "serve", "copy"], default="plain",
+                    help="plain: the default line; batch: 512 mixed instances batched vs one at a time; "
+                         "serve: Poisson arrivals against the serving loop and the plain loop; "
+                         "copy: one 1 GiB device copy (calibrates HBM byte counters)")
+    ap.add_argument("--count", type=int, default=512, help="instances in batch mode")
+    ap.add_argument("--workers", type=int, default=16)
+    ap.add_argument("--requests", type=int, default=160, help="requests per offered load in serve mode")
+    ap.add_argument("--kernels", type=int, default=8, help="profiled kernels listed on the line")
     return ap.parse_args()
 
 
@@ -56,8 +66,137 @@ def cpu_baseline():
                       % (len(c), dt)}
 
 
+def copy_peak(torch, nbytes=2 << 30, reps=5):
+    """Measured device-to-device copy rate in GB/s (read + write bytes)."""
+    src = torch.empty(nbytes // 4, dtype=torch.float32, device="cuda").fill_(1.0)
+    dst = torch.empty_like(src)
+    dst.copy_(src)
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record()
+    for _ in range(reps):
+        dst.copy_(src)
+    e1.record()
+    torch.cuda.synchronize()
+    gbs = 2.0 * nbytes * reps / (e0.elapsed_time(e1) * 1e-3) / 1e9
+    del src, dst
+    torch.cuda.empty_cache()
+    return gbs
+
+
+def copy_mode():
+    """One copy of a 1 GiB buffer: under rocprofv3 --pmc its FETCH_SIZE and
+    WRITE_SIZE (KiB) against the known 1 GiB each give the counters' scale."""
+    import torch
+
+    src = torch.empty(1 << 28, dtype=torch.float32, device="cuda").fill_(1.0)
+    dst = torch.empty_like(src)
+    torch.cuda.synchronize()
+    dst.copy_(src)
+    torch.cuda.synchronize()
+    return {"mode": "copy", "bytes_read": src.numel() * 4, "bytes_written": dst.numel() * 4,
+            "kib_each": src.numel() * 4 / 1024}
+
+
+def batch_mode(a):
+    """512 instances (n log-uniform in [1e3, 2e5]; even: Voronoi grids, odd:
+    Erdos-Renyi), batched on one GPU vs the same instances one at a time."""
+    from rama_amd import _capi, instances
+
+    sizes = instances.batch_sizes(a.count, 1e3, 2e5, seed=12345)
+    insts = [instances.batch_instance(k, int(n))[:3] for k, n in enumerate(sizes)]
+    E = int(sum(len(x[2]) for x in insts))
+    opts = _capi.options("PD", verbose=0)
+    _capi.solve_batch(insts[:4], opts, workers=1)  # warm up the runtime
+    t = time.perf_counter()
+    one = _capi.solve_batch(insts, opts, workers=1)
+    t_one = time.perf_counter() - t
+    t = time.perf_counter()
+    bat = _capi.solve_batch(insts, opts, workers=a.workers)
+    t_bat = time.perf_counter() - t
+    same = all(np.array_equal(x[0], y[0]) and x[1] == y[1] for x, y in zip(one, bat))
+    obj = sum(_capi.objective(l, *x) for (l, _), x in zip(bat, insts))
+    return {"metric": "batched solves/s", "value": a.count / t_bat, "unit": "solves/s", "n_gpus": 1,
+            "higher_is_better": True, "dtype": "fp32", "data": "synthetic", "vs_baseline": None,
+            "config": {"workload": "%d instances, n log-uniform [1e3, 2e5], grids + Erdos-Renyi, PD" % a.count,
+                       "edges": E, "workers": a.workers},
+            "one_at_a_time_solves_per_s": a.count / t_one, "speedup": t_one / t_bat,
+            "identical_results": bool(same), "total_objective": obj,
+            "seconds": {"batched": t_bat, "one_at_a_time": t_one}}
+
+
+def serve_mode(a):
+    """Poisson arrivals (lognormal edge counts, median 5e4, sigma 1, cap 2e7)
+    at 0.5, 0.9 and 1.5 x the plain loop's capacity, against the serving loop
+    (a.workers workers, 200 GB HBM budget) and the plain one-at-a-time loop."""
+    import math
+
+    from rama_amd import _capi, instances
+
+    rng = np.random.Generator(np.random.PCG64(2024))
+    pool = []
+    for k in range(48):
+        m = int(min(2e7, round(math.exp(rng.normal(math.log(5e4), 1.0)))))
+        n = max(64, m // 2 if k % 2 == 0 else m // 4)
+        pool.append(instances.batch_instance(k, n)[:3])
+    opts = _capi.options("PD", verbose=0)
+    # capacity of the plain loop: mean solve time over the pool, one at a time
+    with _capi.Server(workers=1, opts=opts) as srv:
+        srv.wait(srv.submit(*pool[0]))
+        t = time.perf_counter()
+        for x in pool:
+            srv.wait(srv.submit(*x))
+        cap = len(pool) / (time.perf_counter() - t)
+
+    def run(workers, rate, nreq):
+        seq = rng.integers(0, len(pool), size=nreq)
+        gaps = rng.exponential(1.0 / rate, size=nreq)
+        with _capi.Server(workers=workers, hbm_budget_bytes=200 << 30, opts=opts) as srv:
+            t0 = time.perf_counter()
+            at = t0
+            tickets = []
+            for q, g in zip(seq, gaps):
+                at += g
+                d = at - time.perf_counter()
+                if d > 0:
+                    time.sleep(d)
+                tickets.append((q, srv.submit(*pool[q])))
+            lat = []
+            res = {}
+            for q, tk in tickets:
+                lab, lb, l_ms, _ = srv.wait(tk)
+                lat.append(l_ms)
+                res.setdefault(int(q), (lab, lb))
+            wall = time.perf_counter() - t0
+        lat = np.asarray(lat)
+        return {"solves_per_s": nreq / wall, "p50_ms": float(np.percentile(lat, 50)),
+                "p99_ms": float(np.percentile(lat, 99))}, res
+
+    loads = {}
+    for f in (0.5, 0.9, 1.5):
+        served, r1 = run(a.workers, f * cap, a.requests)
+        plain, r2 = run(1, f * cap, a.requests)
+        same = all(np.array_equal(r1[q][0], r2[q][0]) and r1[q][1] == r2[q][1] for q in r1 if q in r2)
+        loads[str(f)] = {"offered_per_s": f * cap, "served": served, "plain_loop": plain, "identical_results": same}
+    return {"metric": "served solves/s + p99 latency", "value": loads["1.5"]["served"]["solves_per_s"],
+            "unit": "solves/s", "n_gpus": 1, "higher_is_better": True, "dtype": "fp32", "data": "synthetic",
+            "vs_baseline": None,
+            "config": {"workload": "Poisson arrivals, lognormal edges (median 5e4, sigma 1, cap 2e7), grids + ER, PD",
+                       "workers": a.workers, "hbm_budget_GB": 200, "plain_capacity_per_s": cap},
+            "loads": loads}
+
+
 def main():
     a = parse()
+    if a.mode == "copy":
+        print(json.dumps(copy_mode()))
+        return
+    if a.mode != "plain":
+        # more hardware queues so the worker streams run side by side
+        os.environ.setdefault("GPU_MAX_HW_QUEUES", "16")
+        import torch  # noqa: F401  (one HIP runtime for the process)
+
+        print(json.dumps(batch_mode(a) if a.mode == "batch" else serve_mode(a)))
+        return
     if a.gpus != 1:
         print("bench.py measures one GPU: this build runs the solver on a single MI355X; "
               "multi-GPU runs are not part of it (see DESIGN.md for how the path would shard).", file=sys.stderr)
@@ -82,9 +221,18 @@ def main():
         return _capi.solve_device(di.data_ptr(), dj.data_ptr(), dc.data_ptr(), E, labels.data_ptr(), labels.numel(),
                                   opts, device=0, stream=stream)
 
-    for _ in range(a.warmup):
+    # warm-up steps with every kernel timed: the per-kernel table and the
+    # choice of the dominant kernel; the timed steps then record events
+    # around that kernel's launches only
+    profiler.only(None)
+    profiler.enable(True)
+    for _ in range(max(a.warmup, 1)):
         step()
     torch.cuda.synchronize()
+    all_stats = profiler.read()
+    profiler.enable(False)
+    dom = profiler.dominant(all_stats)
+    profiler.only(dom)
     profiler.enable(True)
     torch.cuda.synchronize()
     t0 = time.perf_counter()
@@ -94,17 +242,19 @@ def main():
     dt = (time.perf_counter() - t0) / a.steps
     stats = profiler.read()
     profiler.enable(False)
+    profiler.only(None)
 
     lab = labels[:n_lab].cpu().numpy()
     obj = _capi.objective(lab, i, j, c)
-    roof = profiler.roofline(stats, HBM_PEAK_GBS)
+    roof = profiler.roofline(stats, HBM_PEAK_GBS, name=dom)
+    roof["measured_copy_peak"] = copy_peak(torch)
     out = {
         "metric": "PD edges/s",
         "value": E / dt,
         "unit": "edges/s",
         "n_gpus": 1,
         "steps": a.steps,
-        "warmup": a.warmup,
+        "warmup": max(a.warmup, 1),
         "ms_per_step": dt * 1e3,
         "higher_is_better": True,
         "scaling": "weak",
@@ -115,7 +265,7 @@ def main():
                    "edges": E, "nodes": n_nodes, "mode": a.preset, "objective": obj, "lower_bound": lb,
                    "clusters": int(len(np.unique(lab)))},
         "roofline": roof,
-        "kernels": profiler.top(stats, 8),
+        "kernels": profiler.top(all_stats, a.kernels),
     }
     if not a.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline()

@@ -194,6 +194,40 @@ int rama_stage_woc_mapping(const int32_t* row, const int32_t* col, const float* 
                            int32_t* out_map, int64_t* out_map_size, int32_t* out_count);
 
 /* ------------------------------------------------------------------------
+ * Many instances on one GPU (no counterpart in the reference, which solves
+ * one instance per call).  Every instance is solved by the plain solver, so
+ * results are identical to rama_solve_host on the same instance; a pool of
+ * worker threads, each with its own stream, overlaps the solves on the
+ * device.  For more than four concurrent streams set GPU_MAX_HW_QUEUES
+ * (HIP's hardware-queue count) before the first HIP call in the process.
+ * ------------------------------------------------------------------------ */
+
+/* Batched solve: instance q owns edges [edge_off[q], edge_off[q+1]) of
+ * (i, j, c), with its own 0-based node ids, and receives n_labels[q] labels
+ * at labels_out + label_off[q] (capacity label_off[q+1] - label_off[q]) and
+ * its lower bound in lbs[q]; status[q] (may be NULL) its status. */
+int rama_solve_batch_host(int32_t k, const int64_t* edge_off, const int32_t* i, const int32_t* j, const float* c,
+                          const rama_options_c* opts, int32_t workers, int32_t* labels_out, const int64_t* label_off,
+                          int64_t* n_labels, double* lbs, int32_t* status);
+
+/* Serving loop with continuous admission: requests are queued in arrival
+ * order and admitted by `workers` device workers while the device-memory
+ * estimate of the requests in flight stays within hbm_budget_bytes; each
+ * request's graph and dual state stay resident on the device until its
+ * labels are back on the host. */
+typedef struct rama_server rama_server;
+rama_server* rama_server_create(int device, int workers, int64_t hbm_budget_bytes, const rama_options_c* opts);
+/* Copies the request; returns its ticket (>= 0) or -1. */
+int64_t rama_server_submit(rama_server* s, const int32_t* i, const int32_t* j, const float* c, int64_t m);
+/* 1 if the ticket's result is ready, 0 if not, -1 if unknown. */
+int rama_server_poll(rama_server* s, int64_t ticket);
+/* Blocks until the ticket is done; returns its status, labels, lower bound,
+ * latency (submission to labels on the host) and device solve time. */
+int rama_server_wait(rama_server* s, int64_t ticket, int32_t* labels_out, int64_t cap, int64_t* n_labels,
+                     double* lb, double* latency_ms, double* solve_ms);
+void rama_server_destroy(rama_server* s);
+
+/* ------------------------------------------------------------------------
  * Kernel timing (no counterpart in the reference; its cumulative wall-clock
  * macros, include/time_measure_util.h:11-70, are the closest).  When enabled,
  * instrumented launches are bracketed by HIP events on the solver's stream
@@ -204,6 +238,17 @@ void rama_profile_enable(int on);
 void rama_profile_reset(void);
 int64_t rama_profile_count(void);
 int rama_profile_entry(int64_t k, const char** name, int64_t* launches, double* total_ms, double* total_bytes);
+/* Restrict timing to launches of one kernel name (NULL or "": all). */
+void rama_profile_only(const char* name);
+
+/* ------------------------------------------------------------------------
+ * Device memory cache (no counterpart in the reference, whose thrust
+ * vectors allocate and free per call).  Blocks a finished solve used stay
+ * cached for the next solve on the same device, up to RAMA_CACHE_BYTES
+ * (default 32 GiB) per device.  This returns them to the driver; device < 0
+ * releases every device's cache.  Call it with no solve running.
+ * ------------------------------------------------------------------------ */
+void rama_release_cached(int device);
 
 #ifdef __cplusplus
 }

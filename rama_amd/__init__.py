@@ -13,6 +13,6 @@ except ImportError:
     pass
 
 from . import _capi  # noqa: F401,E402
-from ._capi import RamaError, objective, options, solve, solve_device  # noqa: F401,E402
+from ._capi import RamaError, Server, objective, options, solve, solve_batch, solve_device  # noqa: F401,E402
 
-__all__ = ["RamaError", "options", "solve", "solve_device", "objective"]
+__all__ = ["RamaError", "options", "solve", "solve_device", "solve_batch", "Server", "objective"]

@@ -73,7 +73,15 @@ SIGNATURES = {
     "rama_stage_connected_components": ([I32P, I32P, I32, I64, I32P], I32),
     "rama_stage_max_spanning_forest": ([I32P, I32P, F32P, I64, I32P, I32P, F32P, I64P], I32),
     "rama_stage_woc_mapping": ([I32P, I32P, F32P, I64, I32, I32, I32P, I64P, I32P], I32),
+    "rama_solve_batch_host": ([I32, I64P, I32P, I32P, F32P, OPTP, I32, I32P, I64P, I64P, F64P, I32P], I32),
+    "rama_server_create": ([I32, I32, I64, OPTP], VP),
+    "rama_server_submit": ([VP, I32P, I32P, F32P, I64], I64),
+    "rama_server_poll": ([VP, I64], I32),
+    "rama_server_wait": ([VP, I64, I32P, I64, I64P, F64P, F64P, F64P], I32),
+    "rama_server_destroy": ([VP], None),
     "rama_profile_enable": ([I32], None),
+    "rama_profile_only": ([ctypes.c_char_p], None),
+    "rama_release_cached": ([I32], None),
     "rama_profile_reset": ([], None),
     "rama_profile_count": ([], I64),
     "rama_profile_entry": ([I64, ctypes.POINTER(ctypes.c_char_p), I64P, F64P, F64P], I32),
@@ -303,3 +311,84 @@ def woc_mapping(row, col, val, rows, cols):
     _check(lib().rama_stage_woc_mapping(_p(row, I32P), _p(col, I32P), _p(val, F32P), len(val), rows, cols,
                                         _p(out, I32P), ctypes.byref(n), ctypes.byref(cnt)))
     return out[: n.value].copy(), cnt.value
+
+
+# ------------------------------------------------------- batch and serving
+
+def solve_batch(instances, opts=None, workers=8):
+    """Solve a list of (i, j, c) instances concurrently on one GPU.
+    Returns a list of (labels, lb)."""
+    opts = opts or options()
+    k = len(instances)
+    edge_off = np.zeros(k + 1, np.int64)
+    label_off = np.zeros(k + 1, np.int64)
+    for q, (i, j, c) in enumerate(instances):
+        edge_off[q + 1] = edge_off[q] + len(c)
+        label_off[q + 1] = label_off[q] + int(max(np.max(i), np.max(j))) + 2
+    I = _i32(np.concatenate([np.asarray(x[0]) for x in instances])) if k else np.zeros(0, np.int32)
+    J = _i32(np.concatenate([np.asarray(x[1]) for x in instances])) if k else np.zeros(0, np.int32)
+    C = _f32(np.concatenate([np.asarray(x[2]) for x in instances])) if k else np.zeros(0, np.float32)
+    labels = np.zeros(max(1, int(label_off[-1])), np.int32)
+    n = np.zeros(max(1, k), np.int64)
+    lbs = np.zeros(max(1, k), np.float64)
+    st = np.zeros(max(1, k), np.int32)
+    _check(lib().rama_solve_batch_host(k, _p(edge_off, I64P), _p(I, I32P), _p(J, I32P), _p(C, F32P),
+                                       ctypes.byref(opts), workers, _p(labels, I32P), _p(label_off, I64P),
+                                       _p(n, I64P), _p(lbs, F64P), _p(st, I32P)))
+    return [(labels[label_off[q]: label_off[q] + n[q]].copy(), float(lbs[q])) for q in range(k)]
+
+
+class Server:
+    """Serving loop (rama_server_*): submit instances as they arrive, wait for
+    results; admission keeps the in-flight device-memory estimate within the
+    HBM budget."""
+
+    def __init__(self, workers=8, hbm_budget_bytes=200 << 30, opts=None, device=0):
+        self._opts = opts or options(verbose=0)
+        self._h = lib().rama_server_create(device, workers, hbm_budget_bytes, ctypes.byref(self._opts))
+        if not self._h:
+            raise RamaError(RAMA_ERR_INTERNAL_STATUS, lib().rama_last_error().decode())
+        self._caps = {}
+
+    def submit(self, i, j, c):
+        i, j, c = _i32(i), _i32(j), _f32(c)
+        t = lib().rama_server_submit(self._h, _p(i, I32P), _p(j, I32P), _p(c, F32P), len(c))
+        if t < 0:
+            raise RamaError(1, lib().rama_last_error().decode())
+        self._caps[t] = int(max(i.max(initial=0), j.max(initial=0))) + 2
+        return t
+
+    def done(self, ticket):
+        return lib().rama_server_poll(self._h, ticket) == 1
+
+    def wait(self, ticket):
+        """-> (labels, lb, latency_ms, solve_ms)"""
+        cap = self._caps.pop(ticket)
+        lab = np.zeros(cap, np.int32)
+        n = ctypes.c_int64()
+        lb = ctypes.c_double()
+        lat = ctypes.c_double()
+        sms = ctypes.c_double()
+        _check(lib().rama_server_wait(self._h, ticket, _p(lab, I32P), cap, ctypes.byref(n), ctypes.byref(lb),
+                                      ctypes.byref(lat), ctypes.byref(sms)))
+        return lab[: n.value].copy(), lb.value, lat.value, sms.value
+
+    def close(self):
+        if self._h:
+            lib().rama_server_destroy(self._h)
+            self._h = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+RAMA_ERR_INTERNAL_STATUS = 6

@@ -55,7 +55,13 @@ void download(Ctx& ctx, T* h, const T* d, i64 n) {
     ctx.d2h(h, d, n);
 }
 
-// Shared body of the two solve entry points; `i`, `j`, `c` are device buffers
+}  // namespace
+
+namespace rama {
+
+void set_last_error(const std::string& s) { g_last_error = s; }
+
+// Shared body of the solve entry points; `i`, `j`, `c` are device buffers
 // owned by this call.
 void solve_common(Ctx& ctx, DBuf<int>& i, DBuf<int>& j, DBuf<float>& c, i64 m, const rama_options_c* copts,
                   DBuf<int>& labels, i64& n_labels, double* lb_out, int32_t* time_ms, rama_timeline** tl) {
@@ -88,7 +94,7 @@ void solve_common(Ctx& ctx, DBuf<int>& i, DBuf<int>& j, DBuf<float>& c, i64 m, c
     }
 }
 
-}  // namespace
+}  // namespace rama
 
 extern "C" {
 

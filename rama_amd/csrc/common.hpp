@@ -53,6 +53,20 @@ inline unsigned grid_for(i64 n, int block = 256) {
     return (unsigned)g;
 }
 
+// Process-wide cache of device blocks that finished solves handed back
+// (their streams were synchronised first), so the next solve on the device
+// reuses them instead of calling hipMalloc/hipFree.  Bounded by
+// RAMA_CACHE_BYTES (default 32 GiB per device); rama_release_cached() and
+// allocation failures empty it.
+namespace pool {
+constexpr size_t kPinnedBytes = 4096;
+void* take_pinned();
+void give_pinned(void* p);
+void* take(int device, size_t cls);
+void give(int device, size_t cls, void* p);  // frees when over the bound
+void release(int device);                   // device < 0: every device
+}  // namespace pool
+
 // Caching device allocator.  Blocks are rounded to a size class (1/8 steps
 // between powers of two) and recycled on the owning stream's order: a block
 // released while kernels that use it are queued is only handed to later work
@@ -72,11 +86,14 @@ class Arena {
         if (it != free_.end()) {
             p = it->second;
             free_.erase(it);
+        } else if ((p = pool::take(device_, cls)) != nullptr) {
+            reserved_ += cls;
         } else {
             hipError_t e = hipMalloc(&p, cls);
             if (e != hipSuccess) {
                 (void)hipGetLastError();
-                trim();
+                free_blocks();
+                pool::release(device_);
                 e = hipMalloc(&p, cls);
             }
             if (e != hipSuccess) {
@@ -100,8 +117,18 @@ class Arena {
         used_.erase(it);
     }
 
-    // Return cached blocks to the driver.
+    // Hand cached blocks to the process-wide cache.  Only call once the
+    // stream that used them has been synchronised.
     void trim() {
+        for (auto& kv : free_) {
+            pool::give(device_, kv.first, kv.second);
+            reserved_ -= kv.first;
+        }
+        free_.clear();
+    }
+
+    // Return cached blocks to the driver.
+    void free_blocks() {
         for (auto& kv : free_) {
             (void)hipFree(kv.second);
             reserved_ -= kv.first;
@@ -183,12 +210,12 @@ struct Ctx {
             RAMA_HIP(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
             own_stream = true;
         }
-        RAMA_HIP(hipHostMalloc(&pinned, 4096, hipHostMallocDefault));
+        pinned = pool::take_pinned();
     }
     ~Ctx() {
         if (stream) (void)hipStreamSynchronize(stream);
         arena.trim();
-        if (pinned) (void)hipHostFree(pinned);
+        pool::give_pinned(pinned);
         if (own_stream) (void)hipStreamDestroy(stream);
     }
     Ctx(const Ctx&) = delete;
@@ -205,6 +232,20 @@ struct Ctx {
     DBuf<uint64_t> scan_status;
     DBuf<int> scan_counter;  // [0] tile counter, [1] error flag
     uint32_t scan_epoch = 0;
+    // Reduction partials and the ticket counter (reset by the last block).
+    DBuf<uint64_t> red_partials;
+    DBuf<unsigned> red_ticket;
+    void* reduce_scratch(size_t bytes) {
+        if ((size_t)red_partials.size() * 8 < bytes) red_partials = DBuf<uint64_t>(arena, (int64_t)(bytes / 8 + 1));
+        return red_partials.get();
+    }
+    unsigned* reduce_ticket() {
+        if (red_ticket.empty()) {
+            red_ticket = DBuf<unsigned>(arena, 1);
+            zero(red_ticket.get(), 1);
+        }
+        return red_ticket.get();
+    }
     // Returns the status buffer (at least `tiles` words) and the epoch to use.
     uint64_t* scan_begin(int64_t tiles, uint32_t* epoch) {
         if (scan_counter.empty()) {
@@ -233,6 +274,7 @@ struct Ctx {
     // Read `n` device scalars (contiguous) into host memory (synchronises).
     template <class T>
     void read(const T* d, T* h, int n = 1) {
+        RAMA_REQUIRE(sizeof(T) * (size_t)n <= pool::kPinnedBytes, "scalar read-back too large");
         RAMA_HIP(hipMemcpyAsync(pinned, d, sizeof(T) * n, hipMemcpyDeviceToHost, stream));
         sync();
         std::memcpy(h, pinned, sizeof(T) * n);

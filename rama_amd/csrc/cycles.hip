@@ -192,7 +192,7 @@ static i64 run_stage(Ctx& ctx, const Csr& G, const int* rv1, const int* rv2, i64
             },
             xoff.get(), R);
     }
-    const i64 X = ctx.read1(xoff.get() + R);
+    const i64 X = S == 3 ? R : ctx.read1(xoff.get() + R);
     if (ctx.verbose && S > 3) std::printf("%d-cycles: number of expansions: %lld\n", S, (long long)X);
     if (X == 0) return 0;
     const i64 ntiles = (X + kCycTile - 1) / kCycTile;
@@ -227,68 +227,69 @@ static i64 rearrange(Ctx& ctx, Triangles& T, i64 valid, int n_nodes) {
     int* A = T.t1.get();
     int* B = T.t2.get();
     int* C = T.t3.get();
-    DBuf<int> pos;
-    const i64 k = select_positions(
-        ctx, [=] __device__(i64 t) { return (int)(A[t] != B[t] && A[t] != C[t] && B[t] != C[t]); }, valid, pos);
-    if (k == 0) return 0;
     const int kb = bits_for(n_nodes > 1 ? n_nodes - 1 : 1);
-    DBuf<u64> keys = ctx.buf<u64>(k);
-    DBuf<u32> vals = ctx.buf<u32>(k);
-    u64* K = keys.get();
-    u32* V = vals.get();
-    const int* P = pos.get();
-    // order the three ids (write_triangle already does; kept for foreign input)
-    DBuf<int> s1 = ctx.buf<int>(k), s2 = ctx.buf<int>(k), s3 = ctx.buf<int>(k);
+    // order the three ids (write_triangle already does; kept for foreign
+    // input); a triangle with a repeated vertex gets the all-ones key of the
+    // sort width so it sorts after every real triangle and is cut below
+    DBuf<int> s1 = ctx.buf<int>(valid), s2 = ctx.buf<int>(valid), s3 = ctx.buf<int>(valid);
     int* S1 = s1.get();
     int* S2 = s2.get();
     int* S3 = s3.get();
+    DBuf<int> cnt = ctx.buf<int>(2);  // [0] non-degenerate, [1] unique
+    int* CNT = cnt.get();
     for_each(ctx, valid, [=] __device__(i64 t) {
         const int a = A[t], b = B[t], c = C[t];
-        if (a == b || a == c || b == c) return;
-        const i64 q = P[t];
-        S1[q] = min(a, min(b, c));
-        S2[q] = max(min(a, b), min(max(a, b), c));
-        S3[q] = max(a, max(b, c));
+        const bool degen = a == b || a == c || b == c;
+        S1[t] = degen ? -1 : min(a, min(b, c));
+        S2[t] = degen ? -1 : max(min(a, b), min(max(a, b), c));
+        S3[t] = degen ? -1 : max(a, max(b, c));
     });
+    reduce_to<int>(ctx, [=] __device__(i64 t) { return (int)(S1[t] >= 0); }, valid, 0, OpSum(), CNT);
+    DBuf<u64> keys = ctx.buf<u64>(valid);
+    DBuf<u32> vals = ctx.buf<u32>(valid);
+    u64* K = keys.get();
+    u32* V = vals.get();
     if (3 * kb <= 64) {
-        for_each(ctx, k, [=] __device__(i64 q) {
-            K[q] = ((u64)(u32)S1[q] << (2 * kb)) | ((u64)(u32)S2[q] << kb) | (u64)(u32)S3[q];
+        const u64 pad = 3 * kb == 64 ? ~0ull : ((1ull << (3 * kb)) - 1ull);
+        for_each(ctx, valid, [=] __device__(i64 q) {
+            K[q] = S1[q] < 0 ? pad
+                             : ((u64)(u32)S1[q] << (2 * kb)) | ((u64)(u32)S2[q] << kb) | (u64)(u32)S3[q];
             V[q] = (u32)q;
         });
-        radix_sort_pairs(ctx, keys, vals, k, 3 * kb);
+        prof::Site site_("triangles");
+        radix_sort_pairs(ctx, keys, vals, valid, 3 * kb);
     } else {
         // two stable passes: by third id, then by (first, second)
-        for_each(ctx, k, [=] __device__(i64 q) {
-            K[q] = (u64)(u32)S3[q];
+        const u64 pad1 = (1ull << kb) - 1ull, pad2 = (1ull << (2 * kb)) - 1ull;
+        for_each(ctx, valid, [=] __device__(i64 q) {
+            K[q] = S3[q] < 0 ? pad1 : (u64)(u32)S3[q];
             V[q] = (u32)q;
         });
-        radix_sort_pairs(ctx, keys, vals, k, kb);
-        for_each(ctx, k, [=] __device__(i64 q) {
+        prof::Site site_("triangles");
+        radix_sort_pairs(ctx, keys, vals, valid, kb);
+        for_each(ctx, valid, [=] __device__(i64 q) {
             const u32 o = V[q];
-            K[q] = ((u64)(u32)S1[o] << kb) | (u64)(u32)S2[o];
+            K[q] = S1[o] < 0 ? pad2 : ((u64)(u32)S1[o] << kb) | (u64)(u32)S2[o];
         });
-        radix_sort_pairs(ctx, keys, vals, k, 2 * kb);
+        radix_sort_pairs(ctx, keys, vals, valid, 2 * kb);
     }
-    DBuf<int> upos;
-    const i64 u = select_positions(
-        ctx,
+    // first `real` sorted slots hold the non-degenerate triangles; keep uniques
+    select_into(
+        ctx, valid,
         [=] __device__(i64 q) {
-            if (q == 0) return 1;
+            if (q >= CNT[0]) return false;
+            if (q == 0) return true;
             const u32 a = V[q], b = V[q - 1];
-            return (int)(S1[a] != S1[b] || S2[a] != S2[b] || S3[a] != S3[b]);
+            return S1[a] != S1[b] || S2[a] != S2[b] || S3[a] != S3[b];
         },
-        k, upos);
-    const int* UP = upos.get();
-    for_each(ctx, k, [=] __device__(i64 q) {
-        const u32 a = V[q];
-        if (q == 0 || S1[a] != S1[V[q - 1]] || S2[a] != S2[V[q - 1]] || S3[a] != S3[V[q - 1]]) {
-            const i64 o = UP[q];
+        [=] __device__(i64 q, int o) {
+            const u32 a = V[q];
             A[o] = S1[a];
             B[o] = S2[a];
             C[o] = S3[a];
-        }
-    });
-    return u;
+        },
+        CNT + 1);
+    return ctx.read1(CNT + 1);
 }
 
 void normalize_triangles(Ctx& ctx, Triangles& T) { T.n = rearrange(ctx, T, T.n, 1 << 30); }
@@ -300,26 +301,42 @@ Triangles conflicted_cycles(Ctx& ctx, const DCoo& A, int max_len, float factor, 
     if (max_len > 5) throw Error(RAMA_ERR_INVALID, "max_cycle_length should be <= 5. Received: " + std::to_string(max_len));
     if (max_len < 3 || A.nnz == 0) return T;
     const float* W = A.val.get();
-    const float thr = dcoo_min(ctx, W, A.nnz) * tol;  // determine_threshold (:170-174)
     const int N = A.max_dim();
-    // attractive: c > -thr (is_positive_edge); repulsive: c < thr (is_neg_edge)
-    Csr G = symmetric_csr(ctx, A.row.get(), A.col.get(), W, A.nnz, N, true, -1.0f * thr);
-    DBuf<int> rpos;
-    const i64 R = select_positions(ctx, [=] __device__(i64 e) { return (int)(W[e] < thr); }, A.nnz, rpos);
-    if (G.nnz == 0 || R == 0) return T;
+    // determine_threshold (:170-174): thr = min(c) * tol; attractive c > -thr
+    // (is_positive_edge), repulsive c < thr (is_neg_edge).  The minimum and
+    // both class sizes come back in one read.
+    struct Sizes {
+        float mn;
+        float pad;
+        i64 pos;
+        i64 rep;
+    };
+    DBuf<Sizes> sz = ctx.buf<Sizes>(1);
+    Sizes* SZ = sz.get();
+    reduce_to<float>(ctx, [=] __device__(i64 e) { return W[e]; }, A.nnz, 3.402823466e38f, OpMin(), &SZ->mn);
+    reduce_to<i64>(ctx, [=] __device__(i64 e) { return (i64)(W[e] > -1.0f * (SZ->mn * tol)); }, A.nnz, (i64)0,
+                   OpSum(), &SZ->pos);
+    reduce_to<i64>(ctx, [=] __device__(i64 e) { return (i64)(W[e] < SZ->mn * tol); }, A.nnz, (i64)0, OpSum(),
+                   &SZ->rep);
+    Sizes h;
+    ctx.read(SZ, &h, 1);
+    const float thr = h.mn * tol;
+    const i64 R = h.rep;
+    if (h.pos == 0 || R == 0) return T;
+    Csr G = symmetric_csr(ctx, A.row.get(), A.col.get(), W, A.nnz, N, true, -1.0f * thr, h.pos);
     DBuf<int> rv1 = ctx.buf<int>(R), rv2 = ctx.buf<int>(R);
     {
         int* X1 = rv1.get();
         int* X2 = rv2.get();
-        const int* P = rpos.get();
         const int* AR = A.row.get();
         const int* AC = A.col.get();
-        for_each(ctx, A.nnz, [=] __device__(i64 e) {
-            if (W[e] < thr) {
-                X1[P[e]] = AR[e];
-                X2[P[e]] = AC[e];
-            }
-        });
+        select_into(
+            ctx, A.nnz, [=] __device__(i64 e) { return W[e] < thr; },
+            [=] __device__(i64 e, int q) {
+                X1[q] = AR[e];
+                X2[q] = AC[e];
+            },
+            reinterpret_cast<int*>(&SZ->pad));
     }
     const i64 budget = (i64)(int)(factor * (float)R);
     T.t1 = ctx.buf<int>(budget > 0 ? budget : 1);
@@ -339,6 +356,7 @@ Triangles conflicted_cycles(Ctx& ctx, const DCoo& A, int max_len, float factor, 
     }
     T.raw = count < budget ? count : budget;
     T.n = rearrange(ctx, T, T.raw, N);
+    T.max_id = N - 1;
     return T;
 }
 

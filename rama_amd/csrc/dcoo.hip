@@ -47,6 +47,7 @@ static void derive_dims(Ctx& ctx, DCoo& A) {
     ctx.read(tmp.get(), h, 2);
     A.cols = h[0] + 1;
     A.rows = h[1] + 1;
+    A.max_node = h[0];  // rows are below their cols
 }
 
 // dCOO(col, row, data, directed=true) + init (src/dCOO.cu:28-55): orient
@@ -76,6 +77,7 @@ DCoo dcoo_from_edges(Ctx& ctx, const int* row_in, const int* col_in, const float
         V[e] = f2u(val_in[e]);
     });
     RAMA_REQUIRE(ctx.read1(badp) == 0, "node ids must be non-negative");
+    prof::Site site_("init");
     radix_sort_pairs(ctx, keys, vals, m, 2 * kb);
     int* R = A.row.get();
     int* C = A.col.get();
@@ -107,6 +109,7 @@ DCoo dcoo_copy(Ctx& ctx, const DCoo& A) {
     B.nnz = A.nnz;
     B.rows = A.rows;
     B.cols = A.cols;
+    B.max_node = A.max_node;
     B.row = ctx.buf<int>(A.nnz);
     B.col = ctx.buf<int>(A.nnz);
     B.val = ctx.buf<float>(A.nnz);
@@ -119,12 +122,14 @@ DCoo dcoo_copy(Ctx& ctx, const DCoo& A) {
 // Per-segment reduction of the sorted contraction keys: one thread per
 // (row, col) segment sums its costs in double in sorted order and rounds
 // once; self-loops are flagged (negative -> bad contraction) and dropped.
+// The segment count S is read from the device; the grid covers the bound n.
 __global__ __launch_bounds__(kBlock) void k_contract_reduce(const u64* __restrict__ K, const u32* __restrict__ V,
-                                                           const int* __restrict__ seg_start, i64 S, i64 n, int kb,
-                                                           u64* __restrict__ okey, float* __restrict__ oval,
+                                                           const int* __restrict__ seg_start, const int* dS, i64 n,
+                                                           int kb, u64* __restrict__ okey, float* __restrict__ oval,
                                                            int* __restrict__ keep, int* bad, double* diag) {
     __shared__ double ws[kWaves];
     __shared__ int wbad[kWaves];
+    const i64 S = *dS;
     const i64 s = (i64)blockIdx.x * blockDim.x + threadIdx.x;
     double dsum = 0.0;
     int isbad = 0;
@@ -156,6 +161,7 @@ __global__ __launch_bounds__(kBlock) void k_contract_reduce(const u64* __restric
 
 // dCOO::contract_cuda + has_bad_contractions + remove_diagonal
 // (src/dCOO.cu:57-95,97-142; src/rama_cuda.cu:18-30,136-140).
+// Counts stay on the device until one read-back at the end.
 DCoo contract(Ctx& ctx, const DCoo& A, const int* node_map, int max_label, bool* bad, double* diag_sum) {
     const i64 n = A.nnz;
     DCoo B;
@@ -178,68 +184,63 @@ DCoo contract(Ctx& ctx, const DCoo& A, const int* node_map, int max_label, bool*
         K[e] = ((u64)lo << kb) | hi;
         V[e] = f2u(W[e]);
     });
+    prof::Site site_("contract");
     radix_sort_pairs(ctx, keys, vals, n, 2 * kb);
 
-    DBuf<int> head_pos;
-    const i64 S = select_positions(
-        ctx, [=] __device__(i64 e) { return (int)(e == 0 || K[e] != K[e - 1]); }, n, head_pos);
-    DBuf<int> seg_start = ctx.buf<int>(S + 1);
+    // scalars on device: [0] S, [1] S2, [2] max kept col, [3] max col, [4] last row, [5] bad; diag as double
+    DBuf<int> sc = ctx.buf<int>(8);
+    DBuf<double> dg = ctx.buf<double>(1);
+    ctx.zero(sc.get(), 8);
+    ctx.zero(dg.get(), 1);
+    int* SC = sc.get();
+    DBuf<int> seg_start = ctx.buf<int>(n);
     int* SS = seg_start.get();
-    const int* HP = head_pos.get();
-    for_each(ctx, n, [=] __device__(i64 e) {
-        if (e == 0 || K[e] != K[e - 1]) SS[HP[e]] = (int)e;
-    });
-    DBuf<u64> okey = ctx.buf<u64>(S);
-    DBuf<float> oval = ctx.buf<float>(S);
-    DBuf<int> keep = ctx.buf<int>(S);
-    DBuf<double> scal = ctx.buf<double>(2);
-    ctx.zero(scal.get(), 2);
-    int* badp = reinterpret_cast<int*>(scal.get() + 1);
+    select_into(
+        ctx, n, [=] __device__(i64 e) { return e == 0 || K[e] != K[e - 1]; },
+        [=] __device__(i64 e, int pos) { SS[pos] = (int)e; }, SC + 0);
+    DBuf<u64> okey = ctx.buf<u64>(n);
+    DBuf<float> oval = ctx.buf<float>(n);
+    DBuf<int> keep = ctx.buf<int>(n);
     {
-        // 12 B in per sorted edge; seg_start in + key, cost, keep flag out per segment
-        prof::Scope ps(ctx, "contract_reduce", 12.0 * n + 20.0 * S);
-        k_contract_reduce<<<grid_for(S), kBlock, 0, ctx.stream>>>(K, V, SS, S, n, kb, okey.get(), oval.get(),
-                                                                    keep.get(), badp, scal.get());
+        // 12 B in per sorted edge; seg_start in + key, cost, keep flag out per segment (bounded by n)
+        prof::Scope ps(ctx, "contract_reduce", 32.0 * n);
+        k_contract_reduce<<<grid_for(n), kBlock, 0, ctx.stream>>>(K, V, SS, SC + 0, n, kb, okey.get(), oval.get(),
+                                                                    keep.get(), SC + 5, dg.get());
         RAMA_LAUNCHED();
     }
-    // dims before removing the diagonal (src/dCOO.cu:91-92)
     const u64 mask = ((u64)1 << kb) - 1;
     const u64* OK = okey.get();
-    DBuf<int> dims = ctx.buf<int>(2);
-    reduce_to<int>(ctx, [=] __device__(i64 s) { return (int)(OK[s] & mask); }, S, -1, OpMax(), dims.get());
-    int* dp = dims.get();
-    for_each(ctx, 1, [=] __device__(i64) { dp[1] = (int)(OK[S - 1] >> kb); });
-
-    DBuf<int> kpos;
+    const float* OV = oval.get();
     const int* KP = keep.get();
-    const i64 S2 = select_positions(ctx, [=] __device__(i64 s) { return KP[s]; }, S, kpos);
-    B.nnz = S2;
-    B.row = ctx.buf<int>(S2);
-    B.col = ctx.buf<int>(S2);
-    B.val = ctx.buf<float>(S2);
+    B.row = ctx.buf<int>(n);
+    B.col = ctx.buf<int>(n);
+    B.val = ctx.buf<float>(n);
     int* BR = B.row.get();
     int* BC = B.col.get();
     float* BW = B.val.get();
-    const int* P = kpos.get();
-    const float* OV = oval.get();
-    for_each(ctx, S, [=] __device__(i64 s) {
-        if (KP[s]) {
-            const int q = P[s];
+    // drop the self-loops (remove_diagonal keeps rows_/cols_)
+    select_into(
+        ctx, n, [=] __device__(i64 s) { return s < SC[0] && KP[s] != 0; },
+        [=] __device__(i64 s, int q) {
             BR[q] = (int)(OK[s] >> kb);
             BC[q] = (int)(OK[s] & mask);
             BW[q] = OV[s];
-        }
-    });
-    int h[2];
-    ctx.read(dims.get(), h, 2);
-    B.cols = h[0] + 1;
-    B.rows = h[1] + 1;
-    double sc[2];
-    ctx.read(scal.get(), sc, 2);
-    int badv;
-    std::memcpy(&badv, &sc[1], sizeof(int));
-    *bad = badv != 0;
-    if (diag_sum) *diag_sum = sc[0];
+        },
+        SC + 1);
+    // dims as the reference computes them before removing the diagonal
+    // (src/dCOO.cu:91-92), and the largest endpoint among the kept edges
+    reduce_to<int>(ctx, [=] __device__(i64 s) { return s < SC[0] ? (int)(OK[s] & mask) : -1; }, n, -1, OpMax(), SC + 3);
+    reduce_to<int>(ctx, [=] __device__(i64 s) { return (s < SC[0] && KP[s]) ? (int)(OK[s] & mask) : -1; }, n, -1,
+                   OpMax(), SC + 2);
+    for_each(ctx, 1, [=] __device__(i64) { SC[4] = (int)(OK[SC[0] - 1] >> kb); });
+    int h[8];
+    ctx.read(SC, h, 8);
+    B.nnz = h[1];
+    B.cols = h[3] + 1;
+    B.rows = h[4] + 1;
+    B.max_node = h[2];
+    *bad = h[5] != 0;
+    if (diag_sum) *diag_sum = ctx.read1(dg.get());
     return B;
 }
 
@@ -273,7 +274,7 @@ void compute_offsets(Ctx& ctx, const int* sorted_ids, i64 n, int max_value, int*
 // the edge list itself.  Duplicated edges keep their input order, as the
 // reference's stable sort of the doubled list does.
 Csr symmetric_csr(Ctx& ctx, const int* row_in, const int* col_in, const float* val_in, i64 m, int n_nodes,
-                  bool filter, float keep_above) {
+                  bool filter, float keep_above, i64 known_count) {
     Csr G;
     G.n = n_nodes;
     const int* row = row_in;
@@ -283,23 +284,26 @@ Csr symmetric_csr(Ctx& ctx, const int* row_in, const int* col_in, const float* v
     DBuf<float> fv;
     i64 P = m;
     if (filter) {
-        DBuf<int> pos;
-        P = select_positions(ctx, [=] __device__(i64 e) { return (int)(val_in[e] > keep_above); }, m, pos);
+        DBuf<int> cnt = ctx.buf<int>(1);
+        if (known_count < 0) {
+            P = reduce<i64>(ctx, [=] __device__(i64 e) { return (i64)(val_in[e] > keep_above); }, m, (i64)0, OpSum());
+        } else {
+            P = known_count;
+        }
         fr = ctx.buf<int>(P);
         fc = ctx.buf<int>(P);
         fv = ctx.buf<float>(P);
         int* FR = fr.get();
         int* FC = fc.get();
         float* FV = fv.get();
-        const int* PP = pos.get();
-        for_each(ctx, m, [=] __device__(i64 e) {
-            if (val_in[e] > keep_above) {
-                const int q = PP[e];
+        select_into(
+            ctx, m, [=] __device__(i64 e) { return val_in[e] > keep_above; },
+            [=] __device__(i64 e, int q) {
                 FR[q] = row_in[e];
                 FC[q] = col_in[e];
                 FV[q] = val_in[e];
-            }
-        });
+            },
+            cnt.get());
         row = FR;
         col = FC;
         val = FV;
@@ -321,6 +325,7 @@ Csr symmetric_csr(Ctx& ctx, const int* row_in, const int* col_in, const float* v
         TK[e] = (u32)col[e];
         TV[e] = (u32)e;
     });
+    prof::Site site_("csr");
     radix_sort_pairs<u32>(ctx, tk, tv, P, bits_for(n_nodes > 1 ? n_nodes - 1 : 1));
     DBuf<int> offF = ctx.buf<int>((i64)n_nodes + 1), offT = ctx.buf<int>((i64)n_nodes + 1);
     compute_offsets(ctx, row, P, n_nodes - 1, offF.get());
@@ -355,14 +360,14 @@ Csr symmetric_csr(Ctx& ctx, const int* row_in, const int* col_in, const float* v
 
 // compress_label_sequence (include/rama_utils.h:160-177); returns the number
 // of distinct labels.
-int compress_labels(Ctx& ctx, const int* data, i64 n, int max_label, int* out) {
+int compress_labels(Ctx& ctx, const int* data, i64 n, int max_label, int* out, bool want_count) {
     DBuf<int> mask = ctx.buf<int>((i64)max_label + 2);
     ctx.zero(mask.get(), (i64)max_label + 2);
     int* M = mask.get();
     for_each(ctx, n, [=] __device__(i64 k) { M[data[k]] = 1; });
     scan_exclusive_arr<int>(ctx, M, M, (i64)max_label + 1);
     for_each(ctx, n, [=] __device__(i64 k) { out[k] = M[data[k]]; });
-    return ctx.read1(M + max_label + 1);
+    return want_count ? ctx.read1(M + max_label + 1) : -1;
 }
 
 // map_node_labels (src/rama_cuda.cu:32-52)
@@ -392,6 +397,7 @@ Sanitized sanitize_graph(Ctx& ctx, DBuf<int>& i, DBuf<int>& j, DBuf<float>& c, i
         K[e] = ((u64)(a < b ? a : b) << kb) | (a < b ? b : a);
         V[e] = f2u(Cc[e]);
     });
+    prof::Site site_("sanitize");
     radix_sort_pairs(ctx, keys, vals, m, 2 * kb);
     DBuf<int> pos;
     const i64 m2 = select_positions(

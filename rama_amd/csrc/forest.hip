@@ -436,6 +436,7 @@ static void frontier_sorted(Ctx& ctx, const WocState::Frontier& f, int rb, DBuf<
         K[q] = ((u64)(u32)ND[q] << rb) | (u32)RP[q];
         O[q] = (u32)q;
     });
+    prof::Site site_("forest");
     radix_sort_pairs(ctx, keys, order, f.size, kb_total);
 }
 
@@ -575,7 +576,7 @@ int woc_mapping(Ctx& ctx, const DCoo& A, DBuf<int>& map, i64& map_size) {
         itr++;
     }
     map = ctx.buf<int>(S.n);
-    compress_labels(ctx, S.cc.get(), S.n, S.n - 1, map.get());
+    compress_labels(ctx, S.cc.get(), S.n, S.n - 1, map.get(), false);
     map_size = S.n;
     if (ctx.verbose) std::printf("Found conflict-free contraction mapping (%lld forest entries)\n", (long long)S.F);
     return (int)S.F;

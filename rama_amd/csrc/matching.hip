@@ -3,9 +3,10 @@
 //
 // Per round: one kernel picks each unmatched vertex's strongest eligible
 // neighbour (strictly above the running maximum initialised to the
-// threshold, first maximum wins), one kernel matches mutual pairs, and one
-// reduction counts matched vertices.  The result is deterministic: mutual
-// pairs are disjoint, so the racing writes of a pair write the same values.
+// threshold, first maximum wins) and one kernel matches mutual pairs and
+// counts them.  The loop's stop rule runs on the device, so all ten rounds
+// are queued at once and the host reads the outcome once.  The result is
+// deterministic: mutual pairs are disjoint, so racing writes agree.
 #include "rama_internal.hpp"
 
 namespace rama {
@@ -15,10 +16,40 @@ namespace rama {
 // the lowest position), which equals the sequential scan of
 // get_best_neighbour (src/maximum_matching_vertex_based.cu:10-28).
 constexpr int kGroup = 8;
+constexpr int kMaxRounds = 10;
 
-__global__ __launch_bounds__(kBlock) void k_pick_best(int n, float thr, const int* __restrict__ off,
-                                                      const int* __restrict__ adj, const float* __restrict__ w,
-                                                      const int* __restrict__ matched, int* __restrict__ best) {
+// Device state of the matching loop: threshold inputs and, per round, the
+// number of newly matched vertices and whether any pair matched.
+struct MatchState {
+    double pos_sum;
+    double pos_cnt;
+    int newly[kMaxRounds];
+    int running[kMaxRounds];
+};
+
+// Threshold (determine_matching_threshold, :76-86) and whether round t runs:
+// the stop rule `!still_running || rel_increase < 0.1` (:121-127) replayed
+// from the per-round counts, identically in every block.
+__device__ __forceinline__ bool match_round_active(const MatchState* st, float mult, int t, float* thr) {
+    if (st->pos_cnt == 0.0) return false;
+    *thr = mult * (float)st->pos_sum / (float)(i64)st->pos_cnt;
+    if (*thr < 0.0f) return false;
+    int prev = 0;
+    for (int r = 0; r < t; ++r) {
+        const int cur = prev + st->newly[r];
+        const float rel = (cur - prev) / (prev + 1.0f);
+        if (!st->running[r] || rel < 0.1f) return false;
+        prev = cur;
+    }
+    return true;
+}
+
+__global__ __launch_bounds__(kBlock) void k_pick_best(int n, float mult, int round, const MatchState* __restrict__ st,
+                                                      const int* __restrict__ off, const int* __restrict__ adj,
+                                                      const float* __restrict__ w, const int* __restrict__ matched,
+                                                      int* __restrict__ best) {
+    float thr;
+    if (!match_round_active(st, mult, round, &thr)) return;  // uniform over the grid
     const i64 t = (i64)blockIdx.x * kBlock + threadIdx.x;
     const int v = (int)(t / kGroup);
     const int sub = (int)(t % kGroup);
@@ -59,68 +90,77 @@ __global__ __launch_bounds__(kBlock) void k_pick_best(int n, float thr, const in
     if (active && sub == 0) best[v] = nid;
 }
 
-__global__ void k_match(int n, const int* __restrict__ best, int* matched, int* map, int* running) {
+// match_neighbours (:49-74).  A pair matches when both ends are unmatched
+// and each is the other's best; such pairs are disjoint.  Only the smaller
+// end of a pair acts (writes the map, both flags and counts 2 vertices), so
+// no thread reads a flag that another thread of this launch writes: the
+// matched set and the count are the same in any execution order.
+__global__ __launch_bounds__(kBlock) void k_match(int n, float mult, int round, MatchState* st,
+                                                  const int* __restrict__ best, int* matched, int* map) {
+    __shared__ int ws[kWaves];
+    float thr;
+    if (!match_round_active(st, mult, round, &thr)) return;
     const int v = blockIdx.x * blockDim.x + threadIdx.x;
-    if (v >= n) return;
-    if (matched[v]) return;
-    const int vb = best[v];
-    if (vb == -1 || matched[vb]) return;
-    if (best[vb] == v) {
-        map[vb > v ? vb : v] = vb < v ? vb : v;
-        matched[v] = 1;
-        matched[vb] = 1;
-        *running = 1;
+    int add = 0;
+    if (v < n && !matched[v]) {
+        const int vb = best[v];
+        if (vb > v && !matched[vb] && best[vb] == v) {
+            map[vb] = v;
+            matched[v] = 1;
+            matched[vb] = 1;
+            add = 2;
+        }
     }
-}
-
-// determine_matching_threshold (src/maximum_matching_vertex_based.cu:76-86):
-// mean of the non-negative costs (summed in double) times the multiplier.
-static bool positive_mean_threshold(Ctx& ctx, const float* w, i64 n, float mult, float* thr) {
-    DBuf<double> s = ctx.buf<double>(2);
-    reduce_to<double>(
-        ctx, [=] __device__(i64 e) { const float x = w[e]; return x >= 0.0f ? (double)x : 0.0; }, n, 0.0, OpSum(),
-        s.get());
-    reduce_to<double>(
-        ctx, [=] __device__(i64 e) { return w[e] >= 0.0f ? 1.0 : 0.0; }, n, 0.0, OpSum(), s.get() + 1);
-    double h[2];
-    ctx.read(s.get(), h, 2);
-    if (h[1] == 0.0) return false;
-    const float sf = (float)h[0];
-    *thr = mult * sf / (float)(i64)h[1];
-    return true;
+    const int tot = block_reduce(add, OpSum(), ws);
+    if (threadIdx.x == 0 && tot) {
+        atomicAdd(&st->newly[round], tot);
+        st->running[round] = 1;
+    }
 }
 
 int matching_vertex_based(Ctx& ctx, const Csr& U, float mean_multiplier, DBuf<int>& map) {
     const int n = U.n;
     map = ctx.buf<int>(n);
     iota(ctx, map.get(), n);
-    float thr = 0.0f;
-    if (n == 0 || !positive_mean_threshold(ctx, U.w.get(), U.nnz, mean_multiplier, &thr) || thr < 0.0f) return 0;
-    DBuf<int> best = ctx.buf<int>(n), matched = ctx.buf<int>(n), scal = ctx.buf<int>(2);
+    if (n == 0) return 0;
+    DBuf<MatchState> stb = ctx.buf<MatchState>(1);
+    MatchState* st = stb.get();
+    ctx.zero(st, 1);
+    const float* W = U.w.get();
+    reduce_to<double>(
+        ctx, [=] __device__(i64 e) { const float x = W[e]; return x >= 0.0f ? (double)x : 0.0; }, U.nnz, 0.0, OpSum(),
+        &st->pos_sum);
+    reduce_to<double>(ctx, [=] __device__(i64 e) { return W[e] >= 0.0f ? 1.0 : 0.0; }, U.nnz, 0.0, OpSum(),
+                      &st->pos_cnt);
+    DBuf<int> best = ctx.buf<int>(n), matched = ctx.buf<int>(n);
     fill(ctx, best.get(), n, -1);
     ctx.zero(matched.get(), n);
-    int prev = 0;
-    const int* M = matched.get();
-    for (int t = 0; t < 10; ++t) {
-        ctx.zero(scal.get() + 1, 1);
-        const i64 threads = (i64)n * kGroup;
+    const i64 threads = (i64)n * kGroup;
+    for (int t = 0; t < kMaxRounds; ++t) {
         {
             // offsets + (neighbour, cost, neighbour's matched flag) per CSR entry + best per vertex
             prof::Scope ps(ctx, "match_pick_best", 12.0 * U.nnz + 12.0 * n);
-            k_pick_best<<<grid_for(threads), kBlock, 0, ctx.stream>>>(n, thr, U.off.get(), U.adj.get(), U.w.get(),
-                                                                      matched.get(), best.get());
+            k_pick_best<<<grid_for(threads), kBlock, 0, ctx.stream>>>(n, mean_multiplier, t, st, U.off.get(),
+                                                                      U.adj.get(), U.w.get(), matched.get(),
+                                                                      best.get());
             RAMA_LAUNCHED();
         }
-        k_match<<<grid_for(n), kBlock, 0, ctx.stream>>>(n, best.get(), matched.get(), map.get(), scal.get() + 1);
+        k_match<<<grid_for(n), kBlock, 0, ctx.stream>>>(n, mean_multiplier, t, st, best.get(), matched.get(),
+                                                        map.get());
         RAMA_LAUNCHED();
-        reduce_to<int>(ctx, [=] __device__(i64 v) { return M[v]; }, n, 0, OpSum(), scal.get());
-        int h[2];
-        ctx.read(scal.get(), h, 2);
-        const int cur = h[0];
-        const float rel = (cur - prev) / (prev + 1.0f);
-        prev = cur;
-        if (ctx.verbose) std::printf("matched sum: %d, rel_increase: %g\n", cur, rel);
-        if (!h[1] || rel < 0.1f) break;
+    }
+    MatchState h;
+    ctx.read(st, &h, 1);
+    // replay the stop rule on the host for the returned count
+    int prev = 0;
+    if (h.pos_cnt > 0.0 && mean_multiplier * (float)h.pos_sum / (float)(i64)h.pos_cnt >= 0.0f) {
+        for (int t = 0; t < kMaxRounds; ++t) {
+            const int cur = prev + h.newly[t];
+            const float rel = (cur - prev) / (prev + 1.0f);
+            prev = cur;
+            if (ctx.verbose) std::printf("matched sum: %d, rel_increase: %g\n", cur, rel);
+            if (!h.running[t] || rel < 0.1f) break;
+        }
     }
     return prev;
 }

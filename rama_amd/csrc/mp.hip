@@ -61,6 +61,7 @@ static void mp_build_impl(Ctx& ctx, const DCoo& A, Triangles& tri, MPState& S, i
         VT[3 * t + 1] = (u32)(3 * t + 1);
         VT[3 * t + 2] = (u32)(3 * t + 2);
     });
+    prof::Site site_("mp_build");
     radix_sort_pairs<K>(ctx, kt, vt, N3, 2 * kb);
     // chords: first item of a key run whose key is not a graph edge
     auto is_chord = [=] __device__(i64 q) {
@@ -139,7 +140,8 @@ static void mp_build_impl(Ctx& ctx, const DCoo& A, Triangles& tri, MPState& S, i
 
 void mp_build(Ctx& ctx, const DCoo& A, Triangles& tri, MPState& S) {
     int maxv = A.max_dim() - 1;
-    if (tri.n > 0) maxv = std::max(maxv, max_id(ctx, tri.t3.get(), tri.t3.get(), tri.n));
+    if (tri.n > 0)
+        maxv = std::max(maxv, tri.max_id >= 0 ? tri.max_id : max_id(ctx, tri.t3.get(), tri.t3.get(), tri.n));
     const int kb = bits_for(maxv < 1 ? 1 : maxv);
     if (2 * kb <= 32)
         mp_build_impl<u32>(ctx, A, tri, S, kb);
@@ -147,6 +149,14 @@ void mp_build(Ctx& ctx, const DCoo& A, Triangles& tri, MPState& S) {
         mp_build_impl<u64>(ctx, A, tri, S, kb);
     const i64 nb = (std::max(S.T, S.E) + kBlock - 1) / kBlock + 2;
     S.lbparts = ctx.buf<double>(2 * nb + 2);
+    // dimensions of the reparametrised graph, read back with the final bound
+    S.dims = ctx.buf<int>(2);
+    S.max_col = S.last_row = -1;
+    if (S.E > 0) {
+        const int* EJ = S.ej.get();
+        reduce_to<int>(ctx, [=] __device__(i64 e) { return EJ[e]; }, S.E, -1, OpMax(), S.dims.get());
+        ctx.d2d(S.dims.get() + 1, S.ei.get() + S.E - 1, 1);
+    }
 }
 
 // Device-side state of the dual loop (src/dual_solver.cu:17-26): the
@@ -156,6 +166,8 @@ struct MPControl {
     double prev;
     int active;
     int iterations;
+    int max_col;   // copied from MPState::dims so one read returns both
+    int last_row;
 };
 
 // After an iteration: sum the fused lower-bound partials (fixed order) and
@@ -358,11 +370,14 @@ double mp_run(Ctx& ctx, MPState& S, int steps) {
     reduce_to<double>(ctx, [=] __device__(i64 t) { return (double)tri_lb(A[t], B[t], D[t]); }, S.T, 0.0, OpSum(),
                       P + 1);
     MPControl* C = ctl.get();
+    const int* DM = S.E > 0 ? S.dims.get() : nullptr;
     for_each(ctx, 1, [=] __device__(i64) {
         C->lb = P[0] + P[1];
         C->prev = 0.0;
         C->active = 1;
         C->iterations = 0;
+        C->max_col = DM ? DM[0] : -1;
+        C->last_row = DM ? DM[1] : -1;
     });
     const i64 nparts = (i64)grid_for(S.T) + (i64)grid_for(S.E);
     for (int it = 0; it < steps; ++it) {
@@ -372,6 +387,8 @@ double mp_run(Ctx& ctx, MPState& S, int steps) {
     }
     MPControl h;
     ctx.read(C, &h, 1);
+    S.max_col = h.max_col;
+    S.last_row = h.last_row;
     return h.lb;
 }
 
@@ -386,6 +403,17 @@ double mp_fused_lower_bound(Ctx& ctx, MPState& S) {
 
 // reparametrized_edge_costs (:356-360) as a sorted dCOO.
 DCoo mp_reparametrized(Ctx& ctx, MPState& S) {
+    if (S.E > 0 && S.max_col >= 0) {
+        DCoo A;
+        A.row = std::move(S.ei);
+        A.col = std::move(S.ej);
+        A.val = std::move(S.ec);
+        A.nnz = S.E;
+        A.cols = S.max_col + 1;
+        A.rows = S.last_row + 1;
+        A.max_node = S.max_col;
+        return A;
+    }
     return dcoo_from_sorted(ctx, std::move(S.ei), std::move(S.ej), std::move(S.ec), S.E);
 }
 

@@ -254,28 +254,44 @@ inline void scan_check(Ctx& ctx) {
 }
 
 // ------------------------------------------------------------- reduction
+// One launch: every block reduces its grid-stride share and stores the
+// partial; the last block to finish (by an atomic ticket) combines the
+// partials in block order, so the result is deterministic for a given n.
+constexpr unsigned kReduceMaxBlocks = 1024;
+
 template <class T, class Op, class F>
-__global__ __launch_bounds__(kBlock) void k_reduce(F f, i64 n, T ident, Op op, T* out) {
+__global__ __launch_bounds__(kBlock) void k_reduce1(F f, i64 n, T ident, Op op, T* partials, unsigned* ticket,
+                                                    T* dst) {
     __shared__ T ws[kWaves];
+    __shared__ bool last;
     T acc = ident;
     for (i64 i = (i64)blockIdx.x * kBlock + threadIdx.x; i < n; i += (i64)gridDim.x * kBlock) acc = op(acc, (T)f(i));
-    T r = block_reduce(acc, op, ws);
-    if (threadIdx.x == 0) out[blockIdx.x] = r;
+    const T r = block_reduce(acc, op, ws);
+    if (threadIdx.x == 0) {
+        __hip_atomic_store(&partials[blockIdx.x], r, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        const unsigned t = __hip_atomic_fetch_add(ticket, 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
+        last = t == gridDim.x - 1;
+    }
+    __syncthreads();
+    if (!last) return;
+    T a = ident;
+    for (unsigned b = threadIdx.x; b < gridDim.x; b += kBlock)
+        a = op(a, __hip_atomic_load(&partials[b], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT));
+    const T total = block_reduce(a, op, ws);
+    if (threadIdx.x == 0) {
+        *dst = total;
+        *ticket = 0;  // every block has taken its ticket: reset for the next launch
+    }
 }
 
 // Deterministic reduction (fixed grid, fixed tree); result stays on device in *dst.
 template <class T, class Op, class F>
 void reduce_to(Ctx& ctx, F f, i64 n, T ident, Op op, T* dst) {
-    const unsigned nb = (unsigned)std::min<i64>(std::max<i64>((n + kBlock - 1) / kBlock, 1), 1024);
-    if (nb == 1) {
-        k_reduce<T, Op, F><<<1, kBlock, 0, ctx.stream>>>(f, n, ident, op, dst);
-        RAMA_LAUNCHED();
-        return;
-    }
-    DBuf<T> part = ctx.buf<T>(nb);
-    k_reduce<T, Op, F><<<nb, kBlock, 0, ctx.stream>>>(f, n, ident, op, part.get());
-    RAMA_LAUNCHED();
-    k_reduce<T, Op, LoadFn<T>><<<1, kBlock, 0, ctx.stream>>>(LoadFn<T>{part.get()}, (i64)nb, ident, op, dst);
+    const unsigned nb = (unsigned)std::min<i64>(std::max<i64>((n + kBlock - 1) / kBlock, 1), kReduceMaxBlocks);
+    static_assert(sizeof(T) <= 8, "reduction type");
+    void* scratch = ctx.reduce_scratch(kReduceMaxBlocks * 8);
+    k_reduce1<T, Op, F><<<nb, kBlock, 0, ctx.stream>>>(f, n, ident, op, static_cast<T*>(scratch),
+                                                       ctx.reduce_ticket(), dst);
     RAMA_LAUNCHED();
 }
 

@@ -1,17 +1,36 @@
-// prims.hip — stable LSD radix sort of (u64 key, u32 value) pairs.
+// prims.hip — stable LSD radix sort of (u32 or u64 key, u32 value) pairs.
 //
-// One digit of at most 8 bits per pass (digits balanced over the key),
-// three launches per pass:
-//   k_radix_hist     per-tile digit histogram (digit-major table)
-//   scan_exclusive   table -> global (digit, tile) offsets
-//   k_radix_scatter  wave64 ballot "multi-split" ranking, tile reordered in
-//                    LDS so each digit run is written out contiguously.
-// Tiles are 4096 pairs (256 threads x 16 items).  Items are ranked in
+// Digits of at most 8 bits, balanced over the key width.  One launch builds
+// the digit histograms of every pass; then one "onesweep" launch per pass:
+// wave64 ballot multi-split ranking inside a 4096-pair tile, decoupled
+// look-back across tiles for each digit's global offset, and an LDS reorder
+// so each digit run is written out contiguously.  Items are ranked in
 // (wave, item, lane) order, which is their input order, so each pass is
 // stable and the whole sort is stable.
 #include "prims.cuh"
 
 namespace rama {
+
+// Three-launch pass for large arrays: per-tile digit histogram
+// (digit-major table), a scan of the table, and the scatter.
+// Adds one count per active lane to h[d], wave-aggregated: up to two
+// groups of lanes sharing a digit (the common case on partly sorted keys,
+// where whole waves share their high digits) are added by one leader lane
+// each; the remaining lanes add individually.
+__device__ __forceinline__ void wave_count(int* h, u32 d, bool ok) {
+    const int lane = lane_id();
+    unsigned long long active = __ballot(ok);
+#pragma unroll
+    for (int it = 0; it < 2; ++it) {
+        if (!active) return;
+        const int leader = __builtin_ctzll(active);
+        const u32 dl = (u32)__builtin_amdgcn_readlane((int)d, leader);
+        const unsigned long long m = __ballot(d == dl) & active;
+        if (lane == leader) atomicAdd(&h[dl], __popcll(m));
+        active &= ~m;
+    }
+    if ((active >> lane) & 1ull) atomicAdd(&h[d], 1);
+}
 
 template <class K>
 __global__ __launch_bounds__(kBlock) void k_radix_hist(const K* __restrict__ keys, i64 n, int shift, int bits,
@@ -21,13 +40,33 @@ __global__ __launch_bounds__(kBlock) void k_radix_hist(const K* __restrict__ key
     __syncthreads();
     const u32 mask = (1u << bits) - 1u;
     const i64 base = (i64)blockIdx.x * kRsTile;
-#pragma unroll 4
+    K key[kRsItems];
+#pragma unroll
     for (int k = 0; k < kRsItems; ++k) {
         const i64 i = base + (i64)k * kBlock + threadIdx.x;
-        if (i < n) atomicAdd(&h[(int)((u32)(keys[i] >> shift) & mask)], 1);
+        key[k] = i < n ? keys[i] : K(0);
+    }
+#pragma unroll
+    for (int k = 0; k < kRsItems; ++k) {
+        const i64 i = base + (i64)k * kBlock + threadIdx.x;
+        wave_count(h, (u32)(key[k] >> shift) & mask, i < n);
     }
     __syncthreads();
     if (threadIdx.x < (1 << bits)) hist[(i64)threadIdx.x * ntiles + blockIdx.x] = h[threadIdx.x];
+}
+
+// Lanes of the wave holding the same digit as this lane (ballot
+// multi-split), with a shortcut when the whole wave shares one digit.
+__device__ __forceinline__ unsigned long long digit_peers(u32 d, int bits) {
+    const u32 d0 = (u32)__builtin_amdgcn_readfirstlane((int)d);
+    const unsigned long long all = __ballot(1);
+    if (__ballot(d == d0) == all) return all;
+    unsigned long long peers = all;
+    for (int b = 0; b < bits; ++b) {
+        const unsigned long long m = __ballot((d >> b) & 1u);
+        peers &= ((d >> b) & 1u) ? m : ~m;
+    }
+    return peers;
 }
 
 template <class K>
@@ -56,18 +95,14 @@ __global__ __launch_bounds__(kBlock) void k_radix_scatter(const K* __restrict__ 
     for (int k = 0; k < kRsItems; ++k) {
         const i64 i = wbase + (i64)k * 64 + lane;
         const bool ok = i < n;
-        key[k] = ok ? kin[i] : ~K(0);  // padding sorts into the last bucket, after all real keys
+        key[k] = ok ? kin[i] : ~K(0);
         val[k] = ok ? vin[i] : 0u;
     }
     const unsigned long long lt = (1ull << lane) - 1ull;
 #pragma unroll
     for (int k = 0; k < kRsItems; ++k) {
         const u32 d = (u32)(key[k] >> shift) & mask;
-        unsigned long long peers = __ballot(1);
-        for (int b = 0; b < bits; ++b) {
-            const unsigned long long m = __ballot((d >> b) & 1u);
-            peers &= ((d >> b) & 1u) ? m : ~m;
-        }
+        const unsigned long long peers = digit_peers(d, bits);
         const int r = __popcll(peers & lt);
         const int c = __popcll(peers);
         const int before = wcnt[w][d];
@@ -77,7 +112,6 @@ __global__ __launch_bounds__(kBlock) void k_radix_scatter(const K* __restrict__ 
         rnk[k] = before + r;
     }
     __syncthreads();
-    // per digit: exclusive over waves, then exclusive over digits
     {
         const int d = tid;
         int tot = 0;
@@ -88,8 +122,7 @@ __global__ __launch_bounds__(kBlock) void k_radix_scatter(const K* __restrict__ 
             tot += c;
         }
         int all;
-        const int st = block_excl_scan(tot, ws, &all);
-        dstart[d] = st;
+        dstart[d] = block_excl_scan(tot, ws, &all);
         if (d < (1 << bits)) gbase[d] = offs[(i64)d * ntiles + blockIdx.x];
     }
     __syncthreads();
@@ -111,39 +144,241 @@ __global__ __launch_bounds__(kBlock) void k_radix_scatter(const K* __restrict__ 
     }
 }
 
+struct PassPlan {
+    int passes;
+    int shift[8];
+    int width[8];
+};
+
+// Arrays of at most this many tiles use the onesweep path (one launch per
+// pass); larger ones the three-launch pass.
+constexpr int kOnesweepMaxTiles = 64;
+
+// Digit histograms of every pass in one read of the keys.
+template <class K>
+__global__ __launch_bounds__(kBlock) void k_radix_hist_all(const K* __restrict__ keys, i64 n, PassPlan plan,
+                                                           int* __restrict__ hist) {
+    __shared__ int h[8 * 256];
+    for (int k = threadIdx.x; k < plan.passes * 256; k += kBlock) h[k] = 0;
+    __syncthreads();
+    for (i64 i = (i64)blockIdx.x * kBlock + threadIdx.x; i < n; i += (i64)gridDim.x * kBlock) {
+        const K key = keys[i];
+        for (int p = 0; p < plan.passes; ++p)
+            atomicAdd(&h[p * 256 + (int)((u32)(key >> plan.shift[p]) & ((1u << plan.width[p]) - 1u))], 1);
+    }
+    __syncthreads();
+    for (int k = threadIdx.x; k < plan.passes * 256; k += kBlock)
+        if (h[k]) atomicAdd(&hist[k], h[k]);
+}
+
+// One onesweep pass.  Tiles are drawn from an atomic counter (every smaller
+// tile is already running).  A tile publishes its per-digit counts, ranks
+// its keys stably (wave64 ballot multi-split), then each of 256 threads looks
+// back along its digit for the exclusive prefix over earlier tiles, and the
+// tile is reordered in LDS and written out in digit runs.  Status words are
+// (epoch | flag | count) as in the single-pass scan.
+template <class K>
+__global__ __launch_bounds__(kBlock) void k_onesweep(const K* __restrict__ kin, const u32* __restrict__ vin,
+                                                     K* __restrict__ kout, u32* __restrict__ vout, i64 n, int shift,
+                                                     int bits, const int* __restrict__ hist_pass,
+                                                     u64* __restrict__ status, int* ctl, u32 epoch) {
+    __shared__ K sk[kRsTile];
+    __shared__ u32 sv[kRsTile];
+    __shared__ int wcnt[kWaves][256];
+    __shared__ int tcnt[256];
+    __shared__ int dstart[256];
+    __shared__ i64 gbase[256];
+    __shared__ int ws[kWaves];
+    __shared__ int tile_s;
+
+    const int tid = threadIdx.x, lane = lane_id(), w = tid >> 6;
+    const u32 mask = (1u << bits) - 1u;
+    if (tid == 0) tile_s = atomicAdd(&ctl[0], 1);
+    for (int k = tid; k < kWaves * 256; k += kBlock) (&wcnt[0][0])[k] = 0;
+    tcnt[tid] = 0;
+    __syncthreads();
+    const int tile = tile_s;
+    const i64 tile0 = (i64)tile * kRsTile;
+    const i64 wbase = tile0 + (i64)w * (64 * kRsItems);
+    K key[kRsItems];
+    u32 val[kRsItems];
+    int rnk[kRsItems];
+#pragma unroll
+    for (int k = 0; k < kRsItems; ++k) {
+        const i64 i = wbase + (i64)k * 64 + lane;
+        const bool ok = i < n;
+        key[k] = ok ? kin[i] : ~K(0);  // padding sorts into the last bucket, after all real keys
+        val[k] = ok ? vin[i] : 0u;
+    }
+#pragma unroll
+    for (int k = 0; k < kRsItems; ++k) wave_count(tcnt, (u32)(key[k] >> shift) & mask, wbase + (i64)k * 64 + lane < n);
+    __syncthreads();
+    // publish this tile's digit counts early so successors can look back
+    u64* my = status + (i64)tile * 256;
+    if (tid <= (int)mask) {
+        __hip_atomic_store(&my[tid], scan_word(epoch, tile == 0 ? 2u : 1u, (u64)tcnt[tid]), __ATOMIC_RELAXED,
+                           __HIP_MEMORY_SCOPE_AGENT);
+    }
+    // stable ranks within the tile
+    const unsigned long long lt = (1ull << lane) - 1ull;
+#pragma unroll
+    for (int k = 0; k < kRsItems; ++k) {
+        const u32 d = (u32)(key[k] >> shift) & mask;
+        const unsigned long long peers = digit_peers(d, bits);
+        const int r = __popcll(peers & lt);
+        const int c = __popcll(peers);
+        const int before = wcnt[w][d];
+        __builtin_amdgcn_wave_barrier();
+        if (r == 0) wcnt[w][d] = before + c;
+        __builtin_amdgcn_wave_barrier();
+        rnk[k] = before + r;
+    }
+    __syncthreads();
+    {
+        const int d = tid;
+        int tot = 0;
+#pragma unroll
+        for (int q = 0; q < kWaves; ++q) {
+            const int c = wcnt[q][d];
+            wcnt[q][d] = tot;
+            tot += c;
+        }
+        int all;
+        dstart[d] = block_excl_scan(tot, ws, &all);
+        // global start of digit d in this pass, then look back over tiles
+        const int hd = d <= (int)mask ? hist_pass[d] : 0;
+        int dall;
+        const int dglob = block_excl_scan(hd, ws, &dall);
+        if (d <= (int)mask) {
+            i64 prefix = 0;
+            if (tile > 0) {
+                // windowed look-back: kLookWin predecessor words are loaded
+                // together (independent loads in flight), then consumed
+                // nearest first until an inclusive prefix is met; a word not
+                // yet published restarts the window at that tile
+                constexpr int kLookWin = 8;
+                int p = tile - 1;
+                long spins = 0;
+                bool done = false;
+                while (!done) {
+                    u64 wd[kLookWin];
+#pragma unroll
+                    for (int k = 0; k < kLookWin; ++k)
+                        wd[k] = p - k >= 0 ? __hip_atomic_load(&status[(i64)(p - k) * 256 + d], __ATOMIC_RELAXED,
+                                                                __HIP_MEMORY_SCOPE_AGENT)
+                                           : 0ull;
+                    int used = 0;
+#pragma unroll
+                    for (int k = 0; k < kLookWin; ++k) {
+                        if (done || used < k || p - k < 0) continue;  // stop at the first gap
+                        const u32 fl = (u32)(wd[k] >> 46) & 3u;
+                        if ((u32)(wd[k] >> 48) != epoch || fl == 0) continue;
+                        prefix += (i64)(wd[k] & kScanValMask);
+                        used = k + 1;
+                        if (fl == 2) done = true;
+                    }
+                    if (done) break;
+                    p -= used;
+                    if (used < kLookWin) {
+                        // tile 0 always publishes an inclusive word, so p >= 0 here
+                        if (++spins > kScanSpinLimit) {
+                            ctl[1] = 1;
+                            break;
+                        }
+                        __builtin_amdgcn_s_sleep(1);
+                    }
+                }
+                __hip_atomic_store(&my[d], scan_word(epoch, 2, (u64)(prefix + tcnt[d])), __ATOMIC_RELAXED,
+                                   __HIP_MEMORY_SCOPE_AGENT);
+            }
+            gbase[d] = (i64)dglob + prefix;
+        }
+    }
+    __syncthreads();
+    if (tid == 0 && tile == (int)gridDim.x - 1) ctl[0] = 0;  // every block has drawn its tile
+#pragma unroll
+    for (int k = 0; k < kRsItems; ++k) {
+        const u32 d = (u32)(key[k] >> shift) & mask;
+        const int pos = dstart[d] + wcnt[w][d] + rnk[k];
+        sk[pos] = key[k];
+        sv[pos] = val[k];
+    }
+    __syncthreads();
+    const int nvalid = (int)((n - tile0) < kRsTile ? (n - tile0) : kRsTile);
+    for (int p = tid; p < nvalid; p += kBlock) {
+        const K k = sk[p];
+        const u32 d = (u32)(k >> shift) & mask;
+        const i64 o = gbase[d] + (p - dstart[d]);
+        kout[o] = k;
+        vout[o] = sv[p];
+    }
+}
+
 template <class K>
 void radix_sort_pairs(Ctx& ctx, K* keys, u32* vals, K* tk, u32* tv, i64 n, int bits) {
     if (n <= 1 || bits <= 0) return;
     RAMA_REQUIRE(n < ((i64)1 << 31), "radix sort supports fewer than 2^31 elements");
     RAMA_REQUIRE(bits <= (int)(8 * sizeof(K)), "radix sort key wider than its type");
     const int ntiles = (int)((n + kRsTile - 1) / kRsTile);
-    const int passes = (bits + 7) / 8;
-    const int base_w = bits / passes, extra = bits % passes;
-    int maxw = base_w + (extra ? 1 : 0);
-    DBuf<int> hist = ctx.buf<int>(((i64)1 << maxw) * ntiles + 1);
+    PassPlan plan{};
+    plan.passes = (bits + 7) / 8;
+    const int base_w = bits / plan.passes, extra = bits % plan.passes;
+    for (int p = 0, sh = 0; p < plan.passes; ++p) {
+        plan.width[p] = base_w + (p < extra ? 1 : 0);
+        plan.shift[p] = sh;
+        sh += plan.width[p];
+    }
     K* ka = keys;
     u32* va = vals;
     K* kb = tk;
     u32* vb = tv;
-    int shift = 0;
     const double pair_bytes = (double)(sizeof(K) + sizeof(u32));
-    for (int p = 0; p < passes; ++p) {
-        const int wbits = base_w + (p < extra ? 1 : 0);
-        {
-            prof::Scope ps(ctx, "radix_hist", (double)sizeof(K) * n);
-            k_radix_hist<K><<<ntiles, kBlock, 0, ctx.stream>>>(ka, n, shift, wbits, hist.get(), ntiles);
-            RAMA_LAUNCHED();
+    if (ntiles > kOnesweepMaxTiles) {
+        // large arrays: three launches per pass (look-back chains would be long)
+        DBuf<int> th = ctx.buf<int>((i64)256 * ntiles + 1);
+        for (int p = 0; p < plan.passes; ++p) {
+            {
+                prof::Scope ps(ctx, "radix_hist", (double)sizeof(K) * n);
+                k_radix_hist<K><<<ntiles, kBlock, 0, ctx.stream>>>(ka, n, plan.shift[p], plan.width[p], th.get(),
+                                                                   ntiles);
+                RAMA_LAUNCHED();
+            }
+            scan_exclusive_arr<int>(ctx, th.get(), th.get(), ((i64)1 << plan.width[p]) * ntiles);
+            {
+                prof::Scope ps(ctx, "radix_scatter", 2.0 * pair_bytes * n);
+                k_radix_scatter<K><<<ntiles, kBlock, 0, ctx.stream>>>(ka, va, kb, vb, n, plan.shift[p],
+                                                                      plan.width[p], th.get(), ntiles);
+                RAMA_LAUNCHED();
+            }
+            std::swap(ka, kb);
+            std::swap(va, vb);
         }
-        scan_exclusive_arr<int>(ctx, hist.get(), hist.get(), ((i64)1 << wbits) * ntiles);
+        if (ka != keys) {
+            ctx.d2d(keys, ka, n);
+            ctx.d2d(vals, va, n);
+        }
+        return;
+    }
+    DBuf<int> hist = ctx.buf<int>(plan.passes * 256);
+    ctx.zero(hist.get(), plan.passes * 256);
+    {
+        prof::Scope ps(ctx, "radix_hist_all", (double)sizeof(K) * n);
+        const unsigned hb = (unsigned)std::min<i64>(ntiles, 1024);
+        k_radix_hist_all<K><<<hb, kBlock, 0, ctx.stream>>>(keys, n, plan, hist.get());
+        RAMA_LAUNCHED();
+    }
+    for (int p = 0; p < plan.passes; ++p) {
+        u32 epoch;
+        u64* st = ctx.scan_begin((i64)ntiles * 256, &epoch);
         {
             // reads and writes one (key, u32 value) pair per element
-            prof::Scope ps(ctx, "radix_scatter", 2.0 * pair_bytes * n);
-            k_radix_scatter<K><<<ntiles, kBlock, 0, ctx.stream>>>(ka, va, kb, vb, n, shift, wbits, hist.get(), ntiles);
+            prof::Scope ps(ctx, "radix_onesweep", 2.0 * pair_bytes * n);
+            k_onesweep<K><<<ntiles, kBlock, 0, ctx.stream>>>(ka, va, kb, vb, n, plan.shift[p], plan.width[p],
+                                                             hist.get() + p * 256, st, ctx.scan_counter.get(), epoch);
             RAMA_LAUNCHED();
         }
         std::swap(ka, kb);
         std::swap(va, vb);
-        shift += wbits;
     }
     if (ka != keys) {
         ctx.d2d(keys, ka, n);

@@ -2,6 +2,7 @@
 //   rama_profile_enable(on), rama_profile_reset(),
 //   rama_profile_count(), rama_profile_entry(k, name, launches, ms, bytes).
 #include <atomic>
+#include <cstdlib>
 #include <mutex>
 
 #include "profiler.hpp"
@@ -12,8 +13,10 @@ namespace prof {
 namespace {
 std::atomic<bool> g_on{false};
 std::mutex g_mu;
+thread_local const char* tl_site = nullptr;
+const bool g_sites = std::getenv("RAMA_PROFILE_SITES") != nullptr;
 struct Pending {
-    const char* name;
+    std::string name;
     hipEvent_t a, b;
     double bytes;
 };
@@ -52,6 +55,20 @@ void resolve_locked() {
 
 bool enabled() { return g_on.load(std::memory_order_relaxed); }
 
+std::string g_only;  // guarded by g_mu for writes; read while enabled
+std::atomic<bool> g_filtered{false};
+
+bool wants(const char* name) {
+    if (!enabled()) return false;
+    if (!g_filtered.load(std::memory_order_relaxed)) return true;
+    std::lock_guard<std::mutex> l(g_mu);
+    return g_only == name;
+}
+
+const char* current_site() { return tl_site; }
+Site::Site(const char* s) : prev_(tl_site) { tl_site = s; }
+Site::~Site() { tl_site = prev_; }
+
 hipEvent_t take_event() {
     std::lock_guard<std::mutex> l(g_mu);
     if (!g_free.empty()) {
@@ -66,7 +83,9 @@ hipEvent_t take_event() {
 
 void push(const char* name, hipEvent_t a, hipEvent_t b, double bytes) {
     std::lock_guard<std::mutex> l(g_mu);
-    g_pending.push_back(Pending{name, a, b, bytes});
+    std::string key(name);
+    if (g_sites && tl_site) key = key + "@" + tl_site;
+    g_pending.push_back(Pending{std::move(key), a, b, bytes});
     if (g_pending.size() > 4096) resolve_locked();
 }
 
@@ -81,6 +100,12 @@ void rama_profile_enable(int on) {
     std::lock_guard<std::mutex> l(g_mu);
     resolve_locked();
     g_on.store(on != 0);
+}
+
+void rama_profile_only(const char* name) {
+    std::lock_guard<std::mutex> l(g_mu);
+    g_only = name ? name : "";
+    g_filtered.store(!g_only.empty());
 }
 
 void rama_profile_reset(void) {

@@ -10,14 +10,29 @@ namespace rama {
 namespace prof {
 
 bool enabled();
+bool wants(const char* name);  // enabled and not filtered out
 void push(const char* name, hipEvent_t a, hipEvent_t b, double bytes);
 hipEvent_t take_event();
+
+// Call-site tag for shared primitives (sorts): with RAMA_PROFILE_SITES set,
+// entries recorded inside a Site are named "kernel@site".
+const char* current_site();
+class Site {
+   public:
+    explicit Site(const char* s);
+    ~Site();
+    Site(const Site&) = delete;
+    Site& operator=(const Site&) = delete;
+
+   private:
+    const char* prev_;
+};
 
 // Brackets the launches in its scope with events on ctx.stream.
 class Scope {
    public:
     Scope(Ctx& ctx, const char* name, double bytes) : ctx_(ctx), name_(name), bytes_(bytes) {
-        if (enabled()) {
+        if (wants(name)) {
             a_ = take_event();
             b_ = take_event();
             RAMA_HIP(hipEventRecord(a_, ctx_.stream));

@@ -16,6 +16,10 @@ struct DCoo {
     DBuf<float> val;
     i64 nnz = 0;
     int rows = 0, cols = 0;
+    // largest node id that has an edge (-1 if none): export_undirected's
+    // node count is max_node + 1, which can be below cols once self-loops
+    // have been removed
+    int max_node = -1;
     int max_dim() const { return rows > cols ? rows : cols; }
 };
 
@@ -51,10 +55,12 @@ DCoo dcoo_from_edges(Ctx& ctx, const int* row_in, const int* col_in, const float
 DCoo dcoo_from_sorted(Ctx& ctx, DBuf<int>&& row, DBuf<int>&& col, DBuf<float>&& val, i64 m);
 DCoo dcoo_copy(Ctx& ctx, const DCoo& A);
 DCoo contract(Ctx& ctx, const DCoo& A, const int* node_map, int max_label, bool* bad, double* diag_sum);
+// `known_count`: number of edges passing the filter if the caller knows it (-1: count here).
 Csr symmetric_csr(Ctx& ctx, const int* row, const int* col, const float* val, i64 m, int n_nodes,
-                  bool filter, float keep_above);
+                  bool filter, float keep_above, i64 known_count = -1);
 void compute_offsets(Ctx& ctx, const int* sorted_ids, i64 n, int max_value, int* out);
-int compress_labels(Ctx& ctx, const int* data, i64 n, int max_label, int* out);
+// Returns the number of distinct labels if want_count (one read-back), else -1.
+int compress_labels(Ctx& ctx, const int* data, i64 n, int max_label, int* out, bool want_count = true);
 void map_node_labels(Ctx& ctx, const int* cur, i64 cur_size, int* orig, i64 orig_size);
 double dcoo_lb(Ctx& ctx, const float* val, i64 n);
 float dcoo_min(Ctx& ctx, const float* val, i64 n);
@@ -76,6 +82,7 @@ struct Triangles {
     DBuf<int> t1, t2, t3;
     i64 n = 0;    // after normalisation
     i64 raw = 0;  // slots filled by the search (reference triangles_v1.size())
+    int max_id = -1;  // known bound on the vertex ids (-1: unknown, scan them)
 };
 Triangles conflicted_cycles(Ctx& ctx, const DCoo& A, int max_cycle_length, float tri_memory_factor, float tol_ratio);
 void normalize_triangles(Ctx& ctx, Triangles& T);
@@ -92,6 +99,8 @@ struct MPState {
     DBuf<float> t12, t13, t23;   // triangle costs
     DBuf<float> delta;           // per-slot messages (3T)
     DBuf<double> lbparts;        // partial lower bounds
+    DBuf<int> dims;              // [0] max col, [1] last row of (ei, ej), on device
+    int max_col = -1, last_row = -1;  // host copies once read back (-1: not yet)
     i64 E = 0, T = 0;
 };
 void mp_build(Ctx& ctx, const DCoo& A, Triangles& tri, MPState& S);

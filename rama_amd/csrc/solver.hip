@@ -86,13 +86,13 @@ SolveResult solve(Ctx& ctx, DCoo& A, const Options& o) {
         if (st.on) std::fprintf(stderr, "round %d: %lld edges, %s\n", iter, (long long)A.nnz, use_mm ? "matching" : "forest");
         if (use_mm) {
             // export_undirected: node count = largest endpoint + 1
-            const int* C = A.col.get();
-            const int Un = reduce<int>(ctx, [=] __device__(i64 e) { return C[e]; }, A.nnz, -1, OpMax()) + 1;
+            const int Un = A.max_node + 1;
             Csr U = symmetric_csr(ctx, A.row.get(), A.col.get(), A.val.get(), A.nnz, Un, false, 0.0f);
             DBuf<int> m;
             n_contract = matching_vertex_based(ctx, U, o.mean_multiplier_mm, m);
             cur = ctx.buf<int>(Un);
-            max_label = compress_labels(ctx, m.get(), Un, Un - 1, cur.get()) - 1;
+            compress_labels(ctx, m.get(), Un, Un - 1, cur.get(), false);
+            max_label = Un - 1;  // bound on the compressed labels (sizes the sort keys)
             cur_size = Un;
             if ((float)n_contract < (float)A.rows * o.matching_thresh_crossover_ratio) {
                 if (o.verbose)

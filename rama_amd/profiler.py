@@ -16,6 +16,11 @@ def enable(on=True):
     L.rama_profile_enable(1 if on else 0)
 
 
+def only(name=None):
+    """Time only launches named `name` (None: all)."""
+    _capi.lib().rama_profile_only(name.encode() if name else None)
+
+
 def read():
     L = _capi.lib()
     out = []
@@ -34,6 +39,12 @@ def top(stats, k=8):
     return [{"name": d["name"], "launches": d["launches"], "ms": round(d["ms"], 3),
              "GBps": round(d["bytes"] / (d["ms"] * 1e6), 1) if d["bytes"] > 0 and d["ms"] > 0 else None}
             for d in stats[:k]]
+
+
+def dominant(stats):
+    """Name of the kernel with a byte model that takes the most time."""
+    cands = [d for d in stats if d["bytes"] > 0 and d["ms"] > 0]
+    return max(cands, key=lambda x: x["ms"])["name"] if cands else None
 
 
 def roofline(stats, peak_gbs, name=None):

@@ -105,6 +105,19 @@ def test_deterministic(gpu):
     assert np.array_equal(a[0], b[0]) and a[1] == b[1]
 
 
+def test_cached_blocks_reused_and_released(gpu):
+    # the second solve runs on blocks the first one cached (holding stale
+    # data); releasing the cache and solving again must not change anything
+    i, j, c = named("er3000")
+    o = gpu.options("PD", verbose=0)
+    a = gpu.solve(i, j, c, o)
+    b = gpu.solve(i, j, c, o)
+    gpu.lib().rama_release_cached(-1)
+    d = gpu.solve(i, j, c, o)
+    for x in (b, d):
+        assert np.array_equal(a[0], x[0]) and a[1] == x[1]
+
+
 def test_error_paths(gpu):
     i, j, c = named("grid64")
     r, co, v, (rows, cols) = gpu.dcoo_init(i, j, c)
