@@ -95,19 +95,144 @@ __device__ __forceinline__ int count_common(const int* __restrict__ off, const i
     return c;
 }
 
-template <int S>
-__global__ __launch_bounds__(kBlock) void k_cycle_count(CycArgs a, unsigned char* __restrict__ cnt8,
-                                                        i64* __restrict__ tile_tot) {
-    __shared__ i64 ws[kWaves];
-    const i64 x = (i64)blockIdx.x * kCycTile + threadIdx.x;
-    int c = 0;
-    if (x < a.X) {
-        int v1, v2, n1, n2, qa, qb;
-        if (resolve<S>(a, x, a.tile_rep[blockIdx.x], v1, v2, n1, n2, qa, qb)) c = count_common(a.off, a.adj, qa, qb);
-        cnt8[x] = (unsigned char)(c < 255 ? c : 255);
+// Register-list intersection: one sorted list (at most kRegList entries)
+// is held in registers, padded with -1, and the other is streamed from
+// memory with independent loads.  The k-th copy of a value in the streamed
+// list counts when the register list holds more than k copies, so a value
+// occurring a and b times counts min(a, b) — the merge walk's rule.
+constexpr int kRegList = 16;
+
+__device__ __forceinline__ void load_reg_list(const int* __restrict__ adj, int b, int nb, int (&rb)[kRegList]) {
+#pragma unroll
+    for (int q = 0; q < kRegList; ++q) rb[q] = q < nb ? adj[b + q] : -1;
+}
+
+__device__ __forceinline__ int count_vs_regs(const int* __restrict__ adj, int ia, int ie, const int (&rb)[kRegList]) {
+    int c = 0, prev = -1, occ = 0;
+    for (int k = ia; k < ie; ++k) {
+        const int x = adj[k];
+        int m = 0;
+#pragma unroll
+        for (int q = 0; q < kRegList; ++q) m += (rb[q] == x);
+        occ = x == prev ? occ + 1 : 0;
+        prev = x;
+        c += occ < m;
     }
-    const i64 tot = block_reduce((i64)c * (S - 2), OpSum(), ws);
-    if (threadIdx.x == 0) tile_tot[blockIdx.x] = tot;
+    return c;
+}
+
+// Adds `v` to the total of the tile holding expansion x, flushing the
+// running sum whenever the thread's expansions cross into the next tile.
+struct TileAcc {
+    i64 tile = -1;
+    i64 sum = 0;
+    __device__ __forceinline__ void add(i64* tot, i64 x, i64 v) {
+        const i64 t = x / kCycTile;
+        if (t != tile) {
+            flush(tot);
+            tile = t;
+        }
+        sum += v;
+    }
+    __device__ __forceinline__ void flush(i64* tot) {
+        if (tile >= 0 && sum != 0)
+            atomicAdd(reinterpret_cast<unsigned long long*>(tot + tile), (unsigned long long)sum);
+        sum = 0;
+    }
+};
+
+// 3-cycles: one thread per repulsive edge r = expansion r; N(v2) in
+// registers, N(v1) streamed.  Tile totals by block reduction.
+__global__ __launch_bounds__(kBlock) void k_cycle_count3(CycArgs a, unsigned char* __restrict__ cnt8,
+                                                         i64* __restrict__ tile_tot) {
+    __shared__ i64 ws[kWaves];
+    const unsigned blk = xcd_block();
+    const i64 r = (i64)blk * kCycTile + threadIdx.x;
+    int c = 0;
+    if (r < a.R) {
+        const int v1 = a.rv1[r], v2 = a.rv2[r];
+        const int b2 = a.off[v2], n2 = a.off[v2 + 1] - b2;
+        if (n2 <= kRegList) {
+            int rb[kRegList];
+            load_reg_list(a.adj, b2, n2, rb);
+            c = count_vs_regs(a.adj, a.off[v1], a.off[v1 + 1], rb);
+        } else {
+            c = count_common(a.off, a.adj, v1, v2);
+        }
+        cnt8[r] = (unsigned char)(c < 255 ? c : 255);
+    }
+    const i64 tot = block_reduce((i64)c, OpSum(), ws);
+    if (threadIdx.x == 0) tile_tot[blk] = tot;
+}
+
+// 4-cycles: one thread per repulsive edge; its expansions n1 ∈ N(v1) are
+// xoff[r] .. xoff[r+1]-1 in order, each intersecting N(n1) with N(v2)
+// (held in registers).  tile_tot must be zero on entry.
+__global__ __launch_bounds__(kBlock) void k_cycle_count4(CycArgs a, unsigned char* __restrict__ cnt8,
+                                                         i64* __restrict__ tile_tot) {
+    const i64 r = (i64)xcd_block() * kBlock + threadIdx.x;
+    if (r >= a.R) return;
+    const int v1 = a.rv1[r], v2 = a.rv2[r];
+    const int b1 = a.off[v1], d1 = a.off[v1 + 1] - b1;
+    const int b2 = a.off[v2], n2 = a.off[v2 + 1] - b2;
+    const i64 x0 = a.xoff[r];
+    int rb[kRegList];
+    const bool regs = n2 <= kRegList;
+    if (regs) load_reg_list(a.adj, b2, n2, rb);
+    TileAcc acc;
+    for (int l = 0; l < d1; ++l) {
+        const int n1 = a.adj[b1 + l];
+        const int c = regs ? count_vs_regs(a.adj, a.off[n1], a.off[n1 + 1], rb) : count_common(a.off, a.adj, n1, v2);
+        cnt8[x0 + l] = (unsigned char)(c < 255 ? c : 255);
+        acc.add(tile_tot, x0 + l, 2 * (i64)c);
+    }
+    acc.flush(tile_tot);
+}
+
+// 5-cycles: one thread per (repulsive edge r, n2 = l2-th neighbour of v2);
+// its expansions (n1 = l1-th neighbour of v1, n2) are xoff[r] + l2*d1 + l1
+// for l1 = 0 .. d1-1, each intersecting N(n1) with N(n2) (held in
+// registers); n1 == n2 is skipped.  Threads of rep r are yoff[r] ..
+// yoff[r+1]-1; ytile_rep gives each block's first rep.  tile_tot must be
+// zero on entry.
+__global__ __launch_bounds__(kBlock) void k_cycle_count5(CycArgs a, const i64* __restrict__ yoff,
+                                                         const int* __restrict__ ytile_rep, i64 Y,
+                                                         unsigned char* __restrict__ cnt8,
+                                                         i64* __restrict__ tile_tot) {
+    const unsigned blk = xcd_block();
+    const i64 y = (i64)blk * kCycTile + threadIdx.x;
+    if (y >= Y) return;
+    // rep owning thread y (galloping from the block's first rep)
+    i64 lo = ytile_rep[blk], hi = a.R - 1, step = 1;
+    while (lo + step <= hi && yoff[lo + step] <= y) {
+        lo += step;
+        step <<= 1;
+    }
+    hi = lo + step < a.R - 1 ? lo + step : a.R - 1;
+    while (lo < hi) {
+        const i64 mid = (lo + hi + 1) >> 1;
+        if (yoff[mid] <= y) lo = mid; else hi = mid - 1;
+    }
+    const i64 r = lo;
+    const int v1 = a.rv1[r], v2 = a.rv2[r];
+    const int l2 = (int)(y - yoff[r]);
+    const int b1 = a.off[v1], d1 = a.off[v1 + 1] - b1;
+    const int n2 = a.adj[a.off[v2] + l2];
+    const int bq = a.off[n2], nq = a.off[n2 + 1] - bq;
+    const i64 x0 = a.xoff[r] + (i64)l2 * d1;
+    int rb[kRegList];
+    const bool regs = nq <= kRegList;
+    if (regs) load_reg_list(a.adj, bq, nq, rb);
+    TileAcc acc;
+    for (int l = 0; l < d1; ++l) {
+        const int n1 = a.adj[b1 + l];
+        int c = 0;
+        if (n1 != n2)
+            c = regs ? count_vs_regs(a.adj, a.off[n1], a.off[n1 + 1], rb) : count_common(a.off, a.adj, n1, n2);
+        cnt8[x0 + l] = (unsigned char)(c < 255 ? c : 255);
+        acc.add(tile_tot, x0 + l, 3 * (i64)c);
+    }
+    acc.flush(tile_tot);
 }
 
 __device__ __forceinline__ void put_tri(int* t1, int* t2, int* t3, i64 g, i64 budget, int a, int b, int c) {
@@ -126,15 +251,18 @@ __global__ __launch_bounds__(kBlock) void k_cycle_emit(CycArgs a, const unsigned
                                                        int* __restrict__ t1, int* __restrict__ t2,
                                                        int* __restrict__ t3) {
     __shared__ i64 ws[kWaves];
-    const i64 tstart = base + tile_off[blockIdx.x];
+    // plain block order: tiles past the budget exit at once, and keeping the
+    // working tiles spread over all XCDs balances them
+    const unsigned blk = blockIdx.x;
+    const i64 tstart = base + tile_off[blk];
     if (tstart >= budget) return;  // uniform per block
-    const i64 x = (i64)blockIdx.x * kCycTile + threadIdx.x;
+    const i64 x = (i64)blk * kCycTile + threadIdx.x;
     int c = 0;
     int v1 = 0, v2 = 0, n1 = 0, n2 = 0, qa = 0, qb = 0;
     if (x < a.X) {
         c = cnt8[x];
         if (c > 0) {
-            resolve<S>(a, x, a.tile_rep[blockIdx.x], v1, v2, n1, n2, qa, qb);
+            resolve<S>(a, x, a.tile_rep[blk], v1, v2, n1, n2, qa, qb);
             if (c == 255) c = count_common(a.off, a.adj, qa, qb);
         }
     }
@@ -203,10 +331,33 @@ static i64 run_stage(Ctx& ctx, const Csr& G, const int* rv1, const int* rv2, i64
     DBuf<unsigned char> cnt8 = ctx.buf<unsigned char>(X);
     DBuf<i64> tile = ctx.buf<i64>(ntiles + 1);
     RAMA_REQUIRE(ntiles < ((i64)1 << 31), "too many cycle expansions");
-    {
-        prof::Scope ps(ctx, S == 3 ? "cycles3_count" : (S == 4 ? "cycles4_count" : "cycles5_count"), 0.0);
-        k_cycle_count<S><<<(unsigned)ntiles, kBlock, 0, ctx.stream>>>(a, cnt8.get(), tile.get());
+    if (S == 3) {
+        prof::Scope ps(ctx, "cycles3_count", 0.0);
+        k_cycle_count3<<<(unsigned)ntiles, kBlock, 0, ctx.stream>>>(a, cnt8.get(), tile.get());
         RAMA_LAUNCHED();
+    } else if (S == 4) {
+        ctx.zero(tile.get(), ntiles);
+        prof::Scope ps(ctx, "cycles4_count", 0.0);
+        k_cycle_count4<<<grid_for(R), kBlock, 0, ctx.stream>>>(a, cnt8.get(), tile.get());
+        RAMA_LAUNCHED();
+    } else {
+        // thread space: one thread per (rep edge, neighbour of v2)
+        DBuf<i64> yoff = ctx.buf<i64>(R + 1);
+        scan_exclusive<i64>(ctx, [=] __device__(i64 r) { const int u = rv2[r]; return (i64)(OFF[u + 1] - OFF[u]); },
+                            yoff.get(), R);
+        const i64 Y = ctx.read1(yoff.get() + R);
+        ctx.zero(tile.get(), ntiles);
+        if (Y > 0) {
+            const i64 ytiles = (Y + kCycTile - 1) / kCycTile;
+            RAMA_REQUIRE(ytiles < ((i64)1 << 31), "too many cycle expansions");
+            DBuf<int> ytile_rep = ctx.buf<int>(ytiles);
+            k_tile_rep<<<grid_for(R), kBlock, 0, ctx.stream>>>(yoff.get(), R, ytile_rep.get());
+            RAMA_LAUNCHED();
+            prof::Scope ps(ctx, "cycles5_count", 0.0);
+            k_cycle_count5<<<(unsigned)ytiles, kBlock, 0, ctx.stream>>>(a, yoff.get(), ytile_rep.get(), Y,
+                                                                        cnt8.get(), tile.get());
+            RAMA_LAUNCHED();
+        }
     }
     scan_exclusive_arr<i64>(ctx, tile.get(), tile.get(), ntiles);
     const i64 W = ctx.read1(tile.get() + ntiles);
@@ -228,58 +379,78 @@ static i64 rearrange(Ctx& ctx, Triangles& T, i64 valid, int n_nodes) {
     int* B = T.t2.get();
     int* C = T.t3.get();
     const int kb = bits_for(n_nodes > 1 ? n_nodes - 1 : 1);
-    // order the three ids (write_triangle already does; kept for foreign
-    // input); a triangle with a repeated vertex gets the all-ones key of the
-    // sort width so it sorts after every real triangle and is cut below
+    DBuf<int> cnt = ctx.buf<int>(1);  // unique triangles
+    int* CNT = cnt.get();
+    DBuf<u64> keys = ctx.buf<u64>(valid);
+    DBuf<u32> vals = ctx.buf<u32>(valid);
+    u64* K = keys.get();
+    u32* V = vals.get();
+    if (3 * kb <= 64) {
+        // key = the ordered triple (write_triangle already orders it; kept
+        // for foreign input); a triangle with a repeated vertex gets the
+        // all-ones key of the sort width, which no real triangle has
+        // (first < second < third), so it sorts last and is cut below
+        const u64 pad = (1ull << (3 * kb)) - 1ull;
+        for_each(ctx, valid, [=] __device__(i64 t) {
+            const int a = A[t], b = B[t], c = C[t];
+            const bool degen = a == b || a == c || b == c;
+            const u32 lo = (u32)min(a, min(b, c)), md = (u32)max(min(a, b), min(max(a, b), c)),
+                      hi = (u32)max(a, max(b, c));
+            K[t] = degen ? pad : ((u64)lo << (2 * kb)) | ((u64)md << kb) | (u64)hi;
+            V[t] = (u32)t;
+        });
+        {
+            prof::Site site_("triangles");
+            radix_sort_pairs(ctx, keys, vals, valid, 3 * kb);
+        }
+        K = keys.get();  // an odd pass count hands back the scratch buffers
+        const u64 m1 = (1ull << kb) - 1ull;
+        // keep the first of each run of equal keys; decode the triple in place
+        select_into(
+            ctx, valid, [=] __device__(i64 q) { return K[q] != pad && (q == 0 || K[q] != K[q - 1]); },
+            [=] __device__(i64 q, int o) {
+                const u64 k = K[q];
+                A[o] = (int)(k >> (2 * kb));
+                B[o] = (int)((k >> kb) & m1);
+                C[o] = (int)(k & m1);
+            },
+            CNT);
+        return ctx.read1(CNT);
+    }
+    // ids too wide for one key: two stable passes, by third id and then by
+    // (first, second), on the ordered triples kept in S1..S3 (-1: repeated vertex)
     DBuf<int> s1 = ctx.buf<int>(valid), s2 = ctx.buf<int>(valid), s3 = ctx.buf<int>(valid);
     int* S1 = s1.get();
     int* S2 = s2.get();
     int* S3 = s3.get();
-    DBuf<int> cnt = ctx.buf<int>(2);  // [0] non-degenerate, [1] unique
-    int* CNT = cnt.get();
+    const u64 pad1 = (1ull << kb) - 1ull, pad2 = (1ull << (2 * kb)) - 1ull;
     for_each(ctx, valid, [=] __device__(i64 t) {
         const int a = A[t], b = B[t], c = C[t];
         const bool degen = a == b || a == c || b == c;
         S1[t] = degen ? -1 : min(a, min(b, c));
         S2[t] = degen ? -1 : max(min(a, b), min(max(a, b), c));
         S3[t] = degen ? -1 : max(a, max(b, c));
+        K[t] = degen ? pad1 : (u64)(u32)S3[t];
+        V[t] = (u32)t;
     });
-    reduce_to<int>(ctx, [=] __device__(i64 t) { return (int)(S1[t] >= 0); }, valid, 0, OpSum(), CNT);
-    DBuf<u64> keys = ctx.buf<u64>(valid);
-    DBuf<u32> vals = ctx.buf<u32>(valid);
-    u64* K = keys.get();
-    u32* V = vals.get();
-    if (3 * kb <= 64) {
-        const u64 pad = 3 * kb == 64 ? ~0ull : ((1ull << (3 * kb)) - 1ull);
-        for_each(ctx, valid, [=] __device__(i64 q) {
-            K[q] = S1[q] < 0 ? pad
-                             : ((u64)(u32)S1[q] << (2 * kb)) | ((u64)(u32)S2[q] << kb) | (u64)(u32)S3[q];
-            V[q] = (u32)q;
-        });
-        prof::Site site_("triangles");
-        radix_sort_pairs(ctx, keys, vals, valid, 3 * kb);
-    } else {
-        // two stable passes: by third id, then by (first, second)
-        const u64 pad1 = (1ull << kb) - 1ull, pad2 = (1ull << (2 * kb)) - 1ull;
-        for_each(ctx, valid, [=] __device__(i64 q) {
-            K[q] = S3[q] < 0 ? pad1 : (u64)(u32)S3[q];
-            V[q] = (u32)q;
-        });
-        prof::Site site_("triangles");
-        radix_sort_pairs(ctx, keys, vals, valid, kb);
-        for_each(ctx, valid, [=] __device__(i64 q) {
-            const u32 o = V[q];
-            K[q] = S1[o] < 0 ? pad2 : ((u64)(u32)S1[o] << kb) | (u64)(u32)S2[o];
-        });
-        radix_sort_pairs(ctx, keys, vals, valid, 2 * kb);
-    }
-    // first `real` sorted slots hold the non-degenerate triangles; keep uniques
+    prof::Site site_("triangles");
+    radix_sort_pairs(ctx, keys, vals, valid, kb);
+    K = keys.get();
+    V = vals.get();
+    for_each(ctx, valid, [=] __device__(i64 q) {
+        const u32 o = V[q];
+        K[q] = S1[o] < 0 ? pad2 : ((u64)(u32)S1[o] << kb) | (u64)(u32)S2[o];
+    });
+    radix_sort_pairs(ctx, keys, vals, valid, 2 * kb);
+    K = keys.get();
+    V = vals.get();
     select_into(
         ctx, valid,
         [=] __device__(i64 q) {
-            if (q >= CNT[0]) return false;
+            const u32 a = V[q];
+            if (S1[a] < 0) return false;
             if (q == 0) return true;
-            const u32 a = V[q], b = V[q - 1];
+            const u32 b = V[q - 1];
             return S1[a] != S1[b] || S2[a] != S2[b] || S3[a] != S3[b];
         },
         [=] __device__(i64 q, int o) {
@@ -288,8 +459,8 @@ static i64 rearrange(Ctx& ctx, Triangles& T, i64 valid, int n_nodes) {
             B[o] = S2[a];
             C[o] = S3[a];
         },
-        CNT + 1);
-    return ctx.read1(CNT + 1);
+        CNT);
+    return ctx.read1(CNT);
 }
 
 void normalize_triangles(Ctx& ctx, Triangles& T) { T.n = rearrange(ctx, T, T.n, 1 << 30); }
@@ -308,22 +479,25 @@ Triangles conflicted_cycles(Ctx& ctx, const DCoo& A, int max_len, float factor, 
     struct Sizes {
         float mn;
         float pad;
-        i64 pos;
-        i64 rep;
+        u64 counts;  // attractive count << 32 | repulsive count
     };
     DBuf<Sizes> sz = ctx.buf<Sizes>(1);
     Sizes* SZ = sz.get();
     reduce_to<float>(ctx, [=] __device__(i64 e) { return W[e]; }, A.nnz, 3.402823466e38f, OpMin(), &SZ->mn);
-    reduce_to<i64>(ctx, [=] __device__(i64 e) { return (i64)(W[e] > -1.0f * (SZ->mn * tol)); }, A.nnz, (i64)0,
-                   OpSum(), &SZ->pos);
-    reduce_to<i64>(ctx, [=] __device__(i64 e) { return (i64)(W[e] < SZ->mn * tol); }, A.nnz, (i64)0, OpSum(),
-                   &SZ->rep);
+    reduce_to<u64>(
+        ctx,
+        [=] __device__(i64 e) {
+            const float t = SZ->mn * tol;
+            return ((u64)(W[e] > -1.0f * t) << 32) | (u64)(W[e] < t);
+        },
+        A.nnz, (u64)0, OpSum(), &SZ->counts);
     Sizes h;
     ctx.read(SZ, &h, 1);
     const float thr = h.mn * tol;
-    const i64 R = h.rep;
-    if (h.pos == 0 || R == 0) return T;
-    Csr G = symmetric_csr(ctx, A.row.get(), A.col.get(), W, A.nnz, N, true, -1.0f * thr, h.pos);
+    const i64 npos = (i64)(h.counts >> 32);
+    const i64 R = (i64)(h.counts & 0xffffffffull);
+    if (npos == 0 || R == 0) return T;
+    Csr G = symmetric_csr(ctx, A.row.get(), A.col.get(), W, A.nnz, N, true, -1.0f * thr, npos);
     DBuf<int> rv1 = ctx.buf<int>(R), rv2 = ctx.buf<int>(R);
     {
         int* X1 = rv1.get();

@@ -79,6 +79,8 @@ DCoo dcoo_from_edges(Ctx& ctx, const int* row_in, const int* col_in, const float
     RAMA_REQUIRE(ctx.read1(badp) == 0, "node ids must be non-negative");
     prof::Site site_("init");
     radix_sort_pairs(ctx, keys, vals, m, 2 * kb);
+    K = keys.get();  // an odd pass count hands back the scratch buffers
+    V = vals.get();
     int* R = A.row.get();
     int* C = A.col.get();
     float* W = A.val.get();
@@ -159,6 +161,52 @@ __global__ __launch_bounds__(kBlock) void k_contract_reduce(const u64* __restric
     }
 }
 
+// Dimensions of a contraction in one launch: over the S = *dS segments,
+// out[1] = largest col, out[0] = largest col of a kept (off-diagonal)
+// segment, out[2] = row of the last segment.  Block maxima go to `part`; the
+// last block (atomic ticket, reset afterwards) combines them.
+__global__ __launch_bounds__(kBlock) void k_contract_dims(const u64* __restrict__ okey, const int* __restrict__ keep,
+                                                         const int* dS, int kb, int* part, unsigned* ticket,
+                                                         int* out) {
+    __shared__ int ws[kWaves];
+    __shared__ bool last;
+    const i64 S = *dS;
+    const u64 mask = ((u64)1 << kb) - 1;
+    int all = -1, kept = -1;
+    for (i64 s = (i64)blockIdx.x * kBlock + threadIdx.x; s < S; s += (i64)gridDim.x * kBlock) {
+        const int c = (int)(okey[s] & mask);
+        all = c > all ? c : all;
+        if (keep[s] && c > kept) kept = c;
+    }
+    all = block_reduce(all, OpMax(), ws);
+    kept = block_reduce(kept, OpMax(), ws);
+    if (threadIdx.x == 0) {
+        __hip_atomic_store(&part[2 * blockIdx.x], all, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        __hip_atomic_store(&part[2 * blockIdx.x + 1], kept, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        const unsigned t = __hip_atomic_fetch_add(ticket, 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
+        last = t == gridDim.x - 1;
+    }
+    __syncthreads();
+    if (!last) return;
+    __threadfence();  // acquire: the other blocks' partials are visible to plain loads
+    all = -1;
+    kept = -1;
+    for (unsigned b = threadIdx.x; b < gridDim.x; b += kBlock) {
+        const int a = part[2 * b];
+        const int k = part[2 * b + 1];
+        all = a > all ? a : all;
+        kept = k > kept ? k : kept;
+    }
+    all = block_reduce(all, OpMax(), ws);
+    kept = block_reduce(kept, OpMax(), ws);
+    if (threadIdx.x == 0) {
+        out[1] = all;
+        out[0] = kept;
+        out[2] = S > 0 ? (int)(okey[S - 1] >> kb) : -1;
+        *ticket = 0;
+    }
+}
+
 // dCOO::contract_cuda + has_bad_contractions + remove_diagonal
 // (src/dCOO.cu:57-95,97-142; src/rama_cuda.cu:18-30,136-140).
 // Counts stay on the device until one read-back at the end.
@@ -178,21 +226,26 @@ DCoo contract(Ctx& ctx, const DCoo& A, const int* node_map, int max_label, bool*
     const int* R = A.row.get();
     const int* C = A.col.get();
     const float* W = A.val.get();
+    // scalars on device: [0] S, [1] S2, [2] max kept col, [3] max col, [4] last row, [5] bad; diag as double
+    DBuf<int> sc = ctx.buf<int>(8);
+    DBuf<double> dg = ctx.buf<double>(1);
+    int* SC = sc.get();
+    double* DG = dg.get();
     for_each(ctx, n, [=] __device__(i64 e) {
         const int a = node_map[R[e]], b = node_map[C[e]];
         const u32 lo = (u32)(a < b ? a : b), hi = (u32)(a < b ? b : a);
         K[e] = ((u64)lo << kb) | hi;
         V[e] = f2u(W[e]);
+        if (e == 0) {
+            for (int k = 0; k < 8; ++k) SC[k] = 0;
+            *DG = 0.0;
+        }
     });
     prof::Site site_("contract");
     radix_sort_pairs(ctx, keys, vals, n, 2 * kb);
+    K = keys.get();  // an odd pass count hands back the scratch buffers
+    V = vals.get();
 
-    // scalars on device: [0] S, [1] S2, [2] max kept col, [3] max col, [4] last row, [5] bad; diag as double
-    DBuf<int> sc = ctx.buf<int>(8);
-    DBuf<double> dg = ctx.buf<double>(1);
-    ctx.zero(sc.get(), 8);
-    ctx.zero(dg.get(), 1);
-    int* SC = sc.get();
     DBuf<int> seg_start = ctx.buf<int>(n);
     int* SS = seg_start.get();
     select_into(
@@ -229,10 +282,12 @@ DCoo contract(Ctx& ctx, const DCoo& A, const int* node_map, int max_label, bool*
         SC + 1);
     // dims as the reference computes them before removing the diagonal
     // (src/dCOO.cu:91-92), and the largest endpoint among the kept edges
-    reduce_to<int>(ctx, [=] __device__(i64 s) { return s < SC[0] ? (int)(OK[s] & mask) : -1; }, n, -1, OpMax(), SC + 3);
-    reduce_to<int>(ctx, [=] __device__(i64 s) { return (s < SC[0] && KP[s]) ? (int)(OK[s] & mask) : -1; }, n, -1,
-                   OpMax(), SC + 2);
-    for_each(ctx, 1, [=] __device__(i64) { SC[4] = (int)(OK[SC[0] - 1] >> kb); });
+    {
+        const unsigned nb = (unsigned)std::min<i64>(std::max<i64>((n + kBlock - 1) / kBlock, 1), kReduceMaxBlocks);
+        int* part = static_cast<int*>(ctx.reduce_scratch((size_t)kReduceMaxBlocks * 8));
+        k_contract_dims<<<nb, kBlock, 0, ctx.stream>>>(OK, KP, SC + 0, kb, part, ctx.reduce_ticket(), SC + 2);
+        RAMA_LAUNCHED();
+    }
     int h[8];
     ctx.read(SC, h, 8);
     B.nnz = h[1];
@@ -327,33 +382,43 @@ Csr symmetric_csr(Ctx& ctx, const int* row_in, const int* col_in, const float* v
     });
     prof::Site site_("csr");
     radix_sort_pairs<u32>(ctx, tk, tv, P, bits_for(n_nodes > 1 ? n_nodes - 1 : 1));
+    TK = tk.get();  // an odd pass count hands back the scratch buffers
+    TV = tv.get();
     DBuf<int> offF = ctx.buf<int>((i64)n_nodes + 1), offT = ctx.buf<int>((i64)n_nodes + 1);
-    compute_offsets(ctx, row, P, n_nodes - 1, offF.get());
-    {
-        auto colof = [=] __device__(i64 i) { return (int)TK[i]; };
-        k_offsets<<<grid_for(P + 1), kBlock, 0, ctx.stream>>>(colof, P, n_nodes, offT.get());
-        RAMA_LAUNCHED();
-    }
+    int* OFw = offF.get();
+    int* OTw = offT.get();
+    // row offsets of the forward part (edge rows) and of the backward part
+    // (sorted cols), one launch
+    for_each(ctx, P + 1, [=] __device__(i64 i) {
+        const int pr = i > 0 ? row[i - 1] : -1, cr = i < P ? row[i] : n_nodes;
+        for (int r = pr + 1; r <= cr && r <= n_nodes; ++r) OFw[r] = (int)i;
+        const int pc = i > 0 ? (int)TK[i - 1] : -1, cc = i < P ? (int)TK[i] : n_nodes;
+        for (int r = pc + 1; r <= cc && r <= n_nodes; ++r) OTw[r] = (int)i;
+    });
     int* OFF = G.off.get();
-    const int* OF = offF.get();
-    const int* OT = offT.get();
-    for_each(ctx, (i64)n_nodes + 1, [=] __device__(i64 r) { OFF[r] = OF[r] + OT[r]; });
+    const int* OF = OFw;
+    const int* OT = OTw;
     int* ADJ = G.adj.get();
     float* WW = G.w.get();
-    // backward part of row r: transposed entries, in order
-    for_each(ctx, P, [=] __device__(i64 k) {
-        const int r = (int)TK[k];
-        const u32 e = TV[k];
-        const i64 o = (i64)OFF[r] + (k - OT[r]);
-        ADJ[o] = row[e];
-        WW[o] = val[e];
-    });
-    // forward part of row r: the edge list's row segment
-    for_each(ctx, P, [=] __device__(i64 e) {
-        const int r = row[e];
-        const i64 o = (i64)OFF[r] + (OT[r + 1] - OT[r]) + (e - OF[r]);
-        ADJ[o] = col[e];
-        WW[o] = val[e];
+    // one launch: row offsets of the symmetric matrix; backward part of row
+    // r (transposed entries, in order); forward part (the row's segment)
+    for_each(ctx, std::max<i64>(P, (i64)n_nodes + 1), [=] __device__(i64 i) {
+        if (i <= n_nodes) OFF[i] = OF[i] + OT[i];
+        if (i < P) {
+            {
+                const int r = (int)TK[i];
+                const u32 e = TV[i];
+                const i64 o = (i64)OF[r] + OT[r] + (i - OT[r]);
+                ADJ[o] = row[e];
+                WW[o] = val[e];
+            }
+            {
+                const int r = row[i];
+                const i64 o = (i64)OF[r] + OT[r] + (OT[r + 1] - OT[r]) + (i - OF[r]);
+                ADJ[o] = col[i];
+                WW[o] = val[i];
+            }
+        }
     });
     return G;
 }
@@ -368,6 +433,18 @@ int compress_labels(Ctx& ctx, const int* data, i64 n, int max_label, int* out, b
     scan_exclusive_arr<int>(ctx, M, M, (i64)max_label + 1);
     for_each(ctx, n, [=] __device__(i64 k) { out[k] = M[data[k]]; });
     return want_count ? ctx.read1(M + max_label + 1) : -1;
+}
+
+// compress_label_sequence of a matching map (map[v] = v, or the smaller end
+// of v's pair): the labels in use are exactly the v with map[v] == v, so
+// their ranks come from one scan without marking a mask first.  Equal to
+// compress_labels(map, n, n - 1, out).
+void compress_matching(Ctx& ctx, const int* map, i64 n, int* out) {
+    if (n <= 0) return;
+    DBuf<int> rank = ctx.buf<int>(n + 1);
+    int* RK = rank.get();
+    scan_exclusive<int>(ctx, [=] __device__(i64 v) { return (int)(map[v] == (int)v); }, RK, n);
+    for_each(ctx, n, [=] __device__(i64 v) { out[v] = RK[map[v]]; });
 }
 
 // map_node_labels (src/rama_cuda.cu:32-52)
@@ -399,6 +476,8 @@ Sanitized sanitize_graph(Ctx& ctx, DBuf<int>& i, DBuf<int>& j, DBuf<float>& c, i
     });
     prof::Site site_("sanitize");
     radix_sort_pairs(ctx, keys, vals, m, 2 * kb);
+    K = keys.get();  // an odd pass count hands back the scratch buffers
+    V = vals.get();
     DBuf<int> pos;
     const i64 m2 = select_positions(
         ctx, [=] __device__(i64 e) { return (int)(e == 0 || K[e] != K[e - 1]); }, m, pos);

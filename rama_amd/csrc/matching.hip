@@ -50,7 +50,7 @@ __global__ __launch_bounds__(kBlock) void k_pick_best(int n, float mult, int rou
                                                       int* __restrict__ best) {
     float thr;
     if (!match_round_active(st, mult, round, &thr)) return;  // uniform over the grid
-    const i64 t = (i64)blockIdx.x * kBlock + threadIdx.x;
+    const i64 t = (i64)xcd_block() * kBlock + threadIdx.x;
     const int v = (int)(t / kGroup);
     const int sub = (int)(t % kGroup);
     const bool active = v < n && !matched[v];
@@ -121,22 +121,63 @@ __global__ __launch_bounds__(kBlock) void k_match(int n, float mult, int round, 
 int matching_vertex_based(Ctx& ctx, const Csr& U, float mean_multiplier, DBuf<int>& map) {
     const int n = U.n;
     map = ctx.buf<int>(n);
-    iota(ctx, map.get(), n);
     if (n == 0) return 0;
     DBuf<MatchState> stb = ctx.buf<MatchState>(1);
     MatchState* st = stb.get();
-    ctx.zero(st, 1);
+    DBuf<int> best = ctx.buf<int>(n), matched = ctx.buf<int>(n);
+    {
+        // one launch: identity mapping, no best neighbour, nothing matched,
+        // cleared loop state
+        int* M = map.get();
+        int* B = best.get();
+        int* D = matched.get();
+        for_each(ctx, n, [=] __device__(i64 v) {
+            M[v] = (int)v;
+            B[v] = -1;
+            D[v] = 0;
+            if (v == 0) {
+                st->pos_sum = 0.0;
+                st->pos_cnt = 0.0;
+                for (int r = 0; r < kMaxRounds; ++r) {
+                    st->newly[r] = 0;
+                    st->running[r] = 0;
+                }
+            }
+        });
+    }
     const float* W = U.w.get();
     reduce_to<double>(
         ctx, [=] __device__(i64 e) { const float x = W[e]; return x >= 0.0f ? (double)x : 0.0; }, U.nnz, 0.0, OpSum(),
         &st->pos_sum);
     reduce_to<double>(ctx, [=] __device__(i64 e) { return W[e] >= 0.0f ? 1.0 : 0.0; }, U.nnz, 0.0, OpSum(),
                       &st->pos_cnt);
-    DBuf<int> best = ctx.buf<int>(n), matched = ctx.buf<int>(n);
-    fill(ctx, best.get(), n, -1);
-    ctx.zero(matched.get(), n);
     const i64 threads = (i64)n * kGroup;
+    // the stop rule runs on the device; the host only skips launches once it
+    // knows the loop has ended: the first kFirstRounds rounds are queued,
+    // the state is read, and the remaining rounds are queued only if the
+    // loop is still running
+    constexpr int kFirstRounds = 3;
+    auto stopped = [&](const MatchState& h, int rounds) {
+        if (!(h.pos_cnt > 0.0) || mean_multiplier * (float)h.pos_sum / (float)(i64)h.pos_cnt < 0.0f) return true;
+        int prev = 0;
+        for (int t = 0; t < rounds; ++t) {
+            const int cur = prev + h.newly[t];
+            const float rel = (cur - prev) / (prev + 1.0f);
+            prev = cur;
+            if (!h.running[t] || rel < 0.1f) return true;
+        }
+        return false;
+    };
+    MatchState h;
+    bool have = false;  // h holds the final state
     for (int t = 0; t < kMaxRounds; ++t) {
+        if (t == kFirstRounds) {
+            ctx.read(st, &h, 1);
+            if (stopped(h, kFirstRounds)) {
+                have = true;
+                break;
+            }
+        }
         {
             // offsets + (neighbour, cost, neighbour's matched flag) per CSR entry + best per vertex
             prof::Scope ps(ctx, "match_pick_best", 12.0 * U.nnz + 12.0 * n);
@@ -149,8 +190,7 @@ int matching_vertex_based(Ctx& ctx, const Csr& U, float mean_multiplier, DBuf<in
                                                         map.get());
         RAMA_LAUNCHED();
     }
-    MatchState h;
-    ctx.read(st, &h, 1);
+    if (!have) ctx.read(st, &h, 1);
     // replay the stop rule on the host for the returned count
     int prev = 0;
     if (h.pos_cnt > 0.0 && mean_multiplier * (float)h.pos_sum / (float)(i64)h.pos_cnt >= 0.0f) {

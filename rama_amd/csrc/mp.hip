@@ -13,20 +13,53 @@
 //   k_mp_edges      per edge: cost = sum of its slot messages in fixed order
 //                   (replaces float atomicAdd), plus the edge's
 //                   lower-bound term.
-// Lower-bound partials are per block in double; the host reads one scalar
-// per iteration for the stall test.
+// Lower-bound partials are per block in double.  In the device-driven loop
+// the edge kernel's last block sums them and applies the stall test, so the
+// host reads the outcome once per loop.
 #include "rama_internal.hpp"
 
 namespace rama {
 
+// What an edge in `cnt` triangles passes to each of them
+// (send_messages_to_triplets, src/multicut_message_passing.cu:218-229).
+__device__ __forceinline__ float edge_share(float c, int cnt) { return c / float(cnt); }
+
+
 template <class K>
-__device__ __forceinline__ i64 lower_bound_dev(const K* a, i64 n, K x) {
-    i64 lo = 0, hi = n;
+__device__ __forceinline__ i64 lower_bound_dev(const K* a, i64 lo, i64 hi, K x) {
     while (lo < hi) {
         const i64 mid = (lo + hi) >> 1;
         if (a[mid] < x) lo = mid + 1; else hi = mid;
     }
     return lo;
+}
+template <class K>
+__device__ __forceinline__ i64 lower_bound_dev(const K* a, i64 n, K x) {
+    return lower_bound_dev(a, (i64)0, n, x);
+}
+
+// out[i] = lower_bound(a[0..n), q[i]) for ascending queries q[0..m).  Each
+// block bounds its 256 answers by the answers of its first and last query
+// (two full searches), so the per-thread searches run in a narrow, cached
+// window instead of walking the whole array.
+template <class K>
+__global__ __launch_bounds__(kBlock) void k_sorted_lb(const K* __restrict__ a, i64 n, const K* __restrict__ q, i64 m,
+                                                      int* __restrict__ out) {
+    __shared__ i64 range[2];
+    const i64 b0 = (i64)blockIdx.x * kBlock;
+    const i64 b1 = (b0 + kBlock < m ? b0 + kBlock : m) - 1;
+    if (threadIdx.x == 0) range[0] = lower_bound_dev(a, n, q[b0]);
+    if (threadIdx.x == 64) range[1] = lower_bound_dev(a, n, q[b1]);
+    __syncthreads();
+    const i64 i = b0 + threadIdx.x;
+    if (i <= b1) out[i] = (int)lower_bound_dev(a, range[0], range[1], q[i]);
+}
+
+template <class K>
+static void sorted_lb(Ctx& ctx, const K* a, i64 n, const K* q, i64 m, int* out) {
+    if (m <= 0) return;
+    k_sorted_lb<K><<<grid_for(m), kBlock, 0, ctx.stream>>>(a, n, q, m, out);
+    RAMA_LAUNCHED();
 }
 
 // Builds the message-passing problem.  The graph's edges are already sorted
@@ -63,10 +96,19 @@ static void mp_build_impl(Ctx& ctx, const DCoo& A, Triangles& tri, MPState& S, i
     });
     prof::Site site_("mp_build");
     radix_sort_pairs<K>(ctx, kt, vt, N3, 2 * kb);
+    KT = kt.get();  // an odd pass count hands back the scratch buffers
+    VT = vt.get();
+    // position of every sorted triangle item among the graph keys, and of
+    // every graph key among the sorted triangle items
+    DBuf<int> pa = ctx.buf<int>(N3 > 0 ? N3 : 1), pt = ctx.buf<int>(E > 0 ? E : 1);
+    int* PA = pa.get();
+    int* PT = pt.get();
+    sorted_lb<K>(ctx, KA, E, KT, N3, PA);
+    sorted_lb<K>(ctx, KT, N3, KA, E, PT);
     // chords: first item of a key run whose key is not a graph edge
     auto is_chord = [=] __device__(i64 q) {
         if (q > 0 && KT[q] == KT[q - 1]) return 0;
-        const i64 p = lower_bound_dev(KA, E, KT[q]);
+        const i64 p = PA[q];
         return (int)(p == E || KA[p] != KT[q]);
     };
     DBuf<int> cpos;
@@ -76,12 +118,13 @@ static void mp_build_impl(Ctx& ctx, const DCoo& A, Triangles& tri, MPState& S, i
     K* CK = ck.get();
     int* CH = chead.get();
     const int* CP = cpos.get();
-    for_each(ctx, N3, [=] __device__(i64 q) {
-        if (is_chord(q)) {
-            CK[CP[q]] = KT[q];
-            CH[CP[q]] = (int)q;
-        }
-    });
+    if (Nc > 0)
+        for_each(ctx, N3, [=] __device__(i64 q) {
+            if (is_chord(q)) {
+                CK[CP[q]] = KT[q];
+                CH[CP[q]] = (int)q;
+            }
+        });
     const i64 Em = E + Nc;
     S.E = Em;
     S.ei = ctx.buf<int>(Em);
@@ -96,9 +139,6 @@ static void mp_build_impl(Ctx& ctx, const DCoo& A, Triangles& tri, MPState& S, i
     S.t13 = ctx.buf<float>(T);
     S.t23 = ctx.buf<float>(T);
     S.delta = ctx.buf<float>(N3 > 0 ? N3 : 1);
-    ctx.zero(S.t12.get(), T);
-    ctx.zero(S.t13.get(), T);
-    ctx.zero(S.t23.get(), T);
     int* EI = S.ei.get();
     int* EJ = S.ej.get();
     float* EC = S.ec.get();
@@ -107,11 +147,12 @@ static void mp_build_impl(Ctx& ctx, const DCoo& A, Triangles& tri, MPState& S, i
     const u64 mask = ((u64)1 << kb) - 1;
     // graph edges
     for_each(ctx, E, [=] __device__(i64 e) {
-        const i64 u = e + lower_bound_dev(CK, Nc, KA[e]);
+        const i64 u = e + (Nc > 0 ? lower_bound_dev(CK, Nc, KA[e]) : 0);
         EI[u] = AR[e];
         EJ[u] = AC[e];
         EC[u] = AV[e];
-        SOFF[u] = (int)lower_bound_dev(KT, N3, KA[e]);
+        SOFF[u] = PT[e];
+        if (e == 0) SOFF[Em] = (int)N3;
     });
     // chords (cost 0)
     for_each(ctx, Nc, [=] __device__(i64 c) {
@@ -121,21 +162,30 @@ static void mp_build_impl(Ctx& ctx, const DCoo& A, Triangles& tri, MPState& S, i
         EC[u] = 0.0f;
         SOFF[u] = CH[c];
     });
-    for_each(ctx, 1, [=] __device__(i64) { SOFF[Em] = (int)N3; });
-    for_each(ctx, Em, [=] __device__(i64 u) { ECNT[u] = SOFF[u + 1] - SOFF[u]; });
+    if (E == 0) for_each(ctx, 1, [=] __device__(i64) { SOFF[Em] = (int)N3; });
+    S.share = ctx.buf<float>(Em);
+    float* SH = S.share.get();
+    for_each(ctx, Em, [=] __device__(i64 u) {
+        const int cnt = SOFF[u + 1] - SOFF[u];
+        ECNT[u] = cnt;
+        SH[u] = cnt > 0 ? edge_share(EC[u], cnt) : 0.0f;
+    });
     // triangle -> edge correspondences
     int* C12 = S.c12.get();
     int* C13 = S.c13.get();
     int* C23 = S.c23.get();
+    float* TC12 = S.t12.get();
+    float* TC13 = S.t13.get();
+    float* TC23 = S.t23.get();
     for_each(ctx, N3, [=] __device__(i64 q) {
         const K k = KT[q];
-        const int u = (int)(lower_bound_dev(KA, E, k) + lower_bound_dev(CK, Nc, k));
+        const int u = (int)(PA[q] + (Nc > 0 ? lower_bound_dev(CK, Nc, k) : 0));
         const u32 v = VT[q], t = v / 3, s = v % 3;
         (s == 0 ? C12 : (s == 1 ? C13 : C23))[t] = u;
+        (s == 0 ? TC12 : (s == 1 ? TC13 : TC23))[t] = 0.0f;  // triangle costs start at zero
     });
     // the sorted slot ids are the edges' slot lists
-    S.slots = ctx.buf<int>(N3 > 0 ? N3 : 1);
-    ctx.d2d(S.slots.get(), reinterpret_cast<const int*>(VT), N3);
+    S.slots = std::move(vt);
 }
 
 void mp_build(Ctx& ctx, const DCoo& A, Triangles& tri, MPState& S) {
@@ -155,7 +205,6 @@ void mp_build(Ctx& ctx, const DCoo& A, Triangles& tri, MPState& S) {
     if (S.E > 0) {
         const int* EJ = S.ej.get();
         reduce_to<int>(ctx, [=] __device__(i64 e) { return EJ[e]; }, S.E, -1, OpMax(), S.dims.get());
-        ctx.d2d(S.dims.get() + 1, S.ei.get() + S.E - 1, 1);
     }
 }
 
@@ -170,8 +219,8 @@ struct MPControl {
     int last_row;
 };
 
-// After an iteration: sum the fused lower-bound partials (fixed order) and
-// apply the stall test `lb - prev < 1e-3` that stops the loop.
+// After an iteration on a large grid: sum the lower-bound partials in the
+// same fixed order as the fused path and apply the stall test.
 __global__ __launch_bounds__(kBlock) void k_mp_finalize(const double* __restrict__ parts, i64 n, MPControl* ctl) {
     __shared__ double ws[kWaves];
     if (!ctl->active) return;
@@ -203,27 +252,27 @@ __device__ __forceinline__ float min_marginal(float x, float y, float z) {
     return mm1 - mm0;
 }
 
-// send_messages_to_triplets (:218-229) for one triangle.
-__device__ __forceinline__ void absorb(const float* __restrict__ ec, const int* __restrict__ ecnt, int e, float& t) {
-    t += ec[e] / float(ecnt[e]);
-}
+// send_messages_to_triplets (:218-229) for one triangle: each edge passes
+// cost / #triangles (edge_share); the edge kernel stores that share next to
+// the cost, so a triangle gathers one value per edge.
 
 template <bool kSweep>
 __global__ __launch_bounds__(kBlock) void k_mp_triangles(i64 T, const int* __restrict__ c12, const int* __restrict__ c13,
-                                                         const int* __restrict__ c23, const float* __restrict__ ec,
-                                                         const int* __restrict__ ecnt, float* __restrict__ t12,
-                                                         float* __restrict__ t13, float* __restrict__ t23,
-                                                         float* __restrict__ delta, double* __restrict__ lbpart,
+                                                         const int* __restrict__ c23, const float* __restrict__ share,
+                                                         float* __restrict__ t12, float* __restrict__ t13,
+                                                         float* __restrict__ t23, float* __restrict__ delta,
+                                                         double* __restrict__ lbpart,
                                                          const MPControl* __restrict__ ctl) {
     __shared__ double ws[kWaves];
     if (ctl && !ctl->active) return;  // uniform: the loop has stopped
-    const i64 t = (i64)blockIdx.x * kBlock + threadIdx.x;
+    const unsigned blk = xcd_block();
+    const i64 t = (i64)blk * kBlock + threadIdx.x;
     double lb = 0.0;
     if (t < T) {
         float a = t12[t], b = t13[t], d = t23[t];
-        absorb(ec, ecnt, c12[t], a);
-        absorb(ec, ecnt, c13[t], b);
-        absorb(ec, ecnt, c23[t], d);
+        a += share[c12[t]];
+        b += share[c13[t]];
+        d += share[c23[t]];
         if (kSweep) {
             // decrease_triangle_costs_func (:282-332): steps with weights
             // 1/3, 1/2, 1, 1/2, 1, 1 over slots 12, 13, 23, 12, 13, 12.
@@ -268,37 +317,67 @@ __global__ __launch_bounds__(kBlock) void k_mp_triangles(i64 T, const int* __res
         lb = (double)tri_lb(a, b, d);
     }
     const double r = block_reduce(lb, OpSum(), ws);
-    if (threadIdx.x == 0) lbpart[blockIdx.x] = r;
+    if (threadIdx.x == 0) lbpart[blk] = r;
 }
 
 // Edge side: with kSweep, cost = sum of the slot messages (edges in no
 // triangle keep their cost); without, edges in a triangle are zeroed
 // (decrease_edge_costs_func, :231-239).  Adds the edge lower-bound term.
+//
+// With `parts` set (the device-driven loop on small grids), the last block to
+// finish (atomic ticket, reset afterwards) also sums all nparts lower-bound
+// partials of the iteration in a fixed order and applies the stall test
+// `lb - prev < 1e-3` (src/dual_solver.cu:17-26) to ctl, as k_mp_finalize does.
+// Each block's ticket carries a release, whose cost grows with the grid, so
+// large grids finalise in a separate one-block launch instead.
 template <bool kSweep>
-__global__ __launch_bounds__(kBlock) void k_mp_edges(i64 E, const int* __restrict__ ecnt, const int* __restrict__ soff,
-                                                     const int* __restrict__ slots, const float* __restrict__ delta,
-                                                     float* __restrict__ ec, double* __restrict__ lbpart,
-                                                     const MPControl* __restrict__ ctl) {
+__global__ __launch_bounds__(kBlock) void k_mp_edges(i64 E, const int* __restrict__ soff, const u32* __restrict__ slots,
+                                                     const float* __restrict__ delta, float* __restrict__ ec,
+                                                     float* __restrict__ share, double* __restrict__ lbpart,
+                                                     MPControl* ctl, const double* parts, i64 nparts,
+                                                     unsigned* ticket) {
     __shared__ double ws[kWaves];
+    __shared__ bool last;
     if (ctl && !ctl->active) return;
-    const i64 e = (i64)blockIdx.x * kBlock + threadIdx.x;
+    const unsigned blk = xcd_block();
+    const i64 e = (i64)blk * kBlock + threadIdx.x;
     double lb = 0.0;
     if (e < E) {
         float c = ec[e];
-        if (ecnt[e] > 0) {
+        const int s0 = soff[e], s1 = soff[e + 1];
+        if (s1 > s0) {
             if (kSweep) {
                 double s = 0.0;
-                for (int k = soff[e]; k < soff[e + 1]; ++k) s += (double)delta[slots[k]];
+                for (int k = s0; k < s1; ++k) s += (double)delta[slots[k]];
                 c = (float)s;
             } else {
                 c = 0.0f;
             }
             ec[e] = c;
+            share[e] = edge_share(c, s1 - s0);
         }
         lb = c < 0.0f ? (double)c : 0.0;
     }
     const double r = block_reduce(lb, OpSum(), ws);
-    if (threadIdx.x == 0) lbpart[blockIdx.x] = r;
+    if (threadIdx.x == 0) lbpart[blk] = r;
+    if (!parts) return;
+    if (threadIdx.x == 0) {
+        const unsigned t = __hip_atomic_fetch_add(ticket, 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
+        last = t == gridDim.x - 1;
+    }
+    __syncthreads();
+    if (!last) return;
+    __threadfence();  // acquire: the other blocks' partials are visible to plain loads
+    double sum = 0.0;
+    for (i64 k = threadIdx.x; k < nparts; k += kBlock) sum += parts[k];
+    const double tot = block_reduce(sum, OpSum(), ws);
+    if (threadIdx.x == 0) {
+        ctl->prev = ctl->lb;
+        ctl->lb = tot;
+        ctl->iterations += 1;
+        if (tot - ctl->prev < 1e-3) ctl->active = 0;
+        *ticket = 0;
+    }
 }
 
 static double sum_parts(Ctx& ctx, MPState& S, i64 n) {
@@ -308,31 +387,48 @@ static double sum_parts(Ctx& ctx, MPState& S, i64 n) {
     return ctx.read1(P + slot);
 }
 
-static void launch_round(Ctx& ctx, MPState& S, bool sweep, const MPControl* ctl = nullptr) {
+static void launch_round(Ctx& ctx, MPState& S, bool sweep, MPControl* ctl = nullptr) {
     const i64 T = S.T, E = S.E;
     const unsigned gt = grid_for(T), ge = grid_for(E);
     double* P = S.lbparts.get();
-    // per triangle: 3 edge ids, 3 x (cost, count) gathers, 3 costs in and
-    // out, 3 messages out = 72 B
-    prof::Scope ps1(ctx, sweep ? "mp_triangles" : "mp_triangles_absorb", 72.0 * T);
-    if (sweep)
-        k_mp_triangles<true><<<gt, kBlock, 0, ctx.stream>>>(T, S.c12.get(), S.c13.get(), S.c23.get(), S.ec.get(),
-                                                             S.ecnt.get(), S.t12.get(), S.t13.get(), S.t23.get(),
-                                                             S.delta.get(), P, ctl);
-    else
-        k_mp_triangles<false><<<gt, kBlock, 0, ctx.stream>>>(T, S.c12.get(), S.c13.get(), S.c23.get(), S.ec.get(),
-                                                              S.ecnt.get(), S.t12.get(), S.t13.get(), S.t23.get(),
-                                                              S.delta.get(), P, ctl);
-    RAMA_LAUNCHED();
-    // per edge: count, offsets, cost in/out; per slot: slot id + message
-    prof::Scope ps2(ctx, sweep ? "mp_edges" : "mp_edges_zero", 16.0 * E + 24.0 * T);
-    if (sweep)
-        k_mp_edges<true><<<ge, kBlock, 0, ctx.stream>>>(E, S.ecnt.get(), S.soff.get(), S.slots.get(), S.delta.get(),
-                                                         S.ec.get(), P + gt, ctl);
-    else
-        k_mp_edges<false><<<ge, kBlock, 0, ctx.stream>>>(E, S.ecnt.get(), S.soff.get(), S.slots.get(), S.delta.get(),
-                                                          S.ec.get(), P + gt, ctl);
-    RAMA_LAUNCHED();
+    {
+        // per triangle: 3 edge ids, 3 share gathers, 3 costs in and out,
+        // 3 messages out = 60 B
+        prof::Scope ps(ctx, sweep ? "mp_triangles" : "mp_triangles_absorb", 60.0 * T);
+        if (sweep)
+            k_mp_triangles<true><<<gt, kBlock, 0, ctx.stream>>>(T, S.c12.get(), S.c13.get(), S.c23.get(),
+                                                                 S.share.get(), S.t12.get(), S.t13.get(),
+                                                                 S.t23.get(), S.delta.get(), P, ctl);
+        else
+            k_mp_triangles<false><<<gt, kBlock, 0, ctx.stream>>>(T, S.c12.get(), S.c13.get(), S.c23.get(),
+                                                                  S.share.get(), S.t12.get(), S.t13.get(),
+                                                                  S.t23.get(), S.delta.get(), P, ctl);
+        RAMA_LAUNCHED();
+    }
+    // the device-driven loop (ctl set) finalises each iteration in the edge
+    // kernel on small grids, in a separate launch on large ones
+    constexpr unsigned kFusedFinalizeMaxBlocks = 256;
+    const bool fused = ctl && ge <= kFusedFinalizeMaxBlocks;
+    const double* parts = fused ? P : nullptr;
+    const i64 nparts = (i64)gt + (i64)ge;
+    unsigned* ticket = fused ? ctx.reduce_ticket() : nullptr;
+    {
+        // per edge: offsets, cost in/out, share out; per slot: slot id + message
+        prof::Scope ps(ctx, sweep ? "mp_edges" : "mp_edges_zero", 16.0 * E + 24.0 * T);
+        if (sweep)
+            k_mp_edges<true><<<ge, kBlock, 0, ctx.stream>>>(E, S.soff.get(), S.slots.get(), S.delta.get(),
+                                                             S.ec.get(), S.share.get(), P + gt, ctl, parts, nparts,
+                                                             ticket);
+        else
+            k_mp_edges<false><<<ge, kBlock, 0, ctx.stream>>>(E, S.soff.get(), S.slots.get(), S.delta.get(),
+                                                              S.ec.get(), S.share.get(), P + gt, ctl, parts, nparts,
+                                                              ticket);
+        RAMA_LAUNCHED();
+    }
+    if (ctl && !fused) {
+        k_mp_finalize<<<1, kBlock, 0, ctx.stream>>>(P, nparts, ctl);
+        RAMA_LAUNCHED();
+    }
 }
 
 // lower_bound() (:201-216) of the current state.
@@ -371,19 +467,17 @@ double mp_run(Ctx& ctx, MPState& S, int steps) {
                       P + 1);
     MPControl* C = ctl.get();
     const int* DM = S.E > 0 ? S.dims.get() : nullptr;
+    const int* LR = S.E > 0 ? S.ei.get() + (S.E - 1) : nullptr;
     for_each(ctx, 1, [=] __device__(i64) {
         C->lb = P[0] + P[1];
         C->prev = 0.0;
         C->active = 1;
         C->iterations = 0;
         C->max_col = DM ? DM[0] : -1;
-        C->last_row = DM ? DM[1] : -1;
+        C->last_row = LR ? *LR : -1;
     });
-    const i64 nparts = (i64)grid_for(S.T) + (i64)grid_for(S.E);
     for (int it = 0; it < steps; ++it) {
         launch_round(ctx, S, true, C);
-        k_mp_finalize<<<1, kBlock, 0, ctx.stream>>>(P, nparts, C);
-        RAMA_LAUNCHED();
     }
     MPControl h;
     ctx.read(C, &h, 1);

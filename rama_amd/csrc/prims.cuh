@@ -16,6 +16,19 @@ constexpr int kWaves = kBlock / 64;
 
 __device__ __forceinline__ int lane_id() { return __lane_id(); }
 
+// Logical block index that gives each XCD a contiguous range of blocks.
+// Workgroups are dispatched round-robin over the 8 XCDs (block b runs on
+// XCD b % 8), and each XCD has its own L2; with this remap neighbouring
+// blocks — which touch neighbouring edges and nodes — share an L2.  A
+// bijection on [0, gridDim.x).
+__device__ __forceinline__ unsigned xcd_block() {
+    constexpr unsigned kXcd = 8;
+    const unsigned b = blockIdx.x, g = gridDim.x;
+    if (g < 2 * kXcd) return b;
+    const unsigned q = g / kXcd, rem = g % kXcd, x = b % kXcd, k = b / kXcd;
+    return x * q + (x < rem ? x : rem) + k;
+}
+
 template <class T>
 __device__ __forceinline__ T shfl_up64(T x, int d) {
     return __shfl_up(x, d, 64);
@@ -117,23 +130,40 @@ __device__ __forceinline__ u64 scan_word(u32 epoch, u32 flag, u64 v) {
 
 // out(i, exclusive_prefix, value) is called for every i < n; total (if not
 // null) receives the sum of all values.
+// Items are evaluated and emitted in striped order (item k*kBlock + t of the
+// tile by thread t), so the loads inside f and the stores inside out are
+// coalesced across each wave; the scan itself runs on the blocked order
+// (thread t owns items t*kScanItems .. +kScanItems-1) after a transpose
+// through LDS (row stride kScanItems + 1 keeps both orders conflict-free).
+constexpr int kScanPad = kScanItems + 1;
+__device__ __forceinline__ int scan_slot(int j) { return (j / kScanItems) * kScanPad + j % kScanItems; }
+
 template <class T, class F, class Out>
 __global__ __launch_bounds__(kBlock) void k_scan_single(F f, i64 n, Out out, T* total, u64* status, int* ctl,
                                                         u32 epoch) {
     __shared__ T ws[kWaves];
     __shared__ int tile_s;
     __shared__ T prefix_s;
+    __shared__ T sv[kBlock * kScanPad];
     if (threadIdx.x == 0) tile_s = atomicAdd(&ctl[0], 1);
     __syncthreads();
     const int tile = tile_s;
-    const i64 base = (i64)tile * kScanTile + (i64)threadIdx.x * kScanItems;
+    const i64 tile0 = (i64)tile * kScanTile;
     T v[kScanItems];
-    T s = T(0);
 #pragma unroll
     for (int k = 0; k < kScanItems; ++k) {
-        const i64 i = base + k;
+        const int j = k * kBlock + threadIdx.x;
+        const i64 i = tile0 + j;
         v[k] = i < n ? (T)f(i) : T(0);
-        s += v[k];
+        sv[scan_slot(j)] = v[k];
+    }
+    __syncthreads();
+    T s = T(0);
+    T bl[kScanItems];
+#pragma unroll
+    for (int k = 0; k < kScanItems; ++k) {
+        bl[k] = sv[threadIdx.x * kScanPad + k];
+        s += bl[k];
     }
     T agg;
     const T excl = block_excl_scan(s, ws, &agg);
@@ -192,12 +222,19 @@ __global__ __launch_bounds__(kBlock) void k_scan_single(F f, i64 n, Out out, T* 
         }
     }
     __syncthreads();
+    // blocked exclusive prefixes back into LDS, then emit in striped order
     T run = prefix_s + excl;
 #pragma unroll
     for (int k = 0; k < kScanItems; ++k) {
-        const i64 i = base + k;
-        if (i < n) out(i, run, v[k]);
-        run += v[k];
+        sv[threadIdx.x * kScanPad + k] = run;
+        run += bl[k];
+    }
+    __syncthreads();
+#pragma unroll
+    for (int k = 0; k < kScanItems; ++k) {
+        const int j = k * kBlock + threadIdx.x;
+        const i64 i = tile0 + j;
+        if (i < n) out(i, sv[scan_slot(j)], v[k]);
     }
 }
 
@@ -274,9 +311,9 @@ __global__ __launch_bounds__(kBlock) void k_reduce1(F f, i64 n, T ident, Op op, 
     }
     __syncthreads();
     if (!last) return;
+    __threadfence();  // acquire: the other blocks' partials are visible to plain loads
     T a = ident;
-    for (unsigned b = threadIdx.x; b < gridDim.x; b += kBlock)
-        a = op(a, __hip_atomic_load(&partials[b], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT));
+    for (unsigned b = threadIdx.x; b < gridDim.x; b += kBlock) a = op(a, partials[b]);
     const T total = block_reduce(a, op, ws);
     if (threadIdx.x == 0) {
         *dst = total;

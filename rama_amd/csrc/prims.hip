@@ -154,12 +154,18 @@ struct PassPlan {
 // pass); larger ones the three-launch pass.
 constexpr int kOnesweepMaxTiles = 64;
 
-// Digit histograms of every pass in one read of the keys.
+// Digit histograms of every pass in one read of the keys.  Each block
+// stores its partial histograms (row blockIdx.x of `part`); the last block
+// to finish (atomic ticket, reset for the next launch) adds the rows in
+// block order into `hist`, so nothing needs clearing beforehand.
 template <class K>
 __global__ __launch_bounds__(kBlock) void k_radix_hist_all(const K* __restrict__ keys, i64 n, PassPlan plan,
-                                                           int* __restrict__ hist) {
+                                                           int* __restrict__ part, int* __restrict__ hist,
+                                                           unsigned* ticket) {
     __shared__ int h[8 * 256];
-    for (int k = threadIdx.x; k < plan.passes * 256; k += kBlock) h[k] = 0;
+    __shared__ bool last;
+    const int nb = plan.passes * 256;
+    for (int k = threadIdx.x; k < nb; k += kBlock) h[k] = 0;
     __syncthreads();
     for (i64 i = (i64)blockIdx.x * kBlock + threadIdx.x; i < n; i += (i64)gridDim.x * kBlock) {
         const K key = keys[i];
@@ -167,8 +173,23 @@ __global__ __launch_bounds__(kBlock) void k_radix_hist_all(const K* __restrict__
             atomicAdd(&h[p * 256 + (int)((u32)(key >> plan.shift[p]) & ((1u << plan.width[p]) - 1u))], 1);
     }
     __syncthreads();
-    for (int k = threadIdx.x; k < plan.passes * 256; k += kBlock)
-        if (h[k]) atomicAdd(&hist[k], h[k]);
+    int* mine = part + (i64)blockIdx.x * nb;
+    for (int k = threadIdx.x; k < nb; k += kBlock)
+        __hip_atomic_store(&mine[k], h[k], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    __syncthreads();  // the row is stored before thread 0 releases it with the ticket
+    if (threadIdx.x == 0) {
+        const unsigned t = __hip_atomic_fetch_add(ticket, 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
+        last = t == gridDim.x - 1;
+    }
+    __syncthreads();
+    if (!last) return;
+    __threadfence();  // acquire: the other blocks' rows are visible to plain loads
+    for (int k = threadIdx.x; k < nb; k += kBlock) {
+        int sum = 0;
+        for (unsigned b = 0; b < gridDim.x; ++b) sum += part[(i64)b * nb + k];
+        hist[k] = sum;
+    }
+    if (threadIdx.x == 0) *ticket = 0;
 }
 
 // One onesweep pass.  Tiles are drawn from an atomic counter (every smaller
@@ -314,9 +335,11 @@ __global__ __launch_bounds__(kBlock) void k_onesweep(const K* __restrict__ kin, 
     }
 }
 
+// Sorts and returns whether the result ended in (tk, tv) instead of
+// (keys, vals) (odd number of passes).
 template <class K>
-void radix_sort_pairs(Ctx& ctx, K* keys, u32* vals, K* tk, u32* tv, i64 n, int bits) {
-    if (n <= 1 || bits <= 0) return;
+static bool radix_sort_core(Ctx& ctx, K* keys, u32* vals, K* tk, u32* tv, i64 n, int bits) {
+    if (n <= 1 || bits <= 0) return false;
     RAMA_REQUIRE(n < ((i64)1 << 31), "radix sort supports fewer than 2^31 elements");
     RAMA_REQUIRE(bits <= (int)(8 * sizeof(K)), "radix sort key wider than its type");
     const int ntiles = (int)((n + kRsTile - 1) / kRsTile);
@@ -353,18 +376,14 @@ void radix_sort_pairs(Ctx& ctx, K* keys, u32* vals, K* tk, u32* tv, i64 n, int b
             std::swap(ka, kb);
             std::swap(va, vb);
         }
-        if (ka != keys) {
-            ctx.d2d(keys, ka, n);
-            ctx.d2d(vals, va, n);
-        }
-        return;
+        return ka != keys;
     }
-    DBuf<int> hist = ctx.buf<int>(plan.passes * 256);
-    ctx.zero(hist.get(), plan.passes * 256);
+    const unsigned hb = (unsigned)std::min<i64>(ntiles, 1024);
+    DBuf<int> hist = ctx.buf<int>((i64)(hb + 1) * plan.passes * 256);
+    int* hsum = hist.get() + (i64)hb * plan.passes * 256;
     {
         prof::Scope ps(ctx, "radix_hist_all", (double)sizeof(K) * n);
-        const unsigned hb = (unsigned)std::min<i64>(ntiles, 1024);
-        k_radix_hist_all<K><<<hb, kBlock, 0, ctx.stream>>>(keys, n, plan, hist.get());
+        k_radix_hist_all<K><<<hb, kBlock, 0, ctx.stream>>>(keys, n, plan, hist.get(), hsum, ctx.reduce_ticket());
         RAMA_LAUNCHED();
     }
     for (int p = 0; p < plan.passes; ++p) {
@@ -374,24 +393,34 @@ void radix_sort_pairs(Ctx& ctx, K* keys, u32* vals, K* tk, u32* tv, i64 n, int b
             // reads and writes one (key, u32 value) pair per element
             prof::Scope ps(ctx, "radix_onesweep", 2.0 * pair_bytes * n);
             k_onesweep<K><<<ntiles, kBlock, 0, ctx.stream>>>(ka, va, kb, vb, n, plan.shift[p], plan.width[p],
-                                                             hist.get() + p * 256, st, ctx.scan_counter.get(), epoch);
+                                                             hsum + p * 256, st, ctx.scan_counter.get(), epoch);
             RAMA_LAUNCHED();
         }
         std::swap(ka, kb);
         std::swap(va, vb);
     }
-    if (ka != keys) {
-        ctx.d2d(keys, ka, n);
-        ctx.d2d(vals, va, n);
+    return ka != keys;
+}
+
+template <class K>
+void radix_sort_pairs(Ctx& ctx, K* keys, u32* vals, K* tk, u32* tv, i64 n, int bits) {
+    if (radix_sort_core<K>(ctx, keys, vals, tk, tv, n, bits)) {
+        ctx.d2d(keys, tk, n);
+        ctx.d2d(vals, tv, n);
     }
 }
 
+// Buffer form: an odd pass count hands over the scratch buffers instead of
+// copying back.
 template <class K>
 void radix_sort_pairs(Ctx& ctx, DBuf<K>& keys, DBuf<u32>& vals, i64 n, int bits) {
     if (n <= 1) return;
     DBuf<K> tk = ctx.buf<K>(n);
     DBuf<u32> tv = ctx.buf<u32>(n);
-    radix_sort_pairs<K>(ctx, keys.get(), vals.get(), tk.get(), tv.get(), n, bits);
+    if (radix_sort_core<K>(ctx, keys.get(), vals.get(), tk.get(), tv.get(), n, bits)) {
+        keys = std::move(tk);
+        vals = std::move(tv);
+    }
 }
 
 template void radix_sort_pairs<u32>(Ctx&, u32*, u32*, u32*, u32*, i64, int);

@@ -61,6 +61,7 @@ Csr symmetric_csr(Ctx& ctx, const int* row, const int* col, const float* val, i6
 void compute_offsets(Ctx& ctx, const int* sorted_ids, i64 n, int max_value, int* out);
 // Returns the number of distinct labels if want_count (one read-back), else -1.
 int compress_labels(Ctx& ctx, const int* data, i64 n, int max_label, int* out, bool want_count = true);
+void compress_matching(Ctx& ctx, const int* map, i64 n, int* out);
 void map_node_labels(Ctx& ctx, const int* cur, i64 cur_size, int* orig, i64 orig_size);
 double dcoo_lb(Ctx& ctx, const float* val, i64 n);
 float dcoo_min(Ctx& ctx, const float* val, i64 n);
@@ -92,14 +93,15 @@ struct MessagePassing;  // opaque
 struct MPState {
     DBuf<int> ei, ej;            // message-passing edges (sorted, unique)
     DBuf<float> ec;              // edge costs
+    DBuf<float> share;           // ec / ecnt for edges in a triangle (what each triangle absorbs)
     DBuf<int> ecnt;              // #triangles per edge
     DBuf<int> soff;              // edge -> slot list offsets (E+1)
-    DBuf<int> slots;             // slot ids 3t+k in (triangle, slot) order
+    DBuf<u32> slots;             // slot ids 3t+k, grouped by edge in (triangle, slot) order
     DBuf<int> c12, c13, c23;     // triangle -> edge index
     DBuf<float> t12, t13, t23;   // triangle costs
     DBuf<float> delta;           // per-slot messages (3T)
     DBuf<double> lbparts;        // partial lower bounds
-    DBuf<int> dims;              // [0] max col, [1] last row of (ei, ej), on device
+    DBuf<int> dims;              // [0] max col of (ei, ej), on device
     int max_col = -1, last_row = -1;  // host copies once read back (-1: not yet)
     i64 E = 0, T = 0;
 };

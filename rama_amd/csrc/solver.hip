@@ -91,7 +91,7 @@ SolveResult solve(Ctx& ctx, DCoo& A, const Options& o) {
             DBuf<int> m;
             n_contract = matching_vertex_based(ctx, U, o.mean_multiplier_mm, m);
             cur = ctx.buf<int>(Un);
-            compress_labels(ctx, m.get(), Un, Un - 1, cur.get(), false);
+            compress_matching(ctx, m.get(), Un, cur.get());
             max_label = Un - 1;  // bound on the compressed labels (sizes the sort keys)
             cur_size = Un;
             if ((float)n_contract < (float)A.rows * o.matching_thresh_crossover_ratio) {
