@@ -99,7 +99,10 @@ def copy_mode():
 
 def batch_mode(a):
     """512 instances (n log-uniform in [1e3, 2e5]; even: Voronoi grids, odd:
-    Erdos-Renyi), batched on one GPU vs the same instances one at a time."""
+    Erdos-Renyi), batched on one GPU (lockstep packed solve; `workers`
+    threads continue the instances that switch to the spanning-forest
+    contraction) vs the same instances solved one at a time by the plain
+    host entry.  Both include the host-to-device copies."""
     from rama_amd import _capi, instances
 
     sizes = instances.batch_sizes(a.count, 1e3, 2e5, seed=12345)
@@ -107,8 +110,11 @@ def batch_mode(a):
     E = int(sum(len(x[2]) for x in insts))
     opts = _capi.options("PD", verbose=0)
     _capi.solve_batch(insts[:4], opts, workers=1)  # warm up the runtime
+    for x in insts[:4]:
+        _capi.solve(*x, opts)
+    # the plain form, one instance after the other
     t = time.perf_counter()
-    one = _capi.solve_batch(insts, opts, workers=1)
+    one = [_capi.solve(*x, opts)[:2] for x in insts]
     t_one = time.perf_counter() - t
     t = time.perf_counter()
     bat = _capi.solve_batch(insts, opts, workers=a.workers)

@@ -15,6 +15,7 @@
 // for the whole solve) and publishes the labels when done.  Latency is
 // measured from submission to labels back on the host.
 #include <atomic>
+#include <cstdlib>
 #include <chrono>
 #include <condition_variable>
 #include <deque>
@@ -26,6 +27,8 @@
 #include "rama_internal.hpp"
 
 namespace rama {
+// the packed (lockstep) batch engine lives in lockstep.hip
+void solve_packed(Ctx& ctx, PackedBatch& B, int workers);
 // solve_common lives in capi.hip
 void solve_common(Ctx& ctx, DBuf<int>& i, DBuf<int>& j, DBuf<float>& c, i64 m, const rama_options_c* copts,
                   DBuf<int>& labels, i64& n_labels, double* lb_out, int32_t* time_ms, rama_timeline** tl);
@@ -246,11 +249,36 @@ int rama_solve_batch_host(int32_t k, const int64_t* edge_off, const int32_t* i, 
             (void)hipGetLastError();
             throw Error(RAMA_ERR_NO_DEVICE, "no HIP device available");
         }
+        std::mutex emu;
+        std::string first_error;
+        // lockstep path: every instance in one packed graph (per-instance
+        // results equal solving alone); options it does not carry (time
+        // budget, timelines, sanitising, printing) use the per-instance path
+        const char* mode = std::getenv("RAMA_BATCH_MODE");
+        const bool packed = k >= 2 && !(mode && std::string(mode) == "workers") && opts->sanitize_graph == 0 &&
+                            opts->dump_timeline == 0 && opts->max_time_sec < 0 && opts->verbose == 0;
+        if (packed) {
+            for (int32_t q = 0; q < k; ++q) {
+                n_labels[q] = 0;
+                lbs[q] = 0.0;
+                if (status) status[q] = 0;
+            }
+            std::vector<int32_t> st((size_t)k, 0);
+            PackedBatch B{k,        edge_off, i,     j,   c,    to_options(*opts), labels_out, label_off, n_labels,
+                          lbs,      st.data(), &first_error, &emu};
+            Ctx ctx(0, nullptr, false);
+            solve_packed(ctx, B, workers);
+            bool failed = false;
+            for (int32_t q = 0; q < k; ++q) {
+                if (status) status[q] = st[q];
+                failed = failed || st[q] != 0;
+            }
+            if (failed) throw Error(RAMA_ERR_INTERNAL, "batch: " + first_error);
+            return RAMA_OK;
+        }
         std::atomic<int32_t> next{0};
         std::atomic<int> failed{0};
         std::vector<std::thread> pool;
-        std::mutex emu;
-        std::string first_error;
         auto work = [&] {
             std::unique_ptr<Ctx> ctx;
             try {

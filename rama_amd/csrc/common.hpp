@@ -5,6 +5,7 @@
 
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cstdint>
 #include <cstring>
 #include <cstdio>
@@ -216,6 +217,7 @@ struct Ctx {
         if (stream) (void)hipStreamSynchronize(stream);
         arena.trim();
         pool::give_pinned(pinned);
+        if (up_pinned) (void)hipHostFree(up_pinned);
         if (own_stream) (void)hipStreamDestroy(stream);
     }
     Ctx(const Ctx&) = delete;
@@ -269,6 +271,42 @@ struct Ctx {
     void sync() {
         ++syncs;
         RAMA_HIP(hipStreamSynchronize(stream));
+    }
+
+    // Host-to-device upload through a pinned staging ring, asynchronous on
+    // the stream.  A staging slot is reused only after a synchronisation
+    // (every copy queued before it has run); a full ring synchronises.
+    void* up_pinned = nullptr;
+    size_t up_cap = 0, up_off = 0;
+    int64_t up_sync = -1;
+    template <class T>
+    void upload(T* d, const T* h, i64 n) {
+        const size_t bytes = sizeof(T) * (size_t)n;
+        if (bytes == 0) return;
+        const size_t need = (bytes + 255) & ~(size_t)255;
+        if (up_sync != syncs) {
+            up_off = 0;
+            up_sync = syncs;
+        }
+        if (up_cap < need) {
+            if (up_pinned) {
+                sync();
+                (void)hipHostFree(up_pinned);
+                up_pinned = nullptr;
+            }
+            up_cap = std::max<size_t>((size_t)1 << 20, 2 * need);
+            RAMA_HIP(hipHostMalloc(&up_pinned, up_cap, hipHostMallocDefault));
+            up_off = 0;
+            up_sync = syncs;
+        } else if (up_off + need > up_cap) {
+            sync();
+            up_off = 0;
+            up_sync = syncs;
+        }
+        char* slot = static_cast<char*>(up_pinned) + up_off;
+        std::memcpy(slot, h, bytes);
+        RAMA_HIP(hipMemcpyAsync(d, slot, bytes, hipMemcpyHostToDevice, stream));
+        up_off += need;
     }
 
     // Read `n` device scalars (contiguous) into host memory (synchronises).

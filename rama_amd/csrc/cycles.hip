@@ -30,7 +30,36 @@ struct CycArgs {
     const int* tile_rep; // first rep edge of each tile
     i64 R;
     i64 X;               // number of expansions
+    // Several instances packed in one graph (disjoint node ranges): nslot
+    // maps a node to its instance; the count kernels add each instance's
+    // attempted writes to slot_total, and the emit kernel places instance q's
+    // writes at slot_out[q] + slot_base[q] + (global position - slot_first[q]),
+    // keeping those below slot_budget[q].  All null for a single instance.
+    const int* nslot = nullptr;
+    i64* slot_total = nullptr;
+    const i64* slot_first = nullptr;
+    const i64* slot_base = nullptr;
+    const i64* slot_budget = nullptr;
+    const i64* slot_out = nullptr;
 };
+
+// Adds v to base[key] for every lane with `on`, one atomic per distinct key
+// in the wave (lanes of a wave mostly share their instance).  Wave-uniform
+// call.
+__device__ __forceinline__ void wave_add_keyed(i64* base, int key, i64 v, bool on) {
+    const int lane = lane_id();
+    unsigned long long active = __ballot(on && v != 0);
+    while (active) {
+        const int leader = __builtin_ctzll(active);
+        const int kl = __builtin_amdgcn_readlane(key, leader);
+        const unsigned long long m = __ballot(key == kl) & active;
+        i64 x = ((m >> lane) & 1ull) ? v : 0;
+#pragma unroll
+        for (int d = 32; d >= 1; d >>= 1) x += shfl_down64(x, d);
+        if (lane == 0) atomicAdd(reinterpret_cast<unsigned long long*>(base + kl), (unsigned long long)x);
+        active &= ~m;
+    }
+}
 
 // Locate the rep edge owning expansion x: xoff[r] <= x < xoff[r+1], searching
 // from the tile's first rep edge.
@@ -161,6 +190,7 @@ __global__ __launch_bounds__(kBlock) void k_cycle_count3(CycArgs a, unsigned cha
         }
         cnt8[r] = (unsigned char)(c < 255 ? c : 255);
     }
+    if (a.nslot) wave_add_keyed(a.slot_total, r < a.R ? a.nslot[a.rv1[r]] : 0, (i64)c, r < a.R);
     const i64 tot = block_reduce((i64)c, OpSum(), ws);
     if (threadIdx.x == 0) tile_tot[blk] = tot;
 }
@@ -171,22 +201,28 @@ __global__ __launch_bounds__(kBlock) void k_cycle_count3(CycArgs a, unsigned cha
 __global__ __launch_bounds__(kBlock) void k_cycle_count4(CycArgs a, unsigned char* __restrict__ cnt8,
                                                          i64* __restrict__ tile_tot) {
     const i64 r = (i64)xcd_block() * kBlock + threadIdx.x;
-    if (r >= a.R) return;
-    const int v1 = a.rv1[r], v2 = a.rv2[r];
-    const int b1 = a.off[v1], d1 = a.off[v1 + 1] - b1;
-    const int b2 = a.off[v2], n2 = a.off[v2 + 1] - b2;
-    const i64 x0 = a.xoff[r];
-    int rb[kRegList];
-    const bool regs = n2 <= kRegList;
-    if (regs) load_reg_list(a.adj, b2, n2, rb);
-    TileAcc acc;
-    for (int l = 0; l < d1; ++l) {
-        const int n1 = a.adj[b1 + l];
-        const int c = regs ? count_vs_regs(a.adj, a.off[n1], a.off[n1 + 1], rb) : count_common(a.off, a.adj, n1, v2);
-        cnt8[x0 + l] = (unsigned char)(c < 255 ? c : 255);
-        acc.add(tile_tot, x0 + l, 2 * (i64)c);
+    const bool on = r < a.R;
+    i64 mine = 0;
+    if (on) {
+        const int v1 = a.rv1[r], v2 = a.rv2[r];
+        const int b1 = a.off[v1], d1 = a.off[v1 + 1] - b1;
+        const int b2 = a.off[v2], n2 = a.off[v2 + 1] - b2;
+        const i64 x0 = a.xoff[r];
+        int rb[kRegList];
+        const bool regs = n2 <= kRegList;
+        if (regs) load_reg_list(a.adj, b2, n2, rb);
+        TileAcc acc;
+        for (int l = 0; l < d1; ++l) {
+            const int n1 = a.adj[b1 + l];
+            const int c =
+                regs ? count_vs_regs(a.adj, a.off[n1], a.off[n1 + 1], rb) : count_common(a.off, a.adj, n1, v2);
+            cnt8[x0 + l] = (unsigned char)(c < 255 ? c : 255);
+            acc.add(tile_tot, x0 + l, 2 * (i64)c);
+            mine += 2 * (i64)c;
+        }
+        acc.flush(tile_tot);
     }
-    acc.flush(tile_tot);
+    if (a.nslot) wave_add_keyed(a.slot_total, on ? a.nslot[a.rv1[r]] : 0, mine, on);
 }
 
 // 5-cycles: one thread per (repulsive edge r, n2 = l2-th neighbour of v2);
@@ -201,7 +237,10 @@ __global__ __launch_bounds__(kBlock) void k_cycle_count5(CycArgs a, const i64* _
                                                          i64* __restrict__ tile_tot) {
     const unsigned blk = xcd_block();
     const i64 y = (i64)blk * kCycTile + threadIdx.x;
-    if (y >= Y) return;
+    const bool on = y < Y;
+    i64 mine = 0;
+    int slot = 0;
+    if (on) {
     // rep owning thread y (galloping from the block's first rep)
     i64 lo = ytile_rep[blk], hi = a.R - 1, step = 1;
     while (lo + step <= hi && yoff[lo + step] <= y) {
@@ -231,8 +270,12 @@ __global__ __launch_bounds__(kBlock) void k_cycle_count5(CycArgs a, const i64* _
             c = regs ? count_vs_regs(a.adj, a.off[n1], a.off[n1 + 1], rb) : count_common(a.off, a.adj, n1, n2);
         cnt8[x0 + l] = (unsigned char)(c < 255 ? c : 255);
         acc.add(tile_tot, x0 + l, 3 * (i64)c);
+        mine += 3 * (i64)c;
     }
     acc.flush(tile_tot);
+    if (a.nslot) slot = a.nslot[v1];
+    }
+    if (a.nslot) wave_add_keyed(a.slot_total, slot, mine, on);
 }
 
 __device__ __forceinline__ void put_tri(int* t1, int* t2, int* t3, i64 g, i64 budget, int a, int b, int c) {
@@ -254,8 +297,9 @@ __global__ __launch_bounds__(kBlock) void k_cycle_emit(CycArgs a, const unsigned
     // plain block order: tiles past the budget exit at once, and keeping the
     // working tiles spread over all XCDs balances them
     const unsigned blk = blockIdx.x;
-    const i64 tstart = base + tile_off[blk];
-    if (tstart >= budget) return;  // uniform per block
+    const bool packed = a.nslot != nullptr;
+    const i64 tstart = (packed ? 0 : base) + tile_off[blk];
+    if (!packed && tstart >= budget) return;  // uniform per block
     const i64 x = (i64)blk * kCycTile + threadIdx.x;
     int c = 0;
     int v1 = 0, v2 = 0, n1 = 0, n2 = 0, qa = 0, qb = 0;
@@ -267,8 +311,18 @@ __global__ __launch_bounds__(kBlock) void k_cycle_emit(CycArgs a, const unsigned
         }
     }
     i64 tot;
-    const i64 mypos = tstart + block_excl_scan((i64)c * (S - 2), ws, &tot);
-    if (c == 0 || mypos >= budget) return;
+    i64 mypos = tstart + block_excl_scan((i64)c * (S - 2), ws, &tot);
+    if (c == 0) return;
+    if (packed) {
+        // instance q's own position, budget and region
+        const int q = a.nslot[v1];
+        mypos = a.slot_base[q] + (mypos - a.slot_first[q]);
+        budget = a.slot_budget[q];
+        t1 += a.slot_out[q];
+        t2 += a.slot_out[q];
+        t3 += a.slot_out[q];
+    }
+    if (mypos >= budget) return;
     int i = a.off[qa], ie = a.off[qa + 1], j = a.off[qb], je = a.off[qb + 1];
     i64 g = mypos;
     while (i < ie && j < je && g < budget) {
@@ -370,10 +424,96 @@ static i64 run_stage(Ctx& ctx, const Csr& G, const int* rv1, const int* rv2, i64
     return W;
 }
 
+// One stage over several packed instances.  Instance q's writes go to its
+// triangle region (d_out[q] onward) at positions continuing from d_base[q],
+// below d_budget[q]; h_total[q] receives its attempted writes (the stage's
+// return value in the single-instance form).
+template <int S>
+void run_stage_packed(Ctx& ctx, const Csr& G, const int* rv1, const int* rv2, i64 R, const int* nslot, int Q,
+                      const i64* d_base, const i64* d_budget, const i64* d_out, int* t1, int* t2, int* t3,
+                      i64* h_total) {
+    for (int q = 0; q < Q; ++q) h_total[q] = 0;
+    if (R == 0) return;
+    const int* OFF = G.off.get();
+    DBuf<i64> xoff = ctx.buf<i64>(R + 1);
+    if (S == 3) {
+        iota(ctx, xoff.get(), R + 1);
+    } else if (S == 4) {
+        scan_exclusive<i64>(ctx, [=] __device__(i64 r) { const int v = rv1[r]; return (i64)(OFF[v + 1] - OFF[v]); },
+                            xoff.get(), R);
+    } else {
+        scan_exclusive<i64>(
+            ctx,
+            [=] __device__(i64 r) {
+                const int v = rv1[r], u = rv2[r];
+                return (i64)(OFF[v + 1] - OFF[v]) * (i64)(OFF[u + 1] - OFF[u]);
+            },
+            xoff.get(), R);
+    }
+    const i64 X = S == 3 ? R : ctx.read1(xoff.get() + R);
+    if (X == 0) return;
+    const i64 ntiles = (X + kCycTile - 1) / kCycTile;
+    RAMA_REQUIRE(ntiles < ((i64)1 << 31), "too many cycle expansions");
+    DBuf<int> tile_rep = ctx.buf<int>(ntiles);
+    k_tile_rep<<<grid_for(R), kBlock, 0, ctx.stream>>>(xoff.get(), R, tile_rep.get());
+    RAMA_LAUNCHED();
+    DBuf<i64> slot = ctx.buf<i64>(2 * (i64)Q + 1);  // [0, Q) totals, [Q, 2Q] their exclusive scan
+    i64* TOT = slot.get();
+    ctx.zero(TOT, Q);
+    CycArgs a{OFF, G.adj.get(), rv1, rv2, xoff.get(), tile_rep.get(), R, X};
+    a.nslot = nslot;
+    a.slot_total = TOT;
+    a.slot_first = TOT + Q;
+    a.slot_base = d_base;
+    a.slot_budget = d_budget;
+    a.slot_out = d_out;
+    DBuf<unsigned char> cnt8 = ctx.buf<unsigned char>(X);
+    DBuf<i64> tile = ctx.buf<i64>(ntiles + 1);
+    if (S == 3) {
+        k_cycle_count3<<<(unsigned)ntiles, kBlock, 0, ctx.stream>>>(a, cnt8.get(), tile.get());
+        RAMA_LAUNCHED();
+    } else if (S == 4) {
+        ctx.zero(tile.get(), ntiles);
+        k_cycle_count4<<<grid_for(R), kBlock, 0, ctx.stream>>>(a, cnt8.get(), tile.get());
+        RAMA_LAUNCHED();
+    } else {
+        DBuf<i64> yoff = ctx.buf<i64>(R + 1);
+        scan_exclusive<i64>(ctx, [=] __device__(i64 r) { const int u = rv2[r]; return (i64)(OFF[u + 1] - OFF[u]); },
+                            yoff.get(), R);
+        const i64 Y = ctx.read1(yoff.get() + R);
+        ctx.zero(tile.get(), ntiles);
+        if (Y > 0) {
+            const i64 ytiles = (Y + kCycTile - 1) / kCycTile;
+            RAMA_REQUIRE(ytiles < ((i64)1 << 31), "too many cycle expansions");
+            DBuf<int> ytile_rep = ctx.buf<int>(ytiles);
+            k_tile_rep<<<grid_for(R), kBlock, 0, ctx.stream>>>(yoff.get(), R, ytile_rep.get());
+            RAMA_LAUNCHED();
+            k_cycle_count5<<<(unsigned)ytiles, kBlock, 0, ctx.stream>>>(a, yoff.get(), ytile_rep.get(), Y,
+                                                                        cnt8.get(), tile.get());
+            RAMA_LAUNCHED();
+        }
+    }
+    scan_exclusive_arr<i64>(ctx, tile.get(), tile.get(), ntiles);
+    scan_exclusive_arr<i64>(ctx, TOT, TOT + Q, Q);
+    ctx.d2h(h_total, TOT, Q);
+    i64 W = 0;
+    for (int q = 0; q < Q; ++q) W += h_total[q];
+    if (W > 0) {
+        k_cycle_emit<S><<<(unsigned)ntiles, kBlock, 0, ctx.stream>>>(a, cnt8.get(), tile.get(), 0, 0, t1, t2, t3);
+        RAMA_LAUNCHED();
+    }
+}
+template void run_stage_packed<3>(Ctx&, const Csr&, const int*, const int*, i64, const int*, int, const i64*,
+                                  const i64*, const i64*, int*, int*, int*, i64*);
+template void run_stage_packed<4>(Ctx&, const Csr&, const int*, const int*, i64, const int*, int, const i64*,
+                                  const i64*, const i64*, int*, int*, int*, i64*);
+template void run_stage_packed<5>(Ctx&, const Csr&, const int*, const int*, i64, const int*, int, const i64*,
+                                  const i64*, const i64*, int*, int*, int*, i64*);
+
 // rearrange_triangles / normalize_triangles (include/rama_utils.h:325-363)
 // on the first `valid` slots: drop triangles with a repeated vertex, sort
 // lexicographically, remove duplicates.  Returns the new count.
-static i64 rearrange(Ctx& ctx, Triangles& T, i64 valid, int n_nodes) {
+i64 rearrange_triangles(Ctx& ctx, Triangles& T, i64 valid, int n_nodes) {
     if (valid <= 0) return 0;
     int* A = T.t1.get();
     int* B = T.t2.get();
@@ -463,7 +603,7 @@ static i64 rearrange(Ctx& ctx, Triangles& T, i64 valid, int n_nodes) {
     return ctx.read1(CNT);
 }
 
-void normalize_triangles(Ctx& ctx, Triangles& T) { T.n = rearrange(ctx, T, T.n, 1 << 30); }
+void normalize_triangles(Ctx& ctx, Triangles& T) { T.n = rearrange_triangles(ctx, T, T.n, 1 << 30); }
 
 // conflicted_cycles_cuda (src/conflicted_cycles_cuda.cu:222-343), followed by
 // the normalisation the message-passing constructor applies.
@@ -519,17 +659,17 @@ Triangles conflicted_cycles(Ctx& ctx, const DCoo& A, int max_len, float factor, 
     i64 count = run_stage<3>(ctx, G, rv1.get(), rv2.get(), R, 0, budget, T);
     if (ctx.verbose) std::printf("3-cycles: found # of triangles: %lld, budget: %lld\n", (long long)count, (long long)budget);
     if (max_len >= 4 && count < budget) {
-        count = rearrange(ctx, T, count, N);
+        count = rearrange_triangles(ctx, T, count, N);
         count += run_stage<4>(ctx, G, rv1.get(), rv2.get(), R, count, budget, T);
         if (ctx.verbose) std::printf("4-cycles: found # of triangles: %lld, budget: %lld\n", (long long)count, (long long)budget);
     }
     if (max_len >= 5 && count < budget) {
-        count = rearrange(ctx, T, count, N);
+        count = rearrange_triangles(ctx, T, count, N);
         count += run_stage<5>(ctx, G, rv1.get(), rv2.get(), R, count, budget, T);
         if (ctx.verbose) std::printf("5-cycles: found # of triangles: %lld, budget: %lld\n", (long long)count, (long long)budget);
     }
     T.raw = count < budget ? count : budget;
-    T.n = rearrange(ctx, T, T.raw, N);
+    T.n = rearrange_triangles(ctx, T, T.raw, N);
     T.max_id = N - 1;
     return T;
 }

@@ -328,8 +328,9 @@ void compute_offsets(Ctx& ctx, const int* sorted_ids, i64 n, int max_value, int*
 // ascending rows), on 8-byte (u32 col, u32 edge) pairs.  The second part is
 // the edge list itself.  Duplicated edges keep their input order, as the
 // reference's stable sort of the doubled list does.
-Csr symmetric_csr(Ctx& ctx, const int* row_in, const int* col_in, const float* val_in, i64 m, int n_nodes,
-                  bool filter, float keep_above, i64 known_count) {
+template <class Keep>
+static Csr symmetric_csr_impl(Ctx& ctx, const int* row_in, const int* col_in, const float* val_in, i64 m,
+                              int n_nodes, bool filter, Keep keep, i64 known_count) {
     Csr G;
     G.n = n_nodes;
     const int* row = row_in;
@@ -341,7 +342,7 @@ Csr symmetric_csr(Ctx& ctx, const int* row_in, const int* col_in, const float* v
     if (filter) {
         DBuf<int> cnt = ctx.buf<int>(1);
         if (known_count < 0) {
-            P = reduce<i64>(ctx, [=] __device__(i64 e) { return (i64)(val_in[e] > keep_above); }, m, (i64)0, OpSum());
+            P = reduce<i64>(ctx, [=] __device__(i64 e) { return (i64)keep(e); }, m, (i64)0, OpSum());
         } else {
             P = known_count;
         }
@@ -352,7 +353,7 @@ Csr symmetric_csr(Ctx& ctx, const int* row_in, const int* col_in, const float* v
         int* FC = fc.get();
         float* FV = fv.get();
         select_into(
-            ctx, m, [=] __device__(i64 e) { return val_in[e] > keep_above; },
+            ctx, m, [=] __device__(i64 e) { return keep(e); },
             [=] __device__(i64 e, int q) {
                 FR[q] = row_in[e];
                 FC[q] = col_in[e];
@@ -423,6 +424,20 @@ Csr symmetric_csr(Ctx& ctx, const int* row_in, const int* col_in, const float* v
     return G;
 }
 
+Csr symmetric_csr(Ctx& ctx, const int* row, const int* col, const float* val, i64 m, int n_nodes, bool filter,
+                  float keep_above, i64 known_count) {
+    return symmetric_csr_impl(ctx, row, col, val, m, n_nodes, filter,
+                              [=] __device__(i64 e) { return val[e] > keep_above; }, known_count);
+}
+
+// Packed instances: edge e is kept when its cost exceeds the threshold of
+// its instance, keep_above[nslot[row[e]]].
+Csr symmetric_csr_slots(Ctx& ctx, const int* row, const int* col, const float* val, i64 m, int n_nodes,
+                        const int* nslot, const float* keep_above, i64 known_count) {
+    return symmetric_csr_impl(ctx, row, col, val, m, n_nodes, true,
+                              [=] __device__(i64 e) { return val[e] > keep_above[nslot[row[e]]]; }, known_count);
+}
+
 // compress_label_sequence (include/rama_utils.h:160-177); returns the number
 // of distinct labels.
 int compress_labels(Ctx& ctx, const int* data, i64 n, int max_label, int* out, bool want_count) {
@@ -445,6 +460,180 @@ void compress_matching(Ctx& ctx, const int* map, i64 n, int* out) {
     int* RK = rank.get();
     scan_exclusive<int>(ctx, [=] __device__(i64 v) { return (int)(map[v] == (int)v); }, RK, n);
     for_each(ctx, n, [=] __device__(i64 v) { out[v] = RK[map[v]]; });
+}
+
+// ------------------------------------------------------------------
+// Packed instances: per-instance statistics of a (row, col)-sorted edge
+// list whose node ids are global (instance q owns ids from nbase[q]).
+// Local ids are global - nbase[q].  Fields as the single-instance dCOO keeps
+// them: cols = max col + 1, rows = last row + 1, max_node = largest kept col.
+void slot_stats_clear(Ctx& ctx, SlotStats* st, int Q) {
+    for_each(ctx, Q, [=] __device__(i64 q) { st[q] = SlotStats{-1, -1, -1, 0, 0}; });
+}
+
+// rb[q] = first position of slot q in a sorted key array (q = 0..Q), the
+// slot of key k being the last q with base[q] <= k.
+template <class K, class Key>
+__global__ void k_slot_bounds(Key key, i64 n, const int* __restrict__ base, int Q, i64* __restrict__ rb) {
+    const int q = blockIdx.x * blockDim.x + threadIdx.x;
+    if (q > Q) return;
+    const K x = (K)base[q];
+    i64 lo = 0, hi = n;
+    while (lo < hi) {
+        const i64 mid = (lo + hi) >> 1;
+        if ((K)key(mid) < x) lo = mid + 1; else hi = mid;
+    }
+    rb[q] = lo;
+}
+
+struct MaxAcc {
+    int v = -1;
+};
+struct MaxRed {
+    __device__ MaxAcc operator()(MaxAcc a, MaxAcc b) const { return MaxAcc{a.v > b.v ? a.v : b.v}; }
+    __device__ MaxAcc shfl_down(MaxAcc a, int d) const { return MaxAcc{__shfl_down(a.v, d, 64)}; }
+};
+
+// Stats of a sorted edge list (every edge kept): ranges by binary search,
+// the largest col by a range reduction, the last row and count from the
+// ranges.
+void slot_stats_edges(Ctx& ctx, const int* row, const int* col, i64 n, const int* nbase, int Q, SlotStats* st) {
+    DBuf<i64> rbuf = ctx.buf<i64>(Q + 1);
+    i64* RB = rbuf.get();
+    auto rowkey = [=] __device__(i64 i) { return row[i]; };
+    k_slot_bounds<int><<<grid_for(Q + 1), kBlock, 0, ctx.stream>>>(rowkey, n, nbase, Q, RB);
+    RAMA_LAUNCHED();
+    slot_stats_clear(ctx, st, Q);
+    range_reduce<MaxAcc>(
+        ctx, RB, Q, [=] __device__(i64 i, MaxAcc& a) { a.v = col[i] > a.v ? col[i] : a.v; }, MaxRed(),
+        [=] __device__(int q, MaxAcc r) { atomicMax(&st[q].max_col, r.v - nbase[q]); });
+    for_each(ctx, Q, [=] __device__(i64 q) {
+        const i64 lo = RB[q], hi = RB[q + 1];
+        st[q].max_kept_col = st[q].max_col;
+        st[q].max_row = hi > lo ? row[hi - 1] - nbase[q] : -1;
+        st[q].kept = (int)(hi - lo);
+    });
+}
+
+struct ContractAcc {
+    int mc = -1;    // largest col, diagonal included
+    int mk = -1;    // largest col of a kept segment
+    int kept = 0;   // kept segments
+    int bad = 0;    // negative self-loop
+};
+struct ContractRed {
+    __device__ ContractAcc operator()(ContractAcc a, ContractAcc b) const {
+        return ContractAcc{a.mc > b.mc ? a.mc : b.mc, a.mk > b.mk ? a.mk : b.mk, a.kept + b.kept, a.bad | b.bad};
+    }
+    __device__ ContractAcc shfl_down(ContractAcc a, int d) const {
+        return ContractAcc{__shfl_down(a.mc, d, 64), __shfl_down(a.mk, d, 64), __shfl_down(a.kept, d, 64),
+                           __shfl_down(a.bad, d, 64)};
+    }
+};
+
+// contract for packed instances: node_map gives every node of a continuing
+// instance its new global id (new_nslot / new_nbase describe the new id
+// space).  B receives the contracted, diagonal-free union; st[q] the new
+// dims, edge count and bad-contraction flag of instance q.
+void contract_packed(Ctx& ctx, const DCoo& A, const int* node_map, int max_label, const int* new_nslot,
+                     const int* new_nbase, int Q, DCoo& B, SlotStats* st) {
+    const i64 n = A.nnz;
+    B = DCoo();
+    slot_stats_clear(ctx, st, Q);
+    if (n == 0) return;
+    const int kb = bits_for(max_label < 1 ? 1 : max_label);
+    DBuf<u64> keys = ctx.buf<u64>(n);
+    DBuf<u32> vals = ctx.buf<u32>(n);
+    u64* K = keys.get();
+    u32* V = vals.get();
+    const int* R = A.row.get();
+    const int* C = A.col.get();
+    const float* W = A.val.get();
+    DBuf<int> sc = ctx.buf<int>(8);
+    DBuf<double> dg = ctx.buf<double>(1);
+    int* SC = sc.get();
+    double* DG = dg.get();
+    for_each(ctx, n, [=] __device__(i64 e) {
+        const int a = node_map[R[e]], b = node_map[C[e]];
+        const u32 lo = (u32)(a < b ? a : b), hi = (u32)(a < b ? b : a);
+        K[e] = ((u64)lo << kb) | hi;
+        V[e] = f2u(W[e]);
+        if (e == 0) {
+            for (int k = 0; k < 8; ++k) SC[k] = 0;
+            *DG = 0.0;
+        }
+    });
+    {
+        prof::Site site_("contract");
+        radix_sort_pairs(ctx, keys, vals, n, 2 * kb);
+    }
+    K = keys.get();
+    V = vals.get();
+    DBuf<int> seg_start = ctx.buf<int>(n);
+    int* SS = seg_start.get();
+    select_into(
+        ctx, n, [=] __device__(i64 e) { return e == 0 || K[e] != K[e - 1]; },
+        [=] __device__(i64 e, int pos) { SS[pos] = (int)e; }, SC + 0);
+    DBuf<u64> okey = ctx.buf<u64>(n);
+    DBuf<float> oval = ctx.buf<float>(n);
+    DBuf<int> keep = ctx.buf<int>(n);
+    k_contract_reduce<<<grid_for(n), kBlock, 0, ctx.stream>>>(K, V, SS, SC + 0, n, kb, okey.get(), oval.get(),
+                                                                keep.get(), SC + 5, DG);
+    RAMA_LAUNCHED();
+    const u64 mask = ((u64)1 << kb) - 1;
+    const u64* OK = okey.get();
+    const float* OV = oval.get();
+    const int* KP = keep.get();
+    // per-instance dims over all segments (diagonal included), kept edges
+    // and bad contractions (a negative self-loop): segment ranges of each
+    // instance by binary search, then range reductions
+    {
+        DBuf<i64> rbuf = ctx.buf<i64>(Q + 1);
+        i64* RB = rbuf.get();
+        const int* NB = new_nbase;
+        auto rowkey = [=] __device__(i64 s) -> i64 { return s < SC[0] ? (i64)(OK[s] >> kb) : ((i64)1 << 40); };
+        k_slot_bounds<i64><<<grid_for(Q + 1), kBlock, 0, ctx.stream>>>(rowkey, n, NB, Q, RB);
+        RAMA_LAUNCHED();
+        range_reduce<ContractAcc>(
+            ctx, RB, Q,
+            [=] __device__(i64 s, ContractAcc& a) {
+                const int gc = (int)(OK[s] & mask);
+                const bool kp = KP[s] != 0;
+                a.mc = gc > a.mc ? gc : a.mc;
+                if (kp) {
+                    a.mk = gc > a.mk ? gc : a.mk;
+                    a.kept += 1;
+                } else if (OV[s] < 0.0f) {
+                    a.bad = 1;
+                }
+            },
+            ContractRed(),
+            [=] __device__(int q, ContractAcc r) {
+                atomicMax(&st[q].max_col, r.mc - NB[q]);
+                if (r.mk >= 0) atomicMax(&st[q].max_kept_col, r.mk - NB[q]);
+                atomicAdd(&st[q].kept, r.kept);
+                if (r.bad) st[q].bad = 1;
+            });
+        for_each(ctx, Q, [=] __device__(i64 q) {
+            const i64 lo = RB[q], hi = RB[q + 1];
+            st[q].max_row = hi > lo ? (int)(OK[hi - 1] >> kb) - NB[q] : -1;
+        });
+    }
+    B.row = ctx.buf<int>(n);
+    B.col = ctx.buf<int>(n);
+    B.val = ctx.buf<float>(n);
+    int* BR = B.row.get();
+    int* BC = B.col.get();
+    float* BW = B.val.get();
+    select_into(
+        ctx, n, [=] __device__(i64 s) { return s < SC[0] && KP[s] != 0; },
+        [=] __device__(i64 s, int q) {
+            BR[q] = (int)(OK[s] >> kb);
+            BC[q] = (int)(OK[s] & mask);
+            BW[q] = OV[s];
+        },
+        SC + 1);
+    B.nnz = -1;  // the caller sums the instances' kept counts
 }
 
 // map_node_labels (src/rama_cuda.cu:32-52)

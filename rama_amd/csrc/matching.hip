@@ -205,4 +205,172 @@ int matching_vertex_based(Ctx& ctx, const Csr& U, float mean_multiplier, DBuf<in
     return prev;
 }
 
+// ------------------------------------------------------------------
+// Packed instances (disjoint node ranges, nslot[v] = instance of node v):
+// each instance has its own MatchState, threshold and stop rule; a vertex
+// takes part in a round only while its instance's loop runs.  The matched
+// pairs of an instance are the pairs it would match alone.
+__global__ __launch_bounds__(kBlock) void k_pick_best_packed(int n, float mult, int round,
+                                                             const int* __restrict__ nslot,
+                                                             const MatchState* __restrict__ sts,
+                                                             const int* __restrict__ off,
+                                                             const int* __restrict__ adj,
+                                                             const float* __restrict__ w,
+                                                             const int* __restrict__ matched,
+                                                             int* __restrict__ best) {
+    const i64 t = (i64)xcd_block() * kBlock + threadIdx.x;
+    const int v = (int)(t / kGroup);
+    const int sub = (int)(t % kGroup);
+    float thr = 0.0f;
+    // all lanes of a group share v, so they agree on `active`
+    const bool active = v < n && match_round_active(sts + nslot[v], mult, round, &thr) && !matched[v];
+    const int b = active ? off[v] : 0, e = active ? off[v + 1] : 0;
+    float mx = thr;
+    int nid = -1;
+    for (int base = b; base < e; base += kGroup) {
+        const int l = base + sub;
+        bool ok = false;
+        float cv = 0.0f;
+        int cn = -1;
+        if (l < e) {
+            cv = w[l];
+            cn = adj[l];
+            ok = !(cv <= thr) && !matched[cn] && cn != v;
+        }
+        float bv = ok ? cv : -3.402823466e38f;
+        int bp = ok ? l : 0x7fffffff;
+        int bn = ok ? cn : -1;
+#pragma unroll
+        for (int d = kGroup / 2; d >= 1; d >>= 1) {
+            const float ov = __shfl_xor(bv, d, 64);
+            const int op = __shfl_xor(bp, d, 64);
+            const int on = __shfl_xor(bn, d, 64);
+            if (ov > bv || (ov == bv && op < bp)) {
+                bv = ov;
+                bp = op;
+                bn = on;
+            }
+        }
+        if (bp != 0x7fffffff && bv > mx) {
+            mx = bv;
+            nid = bn;
+        }
+    }
+    if (active && sub == 0) best[v] = nid;
+}
+
+__global__ __launch_bounds__(kBlock) void k_match_packed(int n, float mult, int round, const int* __restrict__ nslot,
+                                                         MatchState* sts, const int* __restrict__ best, int* matched,
+                                                         int* map) {
+    const int v = blockIdx.x * blockDim.x + threadIdx.x;
+    const int q = v < n ? nslot[v] : 0;
+    float thr;
+    int add = 0;
+    if (v < n && match_round_active(sts + q, mult, round, &thr) && !matched[v]) {
+        const int vb = best[v];
+        if (vb > v && !matched[vb] && best[vb] == v) {
+            map[vb] = v;
+            matched[v] = 1;
+            matched[vb] = 1;
+            add = 2;
+        }
+    }
+    // per-instance count: one atomic per distinct instance in the wave
+    const int lane = lane_id();
+    unsigned long long pending = __ballot(add != 0);
+    while (pending) {
+        const int leader = __builtin_ctzll(pending);
+        const int ql = __builtin_amdgcn_readlane(q, leader);
+        const unsigned long long m = __ballot(q == ql) & pending;
+        if (lane == leader) {
+            atomicAdd(&sts[ql].newly[round], 2 * __popcll(m));
+            sts[ql].running[round] = 1;
+        }
+        pending &= ~m;
+    }
+}
+
+// Matching of packed instances on their joint symmetric CSR U (n rows).
+// Instance q's CSR entries are [hbeg[q], hbeg[q+1]) (dbeg on the device).
+// Fills map (global ids) and matched_out[q] = matched vertices of q.
+void matching_packed(Ctx& ctx, const Csr& U, const int* nslot, int Q, const i64* hbeg, const i64* dbeg,
+                     float mean_multiplier, unsigned* tickets, DBuf<int>& map, int* matched_out) {
+    const int n = U.n;
+    map = ctx.buf<int>(n > 0 ? n : 1);
+    DBuf<MatchState> stb = ctx.buf<MatchState>(Q);
+    MatchState* st = stb.get();
+    DBuf<int> best = ctx.buf<int>(n > 0 ? n : 1), matched = ctx.buf<int>(n > 0 ? n : 1);
+    {
+        int* M = map.get();
+        int* B = best.get();
+        int* D = matched.get();
+        for_each(ctx, std::max<i64>(n, Q), [=] __device__(i64 v) {
+            if (v < n) {
+                M[v] = (int)v;
+                B[v] = -1;
+                D[v] = 0;
+            }
+            if (v < Q) {
+                st[v].pos_sum = 0.0;
+                st[v].pos_cnt = 0.0;
+                for (int r = 0; r < kMaxRounds; ++r) {
+                    st[v].newly[r] = 0;
+                    st[v].running[r] = 0;
+                }
+            }
+        });
+    }
+    const float* W = U.w.get();
+    constexpr int kStride = (int)(sizeof(MatchState) / sizeof(double));
+    static_assert(sizeof(MatchState) % sizeof(double) == 0, "MatchState layout");
+    seg_reduce<double>(ctx, [=] __device__(i64 e) { const float x = W[e]; return x >= 0.0f ? (double)x : 0.0; },
+                       hbeg, dbeg, Q, 0.0, OpSum(), &st->pos_sum, kStride, tickets);
+    seg_reduce<double>(ctx, [=] __device__(i64 e) { return W[e] >= 0.0f ? 1.0 : 0.0; }, hbeg, dbeg, Q, 0.0,
+                       OpSum(), &st->pos_cnt, kStride, tickets);
+    std::vector<MatchState> h((size_t)Q);
+    auto replay = [&](const MatchState& x, int rounds, int* count) {
+        // the stop rule of filter_edges_by_matching_vertex_based (:121-127)
+        int prev = 0;
+        bool stopped = true;
+        if (x.pos_cnt > 0.0 && mean_multiplier * (float)x.pos_sum / (float)(i64)x.pos_cnt >= 0.0f) {
+            stopped = false;
+            for (int t = 0; t < rounds; ++t) {
+                const int cur = prev + x.newly[t];
+                const float rel = (cur - prev) / (prev + 1.0f);
+                prev = cur;
+                if (!x.running[t] || rel < 0.1f) {
+                    stopped = true;
+                    break;
+                }
+            }
+        }
+        if (count) *count = prev;
+        return stopped;
+    };
+    bool have = false;
+    if (n > 0) {
+        const i64 threads = (i64)n * kGroup;
+        constexpr int kFirstRounds = 3;
+        for (int t = 0; t < kMaxRounds; ++t) {
+            if (t == kFirstRounds) {
+                ctx.d2h(h.data(), st, Q);
+                bool all = true;
+                for (int q = 0; q < Q && all; ++q) all = replay(h[q], kFirstRounds, nullptr);
+                if (all) {
+                    have = true;
+                    break;
+                }
+            }
+            k_pick_best_packed<<<grid_for(threads), kBlock, 0, ctx.stream>>>(
+                n, mean_multiplier, t, nslot, st, U.off.get(), U.adj.get(), U.w.get(), matched.get(), best.get());
+            RAMA_LAUNCHED();
+            k_match_packed<<<grid_for(n), kBlock, 0, ctx.stream>>>(n, mean_multiplier, t, nslot, st, best.get(),
+                                                                   matched.get(), map.get());
+            RAMA_LAUNCHED();
+        }
+    }
+    if (!have) ctx.d2h(h.data(), st, Q);
+    for (int q = 0; q < Q; ++q) replay(h[q], kMaxRounds, &matched_out[q]);
+}
+
 }  // namespace rama

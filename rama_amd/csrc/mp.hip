@@ -256,6 +256,84 @@ __device__ __forceinline__ float min_marginal(float x, float y, float z) {
 // cost / #triangles (edge_share); the edge kernel stores that share next to
 // the cost, so a triangle gathers one value per edge.
 
+// One triangle's update (send_messages_to_triplets + the six min-marginal
+// steps); returns its lower-bound term.
+template <bool kSweep>
+__device__ __forceinline__ double mp_triangle(i64 t, const int* __restrict__ c12, const int* __restrict__ c13,
+                                              const int* __restrict__ c23, const float* __restrict__ share,
+                                              float* __restrict__ t12, float* __restrict__ t13,
+                                              float* __restrict__ t23, float* __restrict__ delta) {
+    float a = t12[t], b = t13[t], d = t23[t];
+    a += share[c12[t]];
+    b += share[c13[t]];
+    d += share[c23[t]];
+    if (kSweep) {
+        // decrease_triangle_costs_func (:282-332): steps with weights
+        // 1/3, 1/2, 1, 1/2, 1, 1 over slots 12, 13, 23, 12, 13, 12.
+        float e12 = 0.0f, e13 = 0.0f, e23 = 0.0f;
+        {
+            const float mm = min_marginal(a, b, d);
+            a = (float)((double)a - 1.0 / 3.0 * (double)mm);
+            e12 = (float)((double)e12 + 1.0 / 3.0 * (double)mm);
+        }
+        {
+            const float mm = min_marginal(b, a, d);
+            b = (float)((double)b - 1.0 / 2.0 * (double)mm);
+            e13 = (float)((double)e13 + 1.0 / 2.0 * (double)mm);
+        }
+        {
+            const float mm = min_marginal(d, a, b);
+            d -= mm;
+            e23 += mm;
+        }
+        {
+            const float mm = min_marginal(a, b, d);
+            a = (float)((double)a - 1.0 / 2.0 * (double)mm);
+            e12 = (float)((double)e12 + 1.0 / 2.0 * (double)mm);
+        }
+        {
+            const float mm = min_marginal(b, a, d);
+            b -= mm;
+            e13 += mm;
+        }
+        {
+            const float mm = min_marginal(a, b, d);
+            a -= mm;
+            e12 += mm;
+        }
+        delta[3 * t + 0] = e12;
+        delta[3 * t + 1] = e13;
+        delta[3 * t + 2] = e23;
+    }
+    t12[t] = a;
+    t13[t] = b;
+    t23[t] = d;
+    return (double)tri_lb(a, b, d);
+}
+
+// One edge's update: with kSweep, cost = sum of the slot messages (edges in
+// no triangle keep their cost); without, edges in a triangle are zeroed
+// (decrease_edge_costs_func, :231-239).  Returns its lower-bound term.
+template <bool kSweep>
+__device__ __forceinline__ double mp_edge(i64 e, const int* __restrict__ soff, const u32* __restrict__ slots,
+                                          const float* __restrict__ delta, float* __restrict__ ec,
+                                          float* __restrict__ share) {
+    float c = ec[e];
+    const int s0 = soff[e], s1 = soff[e + 1];
+    if (s1 > s0) {
+        if (kSweep) {
+            double s = 0.0;
+            for (int k = s0; k < s1; ++k) s += (double)delta[slots[k]];
+            c = (float)s;
+        } else {
+            c = 0.0f;
+        }
+        ec[e] = c;
+        share[e] = edge_share(c, s1 - s0);
+    }
+    return c < 0.0f ? (double)c : 0.0;
+}
+
 template <bool kSweep>
 __global__ __launch_bounds__(kBlock) void k_mp_triangles(i64 T, const int* __restrict__ c12, const int* __restrict__ c13,
                                                          const int* __restrict__ c23, const float* __restrict__ share,
@@ -267,62 +345,12 @@ __global__ __launch_bounds__(kBlock) void k_mp_triangles(i64 T, const int* __res
     if (ctl && !ctl->active) return;  // uniform: the loop has stopped
     const unsigned blk = xcd_block();
     const i64 t = (i64)blk * kBlock + threadIdx.x;
-    double lb = 0.0;
-    if (t < T) {
-        float a = t12[t], b = t13[t], d = t23[t];
-        a += share[c12[t]];
-        b += share[c13[t]];
-        d += share[c23[t]];
-        if (kSweep) {
-            // decrease_triangle_costs_func (:282-332): steps with weights
-            // 1/3, 1/2, 1, 1/2, 1, 1 over slots 12, 13, 23, 12, 13, 12.
-            float e12 = 0.0f, e13 = 0.0f, e23 = 0.0f;
-            {
-                const float mm = min_marginal(a, b, d);
-                a = (float)((double)a - 1.0 / 3.0 * (double)mm);
-                e12 = (float)((double)e12 + 1.0 / 3.0 * (double)mm);
-            }
-            {
-                const float mm = min_marginal(b, a, d);
-                b = (float)((double)b - 1.0 / 2.0 * (double)mm);
-                e13 = (float)((double)e13 + 1.0 / 2.0 * (double)mm);
-            }
-            {
-                const float mm = min_marginal(d, a, b);
-                d -= mm;
-                e23 += mm;
-            }
-            {
-                const float mm = min_marginal(a, b, d);
-                a = (float)((double)a - 1.0 / 2.0 * (double)mm);
-                e12 = (float)((double)e12 + 1.0 / 2.0 * (double)mm);
-            }
-            {
-                const float mm = min_marginal(b, a, d);
-                b -= mm;
-                e13 += mm;
-            }
-            {
-                const float mm = min_marginal(a, b, d);
-                a -= mm;
-                e12 += mm;
-            }
-            delta[3 * t + 0] = e12;
-            delta[3 * t + 1] = e13;
-            delta[3 * t + 2] = e23;
-        }
-        t12[t] = a;
-        t13[t] = b;
-        t23[t] = d;
-        lb = (double)tri_lb(a, b, d);
-    }
+    const double lb = t < T ? mp_triangle<kSweep>(t, c12, c13, c23, share, t12, t13, t23, delta) : 0.0;
     const double r = block_reduce(lb, OpSum(), ws);
     if (threadIdx.x == 0) lbpart[blk] = r;
 }
 
-// Edge side: with kSweep, cost = sum of the slot messages (edges in no
-// triangle keep their cost); without, edges in a triangle are zeroed
-// (decrease_edge_costs_func, :231-239).  Adds the edge lower-bound term.
+// Edge side (mp_edge per edge), adding the edge lower-bound terms per block.
 //
 // With `parts` set (the device-driven loop on small grids), the last block to
 // finish (atomic ticket, reset afterwards) also sums all nparts lower-bound
@@ -341,23 +369,7 @@ __global__ __launch_bounds__(kBlock) void k_mp_edges(i64 E, const int* __restric
     if (ctl && !ctl->active) return;
     const unsigned blk = xcd_block();
     const i64 e = (i64)blk * kBlock + threadIdx.x;
-    double lb = 0.0;
-    if (e < E) {
-        float c = ec[e];
-        const int s0 = soff[e], s1 = soff[e + 1];
-        if (s1 > s0) {
-            if (kSweep) {
-                double s = 0.0;
-                for (int k = s0; k < s1; ++k) s += (double)delta[slots[k]];
-                c = (float)s;
-            } else {
-                c = 0.0f;
-            }
-            ec[e] = c;
-            share[e] = edge_share(c, s1 - s0);
-        }
-        lb = c < 0.0f ? (double)c : 0.0;
-    }
+    const double lb = e < E ? mp_edge<kSweep>(e, soff, slots, delta, ec, share) : 0.0;
     const double r = block_reduce(lb, OpSum(), ws);
     if (threadIdx.x == 0) lbpart[blk] = r;
     if (!parts) return;
@@ -429,6 +441,120 @@ static void launch_round(Ctx& ctx, MPState& S, bool sweep, MPControl* ctl = null
         k_mp_finalize<<<1, kBlock, 0, ctx.stream>>>(P, nparts, ctl);
         RAMA_LAUNCHED();
     }
+}
+
+// ------------------------------------------------------------------
+// Several instances in one message-passing state (packed batch).  Instance
+// q owns triangles [tbeg[q], tbeg[q+1]) and edges [ebeg[q], ebeg[q+1]); its
+// blocks are [tblk[q], tblk[q+1]) of the triangle launch and [eblk[q],
+// eblk[q+1]) of the edge launch, each covering 256 consecutive elements from
+// the instance's start exactly as its own launches would.  Its lower-bound
+// partials sit at parts + pbeg[q] (triangle blocks, then edge blocks) and
+// are summed in the same order as k_mp_finalize sums them, so every
+// instance's bound and stall test equal those of a solve on its own.
+template <bool kSweep>
+__global__ __launch_bounds__(kBlock) void k_mp_triangles_packed(MPPack pk, const int* __restrict__ c12,
+                                                                const int* __restrict__ c13,
+                                                                const int* __restrict__ c23,
+                                                                const float* __restrict__ share,
+                                                                float* __restrict__ t12, float* __restrict__ t13,
+                                                                float* __restrict__ t23, float* __restrict__ delta,
+                                                                double* __restrict__ parts,
+                                                                const MPControl* __restrict__ ctl) {
+    __shared__ double ws[kWaves];
+    const int q = block_owner(pk.tblk, pk.Q, (int)blockIdx.x);
+    if (!ctl[q].active) return;  // uniform per block
+    const int lb = (int)blockIdx.x - pk.tblk[q];
+    const i64 t = pk.tbeg[q] + (i64)lb * kBlock + threadIdx.x;
+    const double v = t < pk.tbeg[q + 1] ? mp_triangle<kSweep>(t, c12, c13, c23, share, t12, t13, t23, delta) : 0.0;
+    const double r = block_reduce(v, OpSum(), ws);
+    if (threadIdx.x == 0) parts[pk.pbeg[q] + lb] = r;
+}
+
+template <bool kSweep>
+__global__ __launch_bounds__(kBlock) void k_mp_edges_packed(MPPack pk, const int* __restrict__ soff,
+                                                            const u32* __restrict__ slots,
+                                                            const float* __restrict__ delta, float* __restrict__ ec,
+                                                            float* __restrict__ share, double* __restrict__ parts,
+                                                            const MPControl* __restrict__ ctl) {
+    __shared__ double ws[kWaves];
+    const int q = block_owner(pk.eblk, pk.Q, (int)blockIdx.x);
+    if (!ctl[q].active) return;
+    const int lb = (int)blockIdx.x - pk.eblk[q];
+    const i64 e = pk.ebeg[q] + (i64)lb * kBlock + threadIdx.x;
+    const double v = e < pk.ebeg[q + 1] ? mp_edge<kSweep>(e, soff, slots, delta, ec, share) : 0.0;
+    const double r = block_reduce(v, OpSum(), ws);
+    if (threadIdx.x == 0) parts[pk.pbeg[q] + (pk.tblk[q + 1] - pk.tblk[q]) + lb] = r;
+}
+
+// One block per instance: k_mp_finalize on the instance's partials.
+__global__ __launch_bounds__(kBlock) void k_mp_finalize_packed(MPPack pk, const double* __restrict__ parts,
+                                                               MPControl* ctl) {
+    __shared__ double ws[kWaves];
+    const int q = blockIdx.x;
+    MPControl* c = ctl + q;
+    if (!c->active) return;
+    const i64 n = (i64)(pk.tblk[q + 1] - pk.tblk[q]) + (pk.eblk[q + 1] - pk.eblk[q]);
+    const double* p = parts + pk.pbeg[q];
+    double s = 0.0;
+    for (i64 k = threadIdx.x; k < n; k += kBlock) s += p[k];
+    const double lb = block_reduce(s, OpSum(), ws);
+    if (threadIdx.x == 0) {
+        c->prev = c->lb;
+        c->lb = lb;
+        c->iterations += 1;
+        if (lb - c->prev < 1e-3) c->active = 0;
+    }
+}
+
+// Per instance: initial bound = edge part + triangle part (as mp_run's
+// first two reductions), loop state cleared; `on` marks the instances whose
+// loop runs.
+__global__ void k_mp_init_packed(int Q, const double* __restrict__ p_edges, const double* __restrict__ p_tris,
+                                 const int* __restrict__ on, MPControl* ctl) {
+    const int q = blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= Q) return;
+    ctl[q].lb = p_edges[q] + p_tris[q];
+    ctl[q].prev = 0.0;
+    ctl[q].active = on[q];
+    ctl[q].iterations = 0;
+    ctl[q].max_col = -1;
+    ctl[q].last_row = -1;
+}
+
+void mp_run_packed(Ctx& ctx, MPState& S, const MPPack& pk, int total_tblocks, int total_eblocks, int steps,
+                   const i64* seg_tbeg_host, const i64* seg_ebeg_host, const int* d_on, unsigned* tickets,
+                   double* lb_out) {
+    const int Q = pk.Q;
+    // initial bound per instance, reduced as reduce_to reduces each part alone
+    DBuf<double> pe = ctx.buf<double>(2 * (i64)Q);
+    const float* EC = S.ec.get();
+    const float* A = S.t12.get();
+    const float* B = S.t13.get();
+    const float* D = S.t23.get();
+    seg_reduce<double>(ctx, [=] __device__(i64 e) { const float c = EC[e]; return c < 0.0f ? (double)c : 0.0; },
+                       seg_ebeg_host, pk.ebeg, Q, 0.0, OpSum(), pe.get(), 1, tickets);
+    seg_reduce<double>(ctx, [=] __device__(i64 t) { return (double)tri_lb(A[t], B[t], D[t]); }, seg_tbeg_host,
+                       pk.tbeg, Q, 0.0, OpSum(), pe.get() + Q, 1, tickets);
+    DBuf<MPControl> ctl = ctx.buf<MPControl>(Q);
+    k_mp_init_packed<<<grid_for(Q), kBlock, 0, ctx.stream>>>(Q, pe.get(), pe.get() + Q, d_on, ctl.get());
+    RAMA_LAUNCHED();
+    const i64 nparts = (i64)total_tblocks + total_eblocks;
+    DBuf<double> parts = ctx.buf<double>(nparts);
+    for (int it = 0; it < steps; ++it) {
+        k_mp_triangles_packed<true><<<(unsigned)total_tblocks, kBlock, 0, ctx.stream>>>(
+            pk, S.c12.get(), S.c13.get(), S.c23.get(), S.share.get(), S.t12.get(), S.t13.get(), S.t23.get(),
+            S.delta.get(), parts.get(), ctl.get());
+        RAMA_LAUNCHED();
+        k_mp_edges_packed<true><<<(unsigned)total_eblocks, kBlock, 0, ctx.stream>>>(
+            pk, S.soff.get(), S.slots.get(), S.delta.get(), S.ec.get(), S.share.get(), parts.get(), ctl.get());
+        RAMA_LAUNCHED();
+        k_mp_finalize_packed<<<(unsigned)Q, kBlock, 0, ctx.stream>>>(pk, parts.get(), ctl.get());
+        RAMA_LAUNCHED();
+    }
+    std::vector<MPControl> h((size_t)Q);
+    ctx.d2h(h.data(), ctl.get(), Q);
+    for (int q = 0; q < Q; ++q) lb_out[q] = h[q].lb;
 }
 
 // lower_bound() (:201-216) of the current state.

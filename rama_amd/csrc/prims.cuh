@@ -339,7 +339,114 @@ T reduce(Ctx& ctx, F f, i64 n, T ident, Op op) {
     return ctx.read1(r.get());
 }
 
-// ------------------------------------------------------- simple kernels
+// ------------------------------------------------ segmented reduction
+// Segment q of [beg[q], beg[q+1]) reduced exactly as reduce_to reduces an
+// array of that length on its own: the same number of blocks
+// (seg_blocks(n)), the same grid-stride element order, the same block tree
+// and the same combination of partials in block order by the segment's last
+// block.  So a result is bit-identical whether its instance is reduced alone
+// or packed with others.  bpre (Q+1) is the prefix of the segments' block
+// counts; tickets (Q) start at zero and are reset by each segment's last
+// block.
+inline unsigned seg_blocks(i64 n) {
+    return (unsigned)std::min<i64>(std::max<i64>((n + kBlock - 1) / kBlock, 1), kReduceMaxBlocks);
+}
+
+// Owner of block b given the prefix pre[0..Q] of per-segment block counts
+// (every segment has at least one block).
+__device__ __forceinline__ int block_owner(const int* pre, int Q, int b) {
+    int lo = 0, hi = Q - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (pre[mid] <= b) lo = mid; else hi = mid - 1;
+    }
+    return lo;
+}
+
+template <class T, class Op, class F>
+__global__ __launch_bounds__(kBlock) void k_seg_reduce(F f, const i64* __restrict__ beg, const int* __restrict__ bpre,
+                                                       int Q, T ident, Op op, T* partials, unsigned* tickets, T* dst,
+                                                       int stride) {
+    __shared__ T ws[kWaves];
+    __shared__ bool last;
+    const int q = block_owner(bpre, Q, (int)blockIdx.x);
+    const int nb = bpre[q + 1] - bpre[q];
+    const int lb = (int)blockIdx.x - bpre[q];
+    const i64 b0 = beg[q], n = beg[q + 1] - b0;
+    T acc = ident;
+    for (i64 i = (i64)lb * kBlock + threadIdx.x; i < n; i += (i64)nb * kBlock) acc = op(acc, (T)f(b0 + i));
+    const T r = block_reduce(acc, op, ws);
+    if (threadIdx.x == 0) {
+        __hip_atomic_store(&partials[bpre[q] + lb], r, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        const unsigned t = __hip_atomic_fetch_add(&tickets[q], 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
+        last = t == (unsigned)nb - 1;
+    }
+    __syncthreads();
+    if (!last) return;
+    __threadfence();  // acquire: the segment's partials are visible to plain loads
+    T a = ident;
+    for (int k = threadIdx.x; k < nb; k += kBlock) a = op(a, partials[bpre[q] + k]);
+    const T total = block_reduce(a, op, ws);
+    if (threadIdx.x == 0) {
+        dst[(i64)q * stride] = total;
+        tickets[q] = 0;
+    }
+}
+
+// Host side of the segmented reduction: hbeg/dbeg are the segment bounds
+// (Q+1) on the host and on the device; dst[q * stride] receives segment q.
+template <class T, class Op, class F>
+void seg_reduce(Ctx& ctx, F f, const i64* hbeg, const i64* dbeg, int Q, T ident, Op op, T* dst, int stride,
+                unsigned* tickets) {
+    if (Q <= 0) return;
+    static_assert(sizeof(T) <= 8, "reduction type");
+    std::vector<int> bpre((size_t)Q + 1);
+    bpre[0] = 0;
+    for (int q = 0; q < Q; ++q) bpre[q + 1] = bpre[q] + (int)seg_blocks(hbeg[q + 1] - hbeg[q]);
+    DBuf<int> dpre = ctx.buf<int>(Q + 1);
+    ctx.upload(dpre.get(), bpre.data(), Q + 1);
+    DBuf<T> part = ctx.buf<T>(bpre[Q]);
+    k_seg_reduce<T, Op, F><<<(unsigned)bpre[Q], kBlock, 0, ctx.stream>>>(f, dbeg, dpre.get(), Q, ident, op,
+                                                                          part.get(), tickets, dst, stride);
+    RAMA_LAUNCHED();
+}
+
+// Per-segment integer reductions over contiguous ranges known on the device:
+// segment q = [rb[q], rb[q+1]) is covered by kSegSub blocks; each block
+// reduces its stride of the range and adds its result to the segment with
+// one atomic, so a few atomics per segment replace one per element.
+// Integer min/max/sum only (order does not matter).  f(i, acc) folds element
+// i into the block-local accumulator; fin(q, acc) publishes it.
+constexpr int kSegSub = 8;
+template <class Acc, class F, class Red, class Fin>
+__global__ __launch_bounds__(kBlock) void k_range_reduce(const i64* __restrict__ rb, int Q, F f, Red red, Fin fin) {
+    __shared__ __align__(16) unsigned char ws_raw[kWaves * sizeof(Acc)];  // Acc may have initialisers
+    Acc* ws = reinterpret_cast<Acc*>(ws_raw);
+    const int q = blockIdx.x / kSegSub, sub = blockIdx.x % kSegSub;
+    if (q >= Q) return;
+    const i64 lo = rb[q], hi = rb[q + 1];
+    Acc acc = Acc();
+    for (i64 i = lo + (i64)sub * kBlock + threadIdx.x; i < hi; i += (i64)kSegSub * kBlock) f(i, acc);
+    // block combine through LDS (Acc is a small POD with a combine op)
+    const int lane = lane_id(), w = threadIdx.x >> 6;
+#pragma unroll
+    for (int d = 32; d >= 1; d >>= 1) acc = red(acc, red.shfl_down(acc, d));
+    if (lane == 0) ws[w] = acc;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        Acc r = ws[0];
+        for (int k = 1; k < kWaves; ++k) r = red(r, ws[k]);
+        if (hi > lo) fin(q, r);
+    }
+}
+template <class Acc, class F, class Red, class Fin>
+void range_reduce(Ctx& ctx, const i64* rb, int Q, F f, Red red, Fin fin) {
+    if (Q <= 0) return;
+    k_range_reduce<Acc, F, Red, Fin><<<(unsigned)Q * kSegSub, kBlock, 0, ctx.stream>>>(rb, Q, f, red, fin);
+    RAMA_LAUNCHED();
+}
+
+// ------------------------------------------------------- simple kernels// ------------------------------------------------------- simple kernels
 template <class T>
 __global__ void k_fill(T* p, i64 n, T v) {
     const i64 i = (i64)blockIdx.x * blockDim.x + threadIdx.x;
@@ -374,6 +481,19 @@ template <class F>
 void for_each(Ctx& ctx, i64 n, F f) {
     if (n <= 0) return;
     k_for_each<F><<<grid_for(n), kBlock, 0, ctx.stream>>>(f, n);
+    RAMA_LAUNCHED();
+}
+
+// f(i) for every thread of a grid covering [0, n) — also for i >= n in the
+// last block, so wave-wide operations inside f see whole waves; f checks
+// the bound itself.
+template <class F>
+__global__ void k_for_all(F f, i64 n) {
+    f((i64)blockIdx.x * blockDim.x + threadIdx.x);
+}
+template <class F>
+void for_all(Ctx& ctx, i64 n, F f) {
+    k_for_all<F><<<grid_for(n), kBlock, 0, ctx.stream>>>(f, n);
     RAMA_LAUNCHED();
 }
 

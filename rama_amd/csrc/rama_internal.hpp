@@ -3,6 +3,8 @@
 // context's device; host code only sees sizes and a handful of scalars.
 #pragma once
 
+#include <chrono>
+
 #include "prims.cuh"
 
 namespace rama {
@@ -58,6 +60,8 @@ DCoo contract(Ctx& ctx, const DCoo& A, const int* node_map, int max_label, bool*
 // `known_count`: number of edges passing the filter if the caller knows it (-1: count here).
 Csr symmetric_csr(Ctx& ctx, const int* row, const int* col, const float* val, i64 m, int n_nodes,
                   bool filter, float keep_above, i64 known_count = -1);
+Csr symmetric_csr_slots(Ctx& ctx, const int* row, const int* col, const float* val, i64 m, int n_nodes,
+                        const int* nslot, const float* keep_above, i64 known_count);
 void compute_offsets(Ctx& ctx, const int* sorted_ids, i64 n, int max_value, int* out);
 // Returns the number of distinct labels if want_count (one read-back), else -1.
 int compress_labels(Ctx& ctx, const int* data, i64 n, int max_label, int* out, bool want_count = true);
@@ -73,10 +77,25 @@ struct Sanitized {
 Sanitized sanitize_graph(Ctx& ctx, DBuf<int>& i, DBuf<int>& j, DBuf<float>& c, i64& m);
 void desanitize_labels(Ctx& ctx, const Sanitized& s, const int* labels, int* out);
 
+// Per-instance statistics of packed instances (see dcoo.hip).
+struct SlotStats {
+    int max_col;       // largest local col (diagonal included), -1 if none
+    int max_row;       // largest local row (= last row), -1 if none
+    int max_kept_col;  // largest local col of a kept edge, -1 if none
+    int kept;          // kept edges
+    int bad;           // a negative self-loop was contracted
+};
+void slot_stats_clear(Ctx& ctx, SlotStats* st, int Q);
+void slot_stats_edges(Ctx& ctx, const int* row, const int* col, i64 n, const int* nbase, int Q, SlotStats* st);
+void contract_packed(Ctx& ctx, const DCoo& A, const int* node_map, int max_label, const int* new_nslot,
+                     const int* new_nbase, int Q, DCoo& B, SlotStats* st);
+
 // ---- matching.hip
 // export_undirected + vertex matching; returns the (uncompressed) mapping of
 // size U.n and the number of matched vertices.
 int matching_vertex_based(Ctx& ctx, const Csr& U, float mean_multiplier, DBuf<int>& map);
+void matching_packed(Ctx& ctx, const Csr& U, const int* nslot, int Q, const i64* hbeg, const i64* dbeg,
+                     float mean_multiplier, unsigned* tickets, DBuf<int>& map, int* matched_out);
 
 // ---- cycles.hip
 struct Triangles {
@@ -86,7 +105,14 @@ struct Triangles {
     int max_id = -1;  // known bound on the vertex ids (-1: unknown, scan them)
 };
 Triangles conflicted_cycles(Ctx& ctx, const DCoo& A, int max_cycle_length, float tri_memory_factor, float tol_ratio);
+template <int S>
+void run_stage_packed(Ctx& ctx, const Csr& G, const int* rv1, const int* rv2, i64 R, const int* nslot, int Q,
+                      const i64* d_base, const i64* d_budget, const i64* d_out, int* t1, int* t2, int* t3,
+                      i64* h_total);
 void normalize_triangles(Ctx& ctx, Triangles& T);
+// On the first `valid` slots: drop triangles with a repeated vertex, sort,
+// deduplicate (in place); returns the count.  Ids are below n_nodes.
+i64 rearrange_triangles(Ctx& ctx, Triangles& T, i64 valid, int n_nodes);
 
 // ---- mp.hip
 struct MessagePassing;  // opaque
@@ -106,6 +132,21 @@ struct MPState {
     i64 E = 0, T = 0;
 };
 void mp_build(Ctx& ctx, const DCoo& A, Triangles& tri, MPState& S);
+// Packed instances in one message-passing state (see mp_run_packed).
+struct MPPack {
+    int Q;
+    const i64* tbeg;  // Q+1, device
+    const i64* ebeg;  // Q+1, device
+    const int* tblk;  // Q+1, device: prefix of per-instance triangle blocks
+    const int* eblk;  // Q+1, device: prefix of per-instance edge blocks
+    const i64* pbeg;  // Q, device: start of each instance's partials
+};
+// Runs `steps` iterations for every instance with d_on[q] != 0; lb_out[q]
+// (host) receives each instance's final bound.  seg_*beg_host are the host
+// copies of tbeg/ebeg; tickets (Q, zeroed) serve the segmented reductions.
+void mp_run_packed(Ctx& ctx, MPState& S, const MPPack& pk, int total_tblocks, int total_eblocks, int steps,
+                   const i64* seg_tbeg_host, const i64* seg_ebeg_host, const int* d_on, unsigned* tickets,
+                   double* lb_out);
 double mp_lower_bound(Ctx& ctx, MPState& S);
 double mp_fused_lower_bound(Ctx& ctx, MPState& S);
 void mp_send_to_triplets(Ctx& ctx, MPState& S);
@@ -125,8 +166,29 @@ i64 max_spanning_forest(Ctx& ctx, const int* r, const int* c, const float* w, i6
 // edge_contractions_woc(A).find_contraction_mapping(); returns #forest entries.
 int woc_mapping(Ctx& ctx, const DCoo& A, DBuf<int>& map, i64& map_size);
 
+// ---- lockstep.hip
+// A batch of instances packed back to back (instance q owns input edges
+// [edge_off[q], edge_off[q+1]) with its own 0-based ids) and its outputs.
+struct PackedBatch {
+    int k;
+    const int64_t* edge_off;
+    const int32_t* i;
+    const int32_t* j;
+    const float* c;
+    Options o;
+    int32_t* labels_out;
+    const int64_t* label_off;
+    int64_t* n_labels;
+    double* lbs;
+    int32_t* status;
+    std::string* first_error;
+    std::mutex* emu;
+};
+void solve_packed(Ctx& ctx, PackedBatch& B, int workers);
+
 // ---- solver.hip
 struct SolveResult {
+    std::chrono::steady_clock::time_point t0 = std::chrono::steady_clock::now();
     DBuf<int> labels;
     i64 num_labels = 0;
     double lb = 0.0;
@@ -134,5 +196,6 @@ struct SolveResult {
     std::vector<std::vector<int>> timeline;
 };
 SolveResult solve(Ctx& ctx, DCoo& A, const Options& o);
+void solve_rounds(Ctx& ctx, DCoo& A, SolveResult& out, const Options& o, int start_iter, bool use_mm);
 
 }  // namespace rama

@@ -56,8 +56,8 @@ Options to_options(const rama_options_c& c) {
 
 SolveResult solve(Ctx& ctx, DCoo& A, const Options& o) {
     SolveResult out;
+    out.t0 = std::chrono::steady_clock::now();
     StageTimes st;
-    const auto t0 = std::chrono::steady_clock::now();
     st.begin(ctx);
     out.lb = dual_solver(ctx, A, o.max_cycle_length_lb, o.num_dual_itr_lb, o.tri_memory_factor, o.num_outer_itr_dual,
                          1e-4f);
@@ -71,8 +71,18 @@ SolveResult solve(Ctx& ctx, DCoo& A, const Options& o) {
         scan_check(ctx);
         return out;
     }
-    bool use_mm = !(o.matching_thresh_crossover_ratio > 1.0f);
-    for (int iter = 0; A.nnz > 0; ++iter) {
+    solve_rounds(ctx, A, out, o, 0, !(o.matching_thresh_crossover_ratio > 1.0f));
+    return out;
+}
+
+// The contraction rounds from round `start_iter` on (src/rama_cuda.cu:88-158):
+// out.labels (out.num_labels entries) holds the labels composed so far and
+// use_mm whether matching still picks the contractions.
+void solve_rounds(Ctx& ctx, DCoo& A, SolveResult& out, const Options& o, int start_iter, bool use_mm) {
+    StageTimes st;
+    const auto t0 = out.t0;  // the time budget counts from the start of the solve
+    const i64 N = out.num_labels;
+    for (int iter = start_iter; A.nnz > 0; ++iter) {
         st.begin(ctx);
         if (iter > 0)
             dual_solver(ctx, A, o.max_cycle_length_primal, o.num_dual_itr_primal, 1.0f, 1, 1e-4f);
@@ -134,7 +144,6 @@ SolveResult solve(Ctx& ctx, DCoo& A, const Options& o) {
     }
     scan_check(ctx);
     if (st.on) std::fprintf(stderr, "solve: %d rounds, %lld host synchronisations\n", out.iterations, (long long)ctx.syncs);
-    return out;
 }
 
 }  // namespace rama

@@ -56,3 +56,59 @@ def test_server_reports_errors(gpu):
         t = srv.submit([0, -5], [1, 2], [1.0, -1.0])
         with pytest.raises(gpu.RamaError):
             srv.wait(t)
+
+
+# ---- the lockstep (packed) batch path: every instance of the batch shares one
+# graph and each round's launches; results must still equal solving alone.
+
+def _alone(gpu, insts, opts):
+    return [gpu.solve(i, j, c, opts)[:2] for (i, j, c) in insts]
+
+
+def _assert_same(alone, mixed):
+    for q, ((la, lba), (lm, lbm)) in enumerate(zip(alone, mixed)):
+        assert lba == lbm, (q, lba, lbm)
+        assert np.array_equal(la, lm), q
+
+
+@pytest.mark.parametrize("preset", ["P", "PD", "PD+", "D"])
+def test_packed_modes_equal_alone(gpu, preset):
+    from tests.instances_small import named
+    insts = mixed_instances(10, seed=21) + [named("grid64"), named("er800_dense")]
+    opts = gpu.options(preset, verbose=0)
+    _assert_same(_alone(gpu, insts, opts), gpu.solve_batch(insts, opts, workers=4))
+
+
+def test_packed_duplicates_gaps_and_tiny(gpu):
+    from tests.instances_small import named, random_graph
+    rng = np.random.default_rng(5)
+    gap_i, gap_j, gap_c = random_graph(400, 1500, seed=13)
+    # unused ids below the largest one, and a single-edge instance
+    insts = [named("random_dup"), (gap_i * 3, gap_j * 3, gap_c), (np.array([0]), np.array([5]), np.array([-1.0], np.float32)),
+             (np.array([0, 1, 0]), np.array([1, 2, 2]), np.array([1.5, 2.0, -3.0], np.float32))]
+    insts += [random_graph(int(n), int(3 * n), seed=int(s)) for n, s in zip(rng.integers(50, 300, 6), range(6))]
+    opts = gpu.options("PD", verbose=0)
+    _assert_same(_alone(gpu, insts, opts), gpu.solve_batch(insts, opts, workers=3))
+
+
+def test_packed_equals_per_instance_path(gpu, monkeypatch):
+    insts = mixed_instances(16, seed=31)
+    opts = gpu.options("PD", verbose=0)
+    packed = gpu.solve_batch(insts, opts, workers=4)
+    monkeypatch.setenv("RAMA_BATCH_MODE", "workers")
+    per_instance = gpu.solve_batch(insts, opts, workers=4)
+    _assert_same(per_instance, packed)
+
+
+def test_packed_forest_from_start(gpu):
+    # crossover ratio above 1: every instance uses the spanning forest from
+    # round 0, handed to the workers right after the packed initial dual
+    insts = mixed_instances(8, seed=41)
+    opts = gpu.options("PD", verbose=0, matching_thresh_crossover_ratio=1.5)
+    _assert_same(_alone(gpu, insts, opts), gpu.solve_batch(insts, opts, workers=2))
+
+
+def test_packed_reports_bad_instance(gpu):
+    insts = mixed_instances(3, seed=51) + [([0, -5], [1, 2], [1.0, -1.0])]
+    with pytest.raises(gpu.RamaError):
+        gpu.solve_batch(insts, gpu.options("PD", verbose=0), workers=2)
