@@ -253,7 +253,13 @@ def main():
     lab = labels[:n_lab].cpu().numpy()
     obj = _capi.objective(lab, i, j, c)
     roof = profiler.roofline(stats, HBM_PEAK_GBS, name=dom)
-    roof["measured_copy_peak"] = copy_peak(torch)
+    if roof is not None:
+        # HBM bytes per launch of the same kernel from the rocprofv3
+        # FETCH_SIZE / WRITE_SIZE runs summarised under profiles/
+        table = profiler.traffic_table(os.path.join(ROOT, "profiles", "traffic_pd_grid2d_1024x2048.md"))
+        if a.H == 1024 and a.W == 2048 and a.preset == "PD" and roof["kernel"] in table:
+            roof["traffic"] = table[roof["kernel"]]
+        roof["measured_copy_peak"] = copy_peak(torch)
     out = {
         "metric": "PD edges/s",
         "value": E / dt,

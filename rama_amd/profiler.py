@@ -59,3 +59,18 @@ def roofline(stats, peak_gbs, name=None):
     return {"kernel": d["name"], "bound": "hbm", "achieved": round(achieved, 1), "peak": peak_gbs, "unit": "GB/s",
             "frac": round(achieved / peak_gbs, 4), "traffic": None, "launches": d["launches"],
             "avg_launch_us": round(per_launch_s * 1e6, 2), "algorithmic_bytes_per_launch": per_launch_bytes}
+
+
+def traffic_table(path):
+    """Per-kernel HBM traffic (bytes per dispatch) from a committed rocprofv3
+    counter summary: the markdown table whose rows read
+    | profiler name | kernel | dispatches | fetch | write | traffic |."""
+    import os
+    out = {}
+    if not os.path.exists(path):
+        return out
+    for line in open(path):
+        cells = [c.strip() for c in line.strip().strip("|").split("|")]
+        if len(cells) == 6 and cells[2].isdigit():
+            out[cells[0]] = float(cells[5])
+    return out
