@@ -35,6 +35,7 @@ struct Slot {
     i64 nnz;
     int n0;          // number of labels
     double lb;       // lower bound of the initial dual
+    int iter;        // the instance's next round (0 = first)
 };
 
 struct Pack {
@@ -570,606 +571,674 @@ struct Handoff {
 
 // ------------------------------------------------------------- driver
 
-static void set_status(PackedBatch& B, int id, int st, const std::string& msg) {
-    if (B.status) B.status[id] = st;
-    if (st) {
-        std::lock_guard<std::mutex> l(*B.emu);
-        if (B.first_error->empty()) *B.first_error = msg;
+// Receives each instance's outcome (from the lockstep thread or a worker).
+struct LockstepSink {
+    virtual ~LockstepSink() = default;
+    virtual void done(int id, int status, const std::string& error, const int32_t* labels, i64 n, double lb) = 0;
+};
+
+// The lockstep solver: instances are admitted in groups (each group runs
+// its initial dual on arrival), and every call of round() advances all
+// admitted instances by one contraction round.  Instances leave when they
+// finish; those that switch to the forest contraction are continued by
+// `workers` threads with the single-instance rounds.
+class Lockstep {
+   public:
+    Lockstep(Ctx& ctx, const Options& o, int workers, LockstepSink& sink) : ctx_(ctx), o_(o), sink_(sink) {
+        tickets_ = ctx_.buf<unsigned>(kMaxSlots);
+        ctx_.zero(tickets_.get(), kMaxSlots);
+        times_ = std::getenv("RAMA_STAGE_TIMES") != nullptr;
+        t0_ = tlast_ = std::chrono::steady_clock::now();
+        for (int w = 0; w < std::max(1, workers); ++w) pool_.emplace_back([this] { worker(); });
+    }
+    ~Lockstep() { finish(); }
+
+    static constexpr int kMaxSlots = 4096;
+    int slots() const { return P_.Q(); }
+    bool empty() const { return P_.Q() == 0; }
+
+    // Admits k instances (edges [edge_off[q], edge_off[q+1]) of i, j, c, own
+    // 0-based ids) under the external ids ids[q].
+    void admit(int k, const int64_t* edge_off, const int32_t* i, const int32_t* j, const float* c, const int* ids);
+    void round();
+    // Waits for the handed-off instances.
+    void finish() {
+        if (pool_.empty()) return;
+        {
+            std::lock_guard<std::mutex> l(qmu_);
+            closing_ = true;
+        }
+        qcv_.notify_all();
+        for (auto& t : pool_) t.join();
+        pool_.clear();
+        mark("workers done", -1, 0);
+        if (times_)
+            for (auto& kv : stage_ms_)
+                std::fprintf(stderr, "packed total %-14s %9.3f ms\n", kv.first.c_str(), kv.second);
+    }
+
+   private:
+    void worker();
+    void hand_off(Handoff&& h) {
+        {
+            std::lock_guard<std::mutex> l(qmu_);
+            tasks_.push_back(std::move(h));
+        }
+        qcv_.notify_one();
+    }
+    void mark(const char* what, int rounds, int slots) {
+        if (!times_) return;
+        ctx_.sync();
+        const auto now = std::chrono::steady_clock::now();
+        stage_ms_[what] += std::chrono::duration<double, std::milli>(now - tlast_).count();
+        tlast_ = now;
+        if (rounds >= 0 && std::string(what) != "round") return;
+        const double ms = std::chrono::duration<double, std::milli>(now - t0_).count();
+        std::fprintf(stderr, "packed %-14s %9.3f ms  round %d  slots %d\n", what, ms, rounds, slots);
+    }
+    // Publishes the listed slots as finished (labels from the pack).
+    void publish(const std::vector<int>& slots) {
+        const std::vector<int32_t> hl = gather_labels(ctx_, P_, slots);
+        i64 o = 0;
+        for (int q : slots) {
+            sink_.done(P_.s[q].id, 0, "", hl.data() + o, P_.s[q].n0, P_.s[q].lb);
+            o += P_.s[q].n0;
+        }
+    }
+    // Keeps only the slots with keep[q] (edges, labels, ids shifted down).
+    void compact(const std::vector<char>& keep);
+    // Hands the slots in `which` to the workers (next round of each: its iter).
+    void hand_off_slots(const std::vector<int>& which);
+
+    Ctx& ctx_;
+    Options o_;
+    LockstepSink& sink_;
+    Pack P_;
+    DBuf<unsigned> tickets_;
+    bool times_ = false;
+    std::chrono::steady_clock::time_point t0_, tlast_;
+    std::map<std::string, double> stage_ms_;
+    std::mutex qmu_;
+    std::condition_variable qcv_;
+    std::deque<Handoff> tasks_;
+    bool closing_ = false;
+    std::vector<std::thread> pool_;
+};
+
+void Lockstep::worker() {
+    std::unique_ptr<Ctx> wctx;
+    for (;;) {
+        Handoff h;
+        {
+            std::unique_lock<std::mutex> l(qmu_);
+            qcv_.wait(l, [&] { return closing_ || !tasks_.empty(); });
+            if (tasks_.empty()) return;
+            h = std::move(tasks_.front());
+            tasks_.pop_front();
+        }
+        try {
+            if (!wctx) wctx.reset(new Ctx(ctx_.device, nullptr, false));
+            Ctx& w = *wctx;
+            DCoo A;
+            A.nnz = (i64)h.val.size();
+            A.row = w.buf<int>(A.nnz);
+            A.col = w.buf<int>(A.nnz);
+            A.val = w.buf<float>(A.nnz);
+            w.h2d(A.row.get(), h.row.data(), A.nnz);
+            w.h2d(A.col.get(), h.col.data(), A.nnz);
+            w.h2d(A.val.get(), h.val.data(), A.nnz);
+            A.rows = h.rows;
+            A.cols = h.cols;
+            A.max_node = h.max_node;
+            SolveResult out;
+            out.num_labels = (i64)h.labels.size();
+            out.labels = w.buf<int>(out.num_labels);
+            w.h2d(out.labels.get(), h.labels.data(), out.num_labels);
+            out.lb = h.lb;
+            solve_rounds(w, A, out, o_, h.next_iter, false);
+            std::vector<int32_t> lab((size_t)out.num_labels);
+            if (out.num_labels > 0) w.d2h(lab.data(), out.labels.get(), out.num_labels);
+            sink_.done(h.id, 0, "", lab.data(), out.num_labels, h.lb);
+        } catch (const Error& e) {
+            sink_.done(h.id, e.status, e.what(), nullptr, 0, h.lb);
+        } catch (const std::exception& e) {
+            sink_.done(h.id, RAMA_ERR_INTERNAL, e.what(), nullptr, 0, h.lb);
+        }
     }
 }
 
-static void write_labels(PackedBatch& B, int id, const int32_t* lab, i64 n) {
-    B.n_labels[id] = n;
-    const i64 cap = B.label_off[id + 1] - B.label_off[id];
-    if (n > cap) throw Error(RAMA_ERR_INVALID, "label buffer too small for instance " + std::to_string(id));
-    if (n > 0) std::copy(lab, lab + n, B.labels_out + B.label_off[id]);
+void Lockstep::compact(const std::vector<char>& keep) {
+    const int Q = P_.Q();
+    std::vector<int> stay;
+    for (int q = 0; q < Q; ++q)
+        if (keep[q]) stay.push_back(q);
+    if ((int)stay.size() == Q) return;
+    if (stay.empty()) {
+        P_ = Pack();
+        return;
+    }
+    std::vector<int> kp((size_t)Q, 0), shift((size_t)Q, 0);
+    int removed = 0;
+    for (int q = 0; q < Q; ++q) {
+        shift[q] = removed;
+        kp[q] = keep[q] ? 1 : 0;
+        if (!keep[q]) removed += P_.s[q].span;
+    }
+    DBuf<int> dkeep = upload_vec(ctx_, kp), dshift = upload_vec(ctx_, shift);
+    const int* KP = dkeep.get();
+    const int* SH = dshift.get();
+    const int* NS = P_.nslot.get();
+    DCoo A2;
+    i64 m2 = 0;
+    for (int q : stay) m2 += P_.s[q].nnz;
+    A2.nnz = m2;
+    A2.row = ctx_.buf<int>(std::max<i64>(m2, 1));
+    A2.col = ctx_.buf<int>(std::max<i64>(m2, 1));
+    A2.val = ctx_.buf<float>(std::max<i64>(m2, 1));
+    {
+        const int* AR = P_.A.row.get();
+        const int* AC = P_.A.col.get();
+        const float* AW = P_.A.val.get();
+        int* R2 = A2.row.get();
+        int* C2 = A2.col.get();
+        float* W2 = A2.val.get();
+        DBuf<int> cnt = ctx_.buf<int>(1);
+        select_into(
+            ctx_, P_.A.nnz, [=] __device__(i64 e) { return KP[NS[AR[e]]] != 0; },
+            [=] __device__(i64 e, int p) {
+                const int q = NS[AR[e]];
+                R2[p] = AR[e] - SH[q];
+                C2[p] = AC[e] - SH[q];
+                W2[p] = AW[e];
+            },
+            cnt.get());
+    }
+    std::vector<i64> off, n;
+    for (int q : stay) {
+        off.push_back(P_.lab[q]);
+        n.push_back(P_.s[q].n0);
+    }
+    DBuf<int> labels = gather_ranges(ctx_, P_.labels.get(), off, n);
+    std::vector<Slot> ns;
+    for (int q : stay) ns.push_back(P_.s[q]);
+    P_.s = ns;
+    P_.A = std::move(A2);
+    P_.labels = std::move(labels);
+    pack_index(ctx_, P_);
 }
+
+void Lockstep::hand_off_slots(const std::vector<int>& which) {
+    if (which.empty()) return;
+    const int Q = P_.Q();
+    std::vector<int> isin((size_t)Q, 0);
+    std::vector<i64> hoff;
+    i64 mh = 0;
+    for (int q : which) {
+        isin[q] = 1;
+        hoff.push_back(mh);
+        mh += P_.s[q].nnz;
+    }
+    std::vector<int32_t> er((size_t)mh), ec((size_t)mh);
+    std::vector<float> ev((size_t)mh);
+    if (mh > 0) {
+        // the slots' edges, gathered on the device in slot order
+        DBuf<int> din = upload_vec(ctx_, isin);
+        const int* IN = din.get();
+        const int* NS = P_.nslot.get();
+        const int* AR = P_.A.row.get();
+        const int* AC = P_.A.col.get();
+        const float* AW = P_.A.val.get();
+        DBuf<int> gr = ctx_.buf<int>(mh), gc = ctx_.buf<int>(mh);
+        DBuf<float> gw = ctx_.buf<float>(mh);
+        int* GR = gr.get();
+        int* GC = gc.get();
+        float* GW = gw.get();
+        DBuf<int> cnt = ctx_.buf<int>(1);
+        select_into(
+            ctx_, P_.A.nnz, [=] __device__(i64 e) { return IN[NS[AR[e]]] != 0; },
+            [=] __device__(i64 e, int p) {
+                GR[p] = AR[e];
+                GC[p] = AC[e];
+                GW[p] = AW[e];
+            },
+            cnt.get());
+        ctx_.d2h(er.data(), gr.get(), mh);
+        ctx_.d2h(ec.data(), gc.get(), mh);
+        ctx_.d2h(ev.data(), gw.get(), mh);
+    }
+    const std::vector<int32_t> hl = gather_labels(ctx_, P_, which);
+    i64 lo = 0;
+    for (size_t x = 0; x < which.size(); ++x) {
+        const int q = which[x];
+        const Slot& s = P_.s[q];
+        Handoff h;
+        h.id = s.id;
+        h.next_iter = s.iter;
+        h.rows = s.rows;
+        h.cols = s.cols;
+        h.max_node = s.max_node;
+        h.lb = s.lb;
+        h.labels.assign(hl.begin() + lo, hl.begin() + lo + s.n0);
+        lo += s.n0;
+        h.row.resize((size_t)s.nnz);
+        h.col.resize((size_t)s.nnz);
+        h.val.resize((size_t)s.nnz);
+        for (i64 e = 0; e < s.nnz; ++e) {
+            h.row[e] = er[hoff[x] + e] - P_.base[q];
+            h.col[e] = ec[hoff[x] + e] - P_.base[q];
+            h.val[e] = ev[hoff[x] + e];
+        }
+        hand_off(std::move(h));
+    }
+}
+
+void Lockstep::admit(int k, const int64_t* edge_off, const int32_t* i, const int32_t* j, const float* c,
+                     const int* ids) {
+    if (k <= 0) return;
+    const i64 mall = edge_off[k] - edge_off[0];
+    const int64_t e0 = edge_off[0];
+    std::vector<int> hchk((size_t)2 * k, 0);
+    DBuf<int> ri, rj;
+    DBuf<float> rc;
+    DBuf<i64> deoff;
+    if (mall > 0) {
+        // raw edges up, then per-instance checks (max id, negative ids)
+        ri = ctx_.buf<int>(mall);
+        rj = ctx_.buf<int>(mall);
+        rc = ctx_.buf<float>(mall);
+        ctx_.h2d(ri.get(), i + e0, mall);
+        ctx_.h2d(rj.get(), j + e0, mall);
+        ctx_.h2d(rc.get(), c + e0, mall);
+        std::vector<i64> eoff((size_t)k + 1);
+        for (int q = 0; q <= k; ++q) eoff[q] = edge_off[q] - e0;
+        deoff = upload_vec(ctx_, eoff);
+        DBuf<int> chk = ctx_.buf<int>(2 * (i64)k);
+        int* CK = chk.get();
+        for_each(ctx_, 2 * (i64)k, [=] __device__(i64 x) { CK[x] = x < k ? -1 : 0; });
+        const int* I = ri.get();
+        const int* J = rj.get();
+        range_reduce<CheckAcc>(
+            ctx_, deoff.get(), k,
+            [=] __device__(i64 e, CheckAcc& a) {
+                const int x = I[e] > J[e] ? I[e] : J[e];
+                a.mx = x > a.mx ? x : a.mx;
+                a.neg |= (I[e] < 0 || J[e] < 0) ? 1 : 0;
+            },
+            CheckRed(),
+            [=] __device__(int q, CheckAcc r) {
+                atomicMax(&CK[q], r.mx);
+                if (r.neg) CK[k + q] = 1;
+            });
+        ctx_.d2h(hchk.data(), CK, 2 * (i64)k);
+    }
+    const int Q0 = P_.Q();
+    std::vector<int> ibase((size_t)k, -1);
+    i64 m = 0, off = P_.span_total;
+    for (int q = 0; q < k; ++q) {
+        const i64 mq = edge_off[q + 1] - edge_off[q];
+        if (mq <= 0) {
+            sink_.done(ids[q], RAMA_ERR_INVALID, "the graph has no edges", nullptr, 0, 0.0);
+            continue;
+        }
+        if (hchk[k + q]) {
+            sink_.done(ids[q], RAMA_ERR_INVALID, "node ids must be non-negative", nullptr, 0, 0.0);
+            continue;
+        }
+        if (P_.Q() >= kMaxSlots) {
+            sink_.done(ids[q], RAMA_ERR_INVALID, "too many instances in one lockstep pack", nullptr, 0, 0.0);
+            continue;
+        }
+        if (off + hchk[q] + 1 >= ((i64)1 << 31)) {
+            sink_.done(ids[q], RAMA_ERR_INVALID, "batch too large for 32-bit node ids", nullptr, 0, 0.0);
+            continue;
+        }
+        Slot s{};
+        s.id = ids[q];
+        s.span = hchk[q] + 1;
+        ibase[q] = (int)off;
+        off += s.span;
+        m += mq;
+        P_.s.push_back(s);
+    }
+    const int Q = P_.Q();
+    if (Q == Q0) return;
+    {
+        // the new instances' edges with global ids above every admitted one,
+        // sorted among themselves: appending keeps the joint list sorted
+        DBuf<int> dib = upload_vec(ctx_, ibase);
+        const int* IB = dib.get();
+        const int* I = ri.get();
+        const int* J = rj.get();
+        const float* Cc = rc.get();
+        const i64* EO = deoff.get();
+        DBuf<int> di = ctx_.buf<int>(m), dj = ctx_.buf<int>(m);
+        DBuf<float> dc = ctx_.buf<float>(m);
+        int* GI = di.get();
+        int* GJ = dj.get();
+        float* GC = dc.get();
+        DBuf<int> cnt = ctx_.buf<int>(1);
+        select_into(
+            ctx_, mall, [=] __device__(i64 e) { return IB[seg_of(EO, k, e)] >= 0; },
+            [=] __device__(i64 e, int p) {
+                const int b = IB[seg_of(EO, k, e)];
+                GI[p] = I[e] + b;
+                GJ[p] = J[e] + b;
+                GC[p] = Cc[e];
+            },
+            cnt.get());
+        ri.reset();
+        rj.reset();
+        rc.reset();
+        // dCOO(col=i, row=j) + init; the orientation makes both the same
+        DCoo An = dcoo_from_edges(ctx_, dj.get(), di.get(), dc.get(), m);
+        if (Q0 == 0) {
+            P_.A = std::move(An);
+        } else {
+            DCoo A2;
+            A2.nnz = P_.A.nnz + An.nnz;
+            A2.row = ctx_.buf<int>(A2.nnz);
+            A2.col = ctx_.buf<int>(A2.nnz);
+            A2.val = ctx_.buf<float>(A2.nnz);
+            ctx_.d2d(A2.row.get(), P_.A.row.get(), P_.A.nnz);
+            ctx_.d2d(A2.col.get(), P_.A.col.get(), P_.A.nnz);
+            ctx_.d2d(A2.val.get(), P_.A.val.get(), P_.A.nnz);
+            ctx_.d2d(A2.row.get() + P_.A.nnz, An.row.get(), An.nnz);
+            ctx_.d2d(A2.col.get() + P_.A.nnz, An.col.get(), An.nnz);
+            ctx_.d2d(A2.val.get() + P_.A.nnz, An.val.get(), An.nnz);
+            P_.A = std::move(A2);
+        }
+    }
+    std::vector<char> fresh((size_t)Q, 0);
+    for (int q = Q0; q < Q; ++q) fresh[q] = 1;
+    pack_index(ctx_, P_);
+    {
+        DBuf<SlotStats> st = ctx_.buf<SlotStats>(Q);
+        slot_stats_edges(ctx_, P_.A.row.get(), P_.A.col.get(), P_.A.nnz, P_.dbase.get(), Q, st.get());
+        apply_stats(ctx_, P_, st.get(), fresh);
+    }
+    for (int q = Q0; q < Q; ++q) P_.s[q].n0 = std::max(P_.s[q].rows, P_.s[q].cols);
+    mark("admitted", 0, Q);
+    // initial dual of the new instances (the bound)
+    {
+        const std::vector<double> lb = packed_dual(ctx_, P_, o_.max_cycle_length_lb, o_.num_dual_itr_lb,
+                                                   o_.tri_memory_factor, o_.num_outer_itr_dual, 1e-4f, fresh,
+                                                   tickets_.get());
+        for (int q = Q0; q < Q; ++q) P_.s[q].lb = lb[q];
+    }
+    mark("initial dual", 0, Q);
+    // labels: the old ones, then identity labels for the new instances
+    {
+        pack_index(ctx_, P_);  // label offsets with the new instances' n0
+        DBuf<int> L2 = ctx_.buf<int>(std::max<i64>(P_.lab[Q], 1));
+        if (Q0 > 0 && P_.lab[Q0] > 0) ctx_.d2d(L2.get(), P_.labels.get(), P_.lab[Q0]);
+        int* L = L2.get();
+        DBuf<i64> dlab = upload_vec(ctx_, P_.lab);
+        const i64* DL = dlab.get();
+        const i64 from = P_.lab[Q0];
+        for_each(ctx_, P_.lab[Q] - from, [=] __device__(i64 y) {
+            const i64 x = from + y;
+            L[x] = (int)(x - DL[seg_of(DL, Q, x)]);
+        });
+        P_.labels = std::move(L2);
+    }
+    if (o_.only_compute_lb) {
+        std::vector<char> keep((size_t)Q, 1);
+        for (int q = Q0; q < Q; ++q) {
+            sink_.done(P_.s[q].id, 0, "", nullptr, 0, P_.s[q].lb);
+            keep[q] = 0;
+        }
+        compact(keep);
+        return;
+    }
+    if (o_.matching_thresh_crossover_ratio > 1.0f) {
+        // the forest contraction from round 0 on: straight to the workers
+        std::vector<int> which;
+        std::vector<char> keep((size_t)Q, 1);
+        for (int q = Q0; q < Q; ++q) {
+            which.push_back(q);
+            keep[q] = 0;
+        }
+        hand_off_slots(which);
+        compact(keep);
+    }
+}
+
+void Lockstep::round() {
+    int Q = P_.Q();
+    if (Q == 0) return;
+    mark("round", P_.s[0].iter, Q);
+    // the primal dual of every slot past its first round
+    {
+        std::vector<char> on((size_t)Q, 0);
+        bool any = false;
+        for (int q = 0; q < Q; ++q) {
+            on[q] = P_.s[q].iter > 0;
+            any = any || on[q];
+        }
+        if (any)
+            packed_dual(ctx_, P_, o_.max_cycle_length_primal, o_.num_dual_itr_primal, 1.0f, 1, 1e-4f, on,
+                        tickets_.get());
+    }
+    mark("s:dual", 0, Q);
+    // matching on the joint symmetric CSR
+    Csr U = symmetric_csr(ctx_, P_.A.row.get(), P_.A.col.get(), P_.A.val.get(), P_.A.nnz, P_.span_total, false, 0.0f);
+    DBuf<i64> dhb = ctx_.buf<i64>(Q + 1);
+    {
+        const int* OFF = U.off.get();
+        const int* BS = P_.dbase.get();
+        i64* HB = dhb.get();
+        for_each(ctx_, Q + 1, [=] __device__(i64 q) { HB[q] = OFF[BS[q]]; });
+    }
+    std::vector<i64> hb((size_t)Q + 1);
+    ctx_.d2h(hb.data(), dhb.get(), Q + 1);
+    DBuf<int> map;
+    std::vector<int> matched((size_t)Q, 0);
+    matching_packed(ctx_, U, P_.nslot.get(), Q, hb.data(), dhb.get(), o_.mean_multiplier_mm, tickets_.get(), map,
+                    matched.data());
+    U = Csr();
+    mark("s:matching", 0, Q);
+    // per slot: finished (nothing to contract) or contracted, maybe switching
+    std::vector<int> un((size_t)Q, 0), cont((size_t)Q, 0);
+    std::vector<char> sw((size_t)Q, 0);
+    std::vector<int> finished;
+    for (int q = 0; q < Q; ++q) {
+        if (matched[q] == 0) {
+            finished.push_back(q);
+            continue;
+        }
+        cont[q] = 1;
+        un[q] = P_.s[q].max_node + 1;
+        sw[q] = (float)matched[q] < (float)P_.s[q].rows * o_.matching_thresh_crossover_ratio;
+    }
+    if (!finished.empty()) publish(finished);
+    if (finished.size() == (size_t)Q) {
+        P_ = Pack();
+        return;
+    }
+    DBuf<int> dun = upload_vec(ctx_, un), dcont = upload_vec(ctx_, cont);
+    const int* UN = dun.get();
+    const int* CT = dcont.get();
+    const int* NS = P_.nslot.get();
+    const int* BS = P_.dbase.get();
+    const int* MP = map.get();
+    // compress_label_sequence of the matching map, per slot
+    const i64 n = P_.span_total;
+    DBuf<int> rank = ctx_.buf<int>(n + 1);
+    int* RK = rank.get();
+    scan_exclusive<int>(
+        ctx_,
+        [=] __device__(i64 v) {
+            const int q = NS[v];
+            return (int)(CT[q] && (int)v - BS[q] < UN[q] && MP[v] == (int)v);
+        },
+        RK, n);
+    // new bases (rank at the slot's base) and spans, read back
+    DBuf<int> dnb = ctx_.buf<int>(2 * (i64)Q);
+    {
+        int* NB = dnb.get();
+        for_each(ctx_, Q, [=] __device__(i64 q) {
+            NB[q] = RK[BS[q]];
+            NB[Q + q] = RK[BS[q] + UN[q]] - RK[BS[q]];
+        });
+    }
+    std::vector<int> hnb((size_t)2 * Q);
+    ctx_.d2h(hnb.data(), dnb.get(), 2 * (i64)Q);
+    // labels (map_node_labels per slot: entries below Un move to the
+    // compressed label of their representative)
+    {
+        int* L = P_.labels.get();
+        DBuf<i64> dlab = upload_vec(ctx_, P_.lab);
+        const i64* DL = dlab.get();
+        for_each(ctx_, P_.lab[Q], [=] __device__(i64 x) {
+            const int q = seg_of(DL, Q, x);
+            if (!CT[q]) return;
+            const int v = L[x];
+            if (v < UN[q]) L[x] = RK[MP[BS[q] + v]] - RK[BS[q]];
+        });
+    }
+    // edges of finished slots leave before the contraction
+    if (!finished.empty()) {
+        std::vector<char> keep((size_t)Q, 0);
+        for (int q = 0; q < Q; ++q) keep[q] = cont[q] != 0;
+        DCoo A2;
+        const int* AR = P_.A.row.get();
+        const int* AC = P_.A.col.get();
+        const float* AW = P_.A.val.get();
+        i64 m2 = 0;
+        for (int q = 0; q < Q; ++q)
+            if (cont[q]) m2 += P_.s[q].nnz;
+        A2.nnz = m2;
+        A2.row = ctx_.buf<int>(std::max<i64>(m2, 1));
+        A2.col = ctx_.buf<int>(std::max<i64>(m2, 1));
+        A2.val = ctx_.buf<float>(std::max<i64>(m2, 1));
+        int* R2 = A2.row.get();
+        int* C2 = A2.col.get();
+        float* W2 = A2.val.get();
+        DBuf<int> cnt = ctx_.buf<int>(1);
+        select_into(
+            ctx_, P_.A.nnz, [=] __device__(i64 e) { return CT[NS[AR[e]]] != 0; },
+            [=] __device__(i64 e, int p) {
+                R2[p] = AR[e];
+                C2[p] = AC[e];
+                W2[p] = AW[e];
+            },
+            cnt.get());
+        P_.A = std::move(A2);
+    }
+    // contraction of the continuing slots (new slot index = rank among them)
+    std::vector<int> cidx, nbase_new;
+    for (int q = 0; q < Q; ++q)
+        if (cont[q]) {
+            cidx.push_back(q);
+            nbase_new.push_back(hnb[q]);
+        }
+    const int Qc = (int)cidx.size();
+    int total_new = 0;
+    for (int q : cidx) total_new += hnb[Q + q];
+    nbase_new.push_back(total_new);
+    DBuf<int> dnbn = upload_vec(ctx_, nbase_new);
+    DBuf<int> nns = ctx_.buf<int>(std::max(total_new, 1));
+    if (total_new > 0) {
+        k_node_slot<<<grid_for(total_new), kBlock, 0, ctx_.stream>>>(dnbn.get(), Qc, total_new, nns.get());
+        RAMA_LAUNCHED();
+    }
+    DBuf<int> nmap = ctx_.buf<int>(std::max<i64>(n, 1));
+    {
+        int* NM = nmap.get();
+        for_each(ctx_, n, [=] __device__(i64 v) { NM[v] = RK[MP[v]]; });
+    }
+    DCoo Bc;
+    DBuf<SlotStats> st = ctx_.buf<SlotStats>(Qc);
+    contract_packed(ctx_, P_.A, nmap.get(), std::max(total_new - 1, 0), nns.get(), dnbn.get(), Qc, Bc, st.get());
+    std::vector<SlotStats> hst((size_t)Qc);
+    ctx_.d2h(hst.data(), st.get(), Qc);
+    // the pack continues over the contracted slots in the new id space
+    std::vector<i64> off, cnt_lab;
+    std::vector<Slot> ns;
+    i64 nnz = 0;
+    for (int c = 0; c < Qc; ++c) {
+        Slot s = P_.s[cidx[c]];
+        off.push_back(P_.lab[cidx[c]]);
+        cnt_lab.push_back(s.n0);
+        s.cols = hst[c].max_col + 1;
+        s.rows = hst[c].max_row + 1;
+        s.max_node = hst[c].max_kept_col;
+        s.nnz = hst[c].kept;
+        s.span = hnb[Q + cidx[c]];
+        s.iter += 1;
+        nnz += s.nnz;
+        ns.push_back(s);
+    }
+    Bc.nnz = nnz;
+    DBuf<int> labels = gather_ranges(ctx_, P_.labels.get(), off, cnt_lab);
+    std::vector<char> swc;
+    for (int c = 0; c < Qc; ++c) swc.push_back(sw[cidx[c]]);
+    P_.s = ns;
+    P_.A = std::move(Bc);
+    P_.labels = std::move(labels);
+    pack_index(ctx_, P_);
+    mark("s:contract", 0, Qc);
+    // slots leaving: failed (bad contraction), done (no edges left), handed
+    // off (switching to the forest contraction)
+    std::vector<int> done, hand;
+    std::vector<char> keep((size_t)Qc, 1);
+    for (int c = 0; c < Qc; ++c) {
+        if (hst[c].bad) {
+            sink_.done(P_.s[c].id, RAMA_ERR_BAD_CONTRACTION, "Found bad contractions", nullptr, 0, P_.s[c].lb);
+            keep[c] = 0;
+        } else if (P_.s[c].nnz == 0) {
+            done.push_back(c);  // A.nnz == 0 ends the rounds
+            keep[c] = 0;
+        } else if (swc[c]) {
+            hand.push_back(c);
+            keep[c] = 0;
+        }
+    }
+    if (!done.empty()) publish(done);
+    hand_off_slots(hand);
+    compact(keep);
+}
+
+namespace {
+struct BatchSink : LockstepSink {
+    PackedBatch& B;
+    explicit BatchSink(PackedBatch& b) : B(b) {}
+    void done(int id, int status, const std::string& error, const int32_t* labels, i64 n, double lb) override {
+        B.lbs[id] = lb;
+        if (status == 0) {
+            const i64 cap = B.label_off[id + 1] - B.label_off[id];
+            if (n > cap) {
+                status = RAMA_ERR_INVALID;
+            } else {
+                B.n_labels[id] = n;
+                if (n > 0) std::copy(labels, labels + n, B.labels_out + B.label_off[id]);
+            }
+        }
+        if (B.status) B.status[id] = status;
+        if (status) {
+            std::lock_guard<std::mutex> l(*B.emu);
+            if (B.first_error->empty())
+                *B.first_error = status == RAMA_ERR_INVALID && error.empty()
+                                     ? "label buffer too small for instance " + std::to_string(id)
+                                     : error;
+        }
+    }
+};
+}  // namespace
 
 // Solves B.k instances in lockstep on ctx; instances that switch to the
 // forest contraction are continued by `workers` threads.  Per-instance
 // failures go to status; a failure of the whole pack throws.
 void solve_packed(Ctx& ctx, PackedBatch& B, int workers) {
-    const Options& o = B.o;
-    // RAMA_STAGE_TIMES: wall time of the packed phases on stderr (diagnostic)
-    const bool times = std::getenv("RAMA_STAGE_TIMES") != nullptr;
-    const auto tstart = std::chrono::steady_clock::now();
-    std::map<std::string, double> stage_ms;
-    auto tlast = tstart;
-    auto mark = [&](const char* what, int rounds, int slots) {
-        if (!times) return;
-        ctx.sync();
-        const auto now = std::chrono::steady_clock::now();
-        stage_ms[what] += std::chrono::duration<double, std::milli>(now - tlast).count();
-        tlast = now;
-        if (rounds >= 0 && std::string(what) != "round") return;
-        const double ms = std::chrono::duration<double, std::milli>(now - tstart).count();
-        std::fprintf(stderr, "packed %-14s %9.3f ms  round %d  slots %d\n", what, ms, rounds, slots);
-    };
-    const int k = B.k;
-    // the worker pool for handed-off instances
-    std::mutex qmu;
-    std::condition_variable qcv;
-    std::deque<Handoff> tasks;
-    bool closing = false;
-    std::vector<std::thread> pool;
-    auto worker = [&] {
-        std::unique_ptr<Ctx> wctx;
-        for (;;) {
-            Handoff h;
-            {
-                std::unique_lock<std::mutex> l(qmu);
-                qcv.wait(l, [&] { return closing || !tasks.empty(); });
-                if (tasks.empty()) return;
-                h = std::move(tasks.front());
-                tasks.pop_front();
-            }
-            try {
-                if (!wctx) wctx.reset(new Ctx(ctx.device, nullptr, false));
-                Ctx& w = *wctx;
-                DCoo A;
-                A.nnz = (i64)h.val.size();
-                A.row = w.buf<int>(A.nnz);
-                A.col = w.buf<int>(A.nnz);
-                A.val = w.buf<float>(A.nnz);
-                w.h2d(A.row.get(), h.row.data(), A.nnz);
-                w.h2d(A.col.get(), h.col.data(), A.nnz);
-                w.h2d(A.val.get(), h.val.data(), A.nnz);
-                A.rows = h.rows;
-                A.cols = h.cols;
-                A.max_node = h.max_node;
-                SolveResult out;
-                out.num_labels = (i64)h.labels.size();
-                out.labels = w.buf<int>(out.num_labels);
-                w.h2d(out.labels.get(), h.labels.data(), out.num_labels);
-                out.lb = h.lb;
-                solve_rounds(w, A, out, o, h.next_iter, false);
-                std::vector<int32_t> lab((size_t)out.num_labels);
-                if (out.num_labels > 0) w.d2h(lab.data(), out.labels.get(), out.num_labels);
-                write_labels(B, h.id, lab.data(), out.num_labels);
-                B.lbs[h.id] = h.lb;
-                set_status(B, h.id, 0, "");
-            } catch (const Error& e) {
-                set_status(B, h.id, e.status, e.what());
-            } catch (const std::exception& e) {
-                set_status(B, h.id, RAMA_ERR_INTERNAL, e.what());
-            }
-        }
-    };
-    for (int w = 0; w < std::max(1, workers); ++w) pool.emplace_back(worker);
-    auto hand_off = [&](Handoff&& h) {
-        {
-            std::lock_guard<std::mutex> l(qmu);
-            tasks.push_back(std::move(h));
-        }
-        qcv.notify_one();
-    };
-    auto finish_pool = [&] {
-        {
-            std::lock_guard<std::mutex> l(qmu);
-            closing = true;
-        }
-        qcv.notify_all();
-        for (auto& t : pool) t.join();
-    };
-
-    try {
-        // ---------------------------------------------- pack the inputs
-        // raw edges up, then per-instance checks (max id, negative ids) and
-        // the id offsets on the device
-        Pack P;
-        const i64 mall = B.edge_off[k];
-        if (mall <= 0) {
-            for (int q = 0; q < k; ++q) set_status(B, q, RAMA_ERR_INVALID, "the graph has no edges");
-            finish_pool();
-            return;
-        }
-        DBuf<int> ri = ctx.buf<int>(mall), rj = ctx.buf<int>(mall);
-        DBuf<float> rc = ctx.buf<float>(mall);
-        ctx.h2d(ri.get(), B.i, mall);
-        ctx.h2d(rj.get(), B.j, mall);
-        ctx.h2d(rc.get(), B.c, mall);
-        std::vector<i64> eoff(B.edge_off, B.edge_off + k + 1);
-        DBuf<i64> deoff = upload_vec(ctx, eoff);
-        DBuf<int> chk = ctx.buf<int>(2 * (i64)k);  // [0, k) max id, [k, 2k) negative flag
-        {
-            int* CK = chk.get();
-            for_each(ctx, 2 * (i64)k, [=] __device__(i64 x) { CK[x] = x < k ? -1 : 0; });
-            const int* I = ri.get();
-            const int* J = rj.get();
-            range_reduce<CheckAcc>(
-                ctx, deoff.get(), k,
-                [=] __device__(i64 e, CheckAcc& a) {
-                    const int x = I[e] > J[e] ? I[e] : J[e];
-                    a.mx = x > a.mx ? x : a.mx;
-                    a.neg |= (I[e] < 0 || J[e] < 0) ? 1 : 0;
-                },
-                CheckRed(),
-                [=] __device__(int q, CheckAcc r) {
-                    atomicMax(&CK[q], r.mx);
-                    if (r.neg) CK[k + q] = 1;
-                });
-        }
-        std::vector<int> hchk((size_t)2 * k);
-        ctx.d2h(hchk.data(), chk.get(), 2 * (i64)k);
-        std::vector<int> ibase((size_t)k, -1);
-        i64 m = 0, off = 0;
-        for (int q = 0; q < k; ++q) {
-            const i64 mq = B.edge_off[q + 1] - B.edge_off[q];
-            if (mq <= 0) {
-                set_status(B, q, RAMA_ERR_INVALID, "the graph has no edges");
-                continue;
-            }
-            if (hchk[k + q]) {
-                set_status(B, q, RAMA_ERR_INVALID, "node ids must be non-negative");
-                continue;
-            }
-            Slot s{};
-            s.id = q;
-            s.span = hchk[q] + 1;
-            ibase[q] = (int)off;
-            off += s.span;
-            m += mq;
-            RAMA_REQUIRE(off < ((i64)1 << 31), "batch too large for 32-bit node ids");
-            P.s.push_back(s);
-        }
-        if (P.Q() == 0) {
-            finish_pool();
-            return;
-        }
-        int Q = P.Q();
-        {
-            // the joint edge list with global ids (instances that failed the checks dropped)
-            DBuf<int> dib = upload_vec(ctx, ibase);
-            const int* IB = dib.get();
-            const int* I = ri.get();
-            const int* J = rj.get();
-            const float* Cc = rc.get();
-            const i64* EO = deoff.get();
-            DBuf<int> di = ctx.buf<int>(m), dj = ctx.buf<int>(m);
-            DBuf<float> dc = ctx.buf<float>(m);
-            int* GI = di.get();
-            int* GJ = dj.get();
-            float* GC = dc.get();
-            DBuf<int> cnt = ctx.buf<int>(1);
-            select_into(
-                ctx, mall, [=] __device__(i64 e) { return IB[seg_of(EO, k, e)] >= 0; },
-                [=] __device__(i64 e, int p) {
-                    const int b = IB[seg_of(EO, k, e)];
-                    GI[p] = I[e] + b;
-                    GJ[p] = J[e] + b;
-                    GC[p] = Cc[e];
-                },
-                cnt.get());
-            ri.reset();
-            rj.reset();
-            rc.reset();
-            // dCOO(col=i, row=j) + init; the orientation makes both the same
-            P.A = dcoo_from_edges(ctx, dj.get(), di.get(), dc.get(), m);
-        }
-        pack_index(ctx, P);
-        {
-            DBuf<SlotStats> st = ctx.buf<SlotStats>(Q);
-            slot_stats_clear(ctx, st.get(), Q);
-            slot_stats_edges(ctx, P.A.row.get(), P.A.col.get(), P.A.nnz, P.dbase.get(), Q, st.get());
-            apply_stats(ctx, P, st.get(), std::vector<char>((size_t)Q, 1));
-        }
-        for (int q = 0; q < Q; ++q) P.s[q].n0 = std::max(P.s[q].rows, P.s[q].cols);
-        DBuf<unsigned> tickets = ctx.buf<unsigned>(4096);
-        ctx.zero(tickets.get(), 4096);
-        RAMA_REQUIRE(k <= 4096, "at most 4096 instances per packed batch");
-
-        mark("packed", 0, Q);
-        // ------------------------------------------- initial dual (bound)
-        {
-            const std::vector<double> lb =
-                packed_dual(ctx, P, o.max_cycle_length_lb, o.num_dual_itr_lb, o.tri_memory_factor,
-                            o.num_outer_itr_dual, 1e-4f, std::vector<char>((size_t)Q, 1), tickets.get());
-            for (int q = 0; q < Q; ++q) {
-                P.s[q].lb = lb[q];
-                B.lbs[P.s[q].id] = lb[q];
-            }
-        }
-        mark("initial dual", 0, Q);
-        if (o.only_compute_lb) {
-            for (int q = 0; q < Q; ++q) {
-                B.n_labels[P.s[q].id] = 0;
-                set_status(B, P.s[q].id, 0, "");
-            }
-            finish_pool();
-            return;
-        }
-        pack_index(ctx, P);  // label offsets from n0
-        P.labels = ctx.buf<int>(std::max<i64>(P.lab[Q], 1));
-        {
-            int* L = P.labels.get();
-            DBuf<i64> dlab = upload_vec(ctx, P.lab);
-            const i64* DL = dlab.get();
-            for_each(ctx, P.lab[Q], [=] __device__(i64 x) { L[x] = (int)(x - DL[seg_of(DL, Q, x)]); });
-        }
-        const bool use_mm = !(o.matching_thresh_crossover_ratio > 1.0f);
-
-        // ------------------------------------------------- the rounds
-        for (int iter = 0; P.Q() > 0; ++iter) {
-            Q = P.Q();
-            mark("round", iter, Q);
-            std::vector<char> leave((size_t)Q, 0), handoff((size_t)Q, 0), contract_on((size_t)Q, 0);
-            if (!use_mm) {
-                for (int q = 0; q < Q; ++q) handoff[q] = 1;
-            } else {
-                if (iter > 0)
-                    packed_dual(ctx, P, o.max_cycle_length_primal, o.num_dual_itr_primal, 1.0f, 1, 1e-4f,
-                                std::vector<char>((size_t)Q, 1), tickets.get());
-                mark("s:dual", iter, Q);
-                // matching on the joint symmetric CSR
-                Csr U = symmetric_csr(ctx, P.A.row.get(), P.A.col.get(), P.A.val.get(), P.A.nnz, P.span_total,
-                                      false, 0.0f);
-                DBuf<i64> dhb = ctx.buf<i64>(Q + 1);
-                {
-                    const int* OFF = U.off.get();
-                    const int* BS = P.dbase.get();
-                    i64* HB = dhb.get();
-                    for_each(ctx, Q + 1, [=] __device__(i64 q) { HB[q] = OFF[BS[q]]; });
-                }
-                std::vector<i64> hb((size_t)Q + 1);
-                ctx.d2h(hb.data(), dhb.get(), Q + 1);
-                DBuf<int> map;
-                std::vector<int> matched((size_t)Q, 0);
-                matching_packed(ctx, U, P.nslot.get(), Q, hb.data(), dhb.get(), o.mean_multiplier_mm,
-                                tickets.get(), map, matched.data());
-                U = Csr();
-                mark("s:matching", iter, Q);
-                // per slot: finished, or contraction (and maybe the switch)
-                std::vector<int> un((size_t)Q, 0), cont((size_t)Q, 0);
-                bool any_done = false, any_cont = false;
-                for (int q = 0; q < Q; ++q) {
-                    if (matched[q] == 0) {
-                        leave[q] = 1;
-                        any_done = true;
-                        continue;
-                    }
-                    contract_on[q] = 1;
-                    cont[q] = 1;
-                    any_cont = true;
-                    un[q] = P.s[q].max_node + 1;
-                    if ((float)matched[q] < (float)P.s[q].rows * o.matching_thresh_crossover_ratio) handoff[q] = 1;
-                }
-                if (any_cont) {
-                    DBuf<int> dun = upload_vec(ctx, un), dcont = upload_vec(ctx, cont);
-                    const int* UN = dun.get();
-                    const int* CT = dcont.get();
-                    const int* NS = P.nslot.get();
-                    const int* BS = P.dbase.get();
-                    const int* MP = map.get();
-                    // compress_label_sequence of the matching map, per slot
-                    const i64 n = P.span_total;
-                    DBuf<int> rank = ctx.buf<int>(n + 1);
-                    int* RK = rank.get();
-                    scan_exclusive<int>(
-                        ctx,
-                        [=] __device__(i64 v) {
-                            const int q = NS[v];
-                            return (int)(CT[q] && (int)v - BS[q] < UN[q] && MP[v] == (int)v);
-                        },
-                        RK, n);
-                    // new bases (rank at the slot's base) and spans, read back
-                    DBuf<int> dnb = ctx.buf<int>(2 * (i64)Q);
-                    int* NB = dnb.get();
-                    for_each(ctx, Q, [=] __device__(i64 q) {
-                        NB[q] = RK[BS[q]];
-                        NB[Q + q] = RK[BS[q] + UN[q]] - RK[BS[q]];
-                    });
-                    std::vector<int> hnb((size_t)2 * Q);
-                    ctx.d2h(hnb.data(), NB, 2 * (i64)Q);
-                    // edges of finished slots leave before the contraction
-                    if (any_done) {
-                        DCoo A2;
-                        const int* AR = P.A.row.get();
-                        const int* AC = P.A.col.get();
-                        const float* AW = P.A.val.get();
-                        A2.row = ctx.buf<int>(P.A.nnz);
-                        A2.col = ctx.buf<int>(P.A.nnz);
-                        A2.val = ctx.buf<float>(P.A.nnz);
-                        int* R2 = A2.row.get();
-                        int* C2 = A2.col.get();
-                        float* W2 = A2.val.get();
-                        DBuf<int> cnt = ctx.buf<int>(1);
-                        select_into(
-                            ctx, P.A.nnz, [=] __device__(i64 e) { return CT[NS[AR[e]]] != 0; },
-                            [=] __device__(i64 e, int p) {
-                                R2[p] = AR[e];
-                                C2[p] = AC[e];
-                                W2[p] = AW[e];
-                            },
-                            cnt.get());
-                        i64 m2 = 0;
-                        for (int q = 0; q < Q; ++q)
-                            if (cont[q]) m2 += P.s[q].nnz;
-                        A2.nnz = m2;
-                        P.A = std::move(A2);
-                    }
-                    // contraction of the continuing slots (new slot = rank among them)
-                    std::vector<int> cidx, nbase_new;
-                    for (int q = 0; q < Q; ++q)
-                        if (cont[q]) {
-                            cidx.push_back(q);
-                            nbase_new.push_back(hnb[q]);
-                        }
-                    const int Qc = (int)cidx.size();
-                    int total_new = 0;
-                    for (int q : cidx) total_new += hnb[Q + q];
-                    nbase_new.push_back(total_new);
-                    DBuf<int> dnbn = upload_vec(ctx, nbase_new);
-                    DBuf<int> nns = ctx.buf<int>(std::max(total_new, 1));
-                    if (total_new > 0) {
-                        k_node_slot<<<grid_for(total_new), kBlock, 0, ctx.stream>>>(dnbn.get(), Qc, total_new,
-                                                                                     nns.get());
-                        RAMA_LAUNCHED();
-                    }
-                    DBuf<int> nmap = ctx.buf<int>(std::max<i64>(n, 1));
-                    {
-                        int* NM = nmap.get();
-                        for_each(ctx, n, [=] __device__(i64 v) { NM[v] = RK[MP[v]]; });
-                    }
-                    DCoo Bc;
-                    DBuf<SlotStats> st = ctx.buf<SlotStats>(Qc);
-                    contract_packed(ctx, P.A, nmap.get(), std::max(total_new - 1, 0), nns.get(), dnbn.get(), Qc,
-                                    Bc, st.get());
-                    std::vector<SlotStats> hst((size_t)Qc);
-                    ctx.d2h(hst.data(), st.get(), Qc);
-                    // labels (map_node_labels per slot: entries below Un move)
-                    {
-                        int* L = P.labels.get();
-                        DBuf<i64> dlab = upload_vec(ctx, P.lab);
-                        const i64* DL = dlab.get();
-                        const i64 nl = P.lab[Q];
-                        for_each(ctx, nl, [=] __device__(i64 x) {
-                            const int q = seg_of(DL, Q, x);
-                            if (!CT[q]) return;
-                            const int m = L[x];
-                            // cur[m] = compressed label of m's representative
-                            if (m < UN[q]) L[x] = RK[MP[BS[q] + m]] - RK[BS[q]];
-                        });
-                    }
-                    i64 nnz = 0;
-                    for (int c = 0; c < Qc; ++c) {
-                        Slot& s = P.s[cidx[c]];
-                        if (hst[c].bad) {
-                            set_status(B, s.id, RAMA_ERR_BAD_CONTRACTION, "Found bad contractions");
-                            leave[cidx[c]] = 2;  // failed
-                        }
-                        s.cols = hst[c].max_col + 1;
-                        s.rows = hst[c].max_row + 1;
-                        s.max_node = hst[c].max_kept_col;
-                        s.nnz = hst[c].kept;
-                        s.span = hnb[Q + cidx[c]];
-                        nnz += s.nnz;
-                        if (s.nnz == 0) leave[cidx[c]] = 1;  // A.nnz == 0 ends the rounds
-                    }
-                    Bc.nnz = nnz;
-                    // the pack now lives in the new id space of the continuing slots
-                    std::vector<Slot> ns;
-                    std::vector<char> nleave, nhand;
-                    std::vector<i64> old_lab = P.lab;
-                    DBuf<int> labels_old = std::move(P.labels);
-                    // labels of slots that do not continue are final: copy them out
-                    {
-                        std::vector<int> fin;
-                        for (int q = 0; q < Q; ++q)
-                            if (!cont[q] && leave[q] == 1) fin.push_back(q);
-                        P.labels = std::move(labels_old);
-                        const std::vector<int32_t> hl = gather_labels(ctx, P, fin);
-                        labels_old = std::move(P.labels);
-                        i64 o = 0;
-                        for (int q : fin) {
-                            write_labels(B, P.s[q].id, hl.data() + o, P.s[q].n0);
-                            set_status(B, P.s[q].id, 0, "");
-                            o += P.s[q].n0;
-                        }
-                    }
-                    P.A = std::move(Bc);
-                    std::vector<Slot> keep_s;
-                    for (int c = 0; c < Qc; ++c) keep_s.push_back(P.s[cidx[c]]);
-                    std::vector<char> keep_leave, keep_hand;
-                    for (int c = 0; c < Qc; ++c) {
-                        keep_leave.push_back(leave[cidx[c]]);
-                        keep_hand.push_back(handoff[cidx[c]]);
-                    }
-                    // rebuild the pack over the continuing slots
-                    std::vector<i64> keep_lab_off;
-                    for (int c = 0; c < Qc; ++c) keep_lab_off.push_back(old_lab[cidx[c]]);
-                    P.s = keep_s;
-                    pack_index(ctx, P);
-                    // compact the labels of the continuing slots
-                    {
-                        std::vector<i64> n;
-                        for (int c = 0; c < Qc; ++c) n.push_back(P.s[c].n0);
-                        P.labels = gather_ranges(ctx, labels_old.get(), keep_lab_off, n);
-                    }
-                    leave.assign(keep_leave.begin(), keep_leave.end());
-                    handoff.assign(keep_hand.begin(), keep_hand.end());
-                } else {
-                    // nothing contracts: every slot is finished
-                    std::vector<int32_t> hl((size_t)P.lab[Q]);
-                    if (P.lab[Q] > 0) ctx.d2h(hl.data(), P.labels.get(), P.lab[Q]);
-                    for (int q = 0; q < Q; ++q) {
-                        write_labels(B, P.s[q].id, hl.data() + P.lab[q], P.s[q].n0);
-                        set_status(B, P.s[q].id, 0, "");
-                    }
-                    P.s.clear();
-                    break;
-                }
-            }
-            mark("s:contract", iter, Q);
-            // ------------------------------ slots leaving after this round
-            Q = P.Q();
-            bool any_leave = false;
-            for (int q = 0; q < Q; ++q) any_leave = any_leave || leave[q] || handoff[q];
-            if (!any_leave) continue;
-            std::vector<int> out_slots, hand_slots, stay;
-            for (int q = 0; q < Q; ++q) {
-                if (leave[q] == 1 || (handoff[q] && leave[q] != 2)) out_slots.push_back(q);
-                if (handoff[q] && !leave[q]) hand_slots.push_back(q);
-                if (!leave[q] && !handoff[q]) stay.push_back(q);
-            }
-            const std::vector<int32_t> hl = gather_labels(ctx, P, out_slots);
-            // edges of the handed-off slots, gathered on the device in slot order
-            std::vector<int32_t> er, ec;
-            std::vector<float> ev;
-            std::vector<i64> hoff;
-            if (!hand_slots.empty()) {
-                std::vector<int> isin((size_t)Q, 0);
-                i64 mh = 0;
-                for (int q : hand_slots) {
-                    isin[q] = 1;
-                    hoff.push_back(mh);
-                    mh += P.s[q].nnz;
-                }
-                er.resize((size_t)mh);
-                ec.resize((size_t)mh);
-                ev.resize((size_t)mh);
-                if (mh > 0) {
-                    DBuf<int> din = upload_vec(ctx, isin);
-                    const int* IN = din.get();
-                    const int* NS = P.nslot.get();
-                    const int* AR = P.A.row.get();
-                    const int* AC = P.A.col.get();
-                    const float* AW = P.A.val.get();
-                    DBuf<int> gr = ctx.buf<int>(mh), gc = ctx.buf<int>(mh);
-                    DBuf<float> gw = ctx.buf<float>(mh);
-                    int* GR = gr.get();
-                    int* GC = gc.get();
-                    float* GW = gw.get();
-                    DBuf<int> cnt = ctx.buf<int>(1);
-                    select_into(
-                        ctx, P.A.nnz, [=] __device__(i64 e) { return IN[NS[AR[e]]] != 0; },
-                        [=] __device__(i64 e, int p) {
-                            GR[p] = AR[e];
-                            GC[p] = AC[e];
-                            GW[p] = AW[e];
-                        },
-                        cnt.get());
-                    ctx.d2h(er.data(), gr.get(), mh);
-                    ctx.d2h(ec.data(), gc.get(), mh);
-                    ctx.d2h(ev.data(), gw.get(), mh);
-                }
-            }
-            i64 lo = 0;
-            size_t hi_idx = 0;
-            for (int q : out_slots) {
-                const Slot& s = P.s[q];
-                if (leave[q] == 1) {
-                    write_labels(B, s.id, hl.data() + lo, s.n0);
-                    set_status(B, s.id, 0, "");
-                } else {
-                    Handoff h;
-                    h.id = s.id;
-                    h.next_iter = use_mm ? iter + 1 : iter;
-                    h.rows = s.rows;
-                    h.cols = s.cols;
-                    h.max_node = s.max_node;
-                    h.lb = s.lb;
-                    h.labels.assign(hl.begin() + lo, hl.begin() + lo + s.n0);
-                    const i64 e0 = hoff[hi_idx++];
-                    h.row.resize((size_t)s.nnz);
-                    h.col.resize((size_t)s.nnz);
-                    h.val.resize((size_t)s.nnz);
-                    for (i64 e = 0; e < s.nnz; ++e) {
-                        h.row[e] = er[e0 + e] - P.base[q];
-                        h.col[e] = ec[e0 + e] - P.base[q];
-                        h.val[e] = ev[e0 + e];
-                    }
-                    hand_off(std::move(h));
-                }
-                lo += s.n0;
-            }
-            if (stay.empty()) {
-                P.s.clear();
-                break;
-            }
-            // compact the pack: edges and labels of the staying slots, ids shifted down
-            std::vector<int> keep((size_t)Q, 0), shift((size_t)Q, 0);
-            int removed = 0;
-            for (int q = 0; q < Q; ++q) {
-                shift[q] = removed;
-                const bool st = std::find(stay.begin(), stay.end(), q) != stay.end();
-                keep[q] = st;
-                if (!st) removed += P.s[q].span;
-            }
-            DBuf<int> dkeep = upload_vec(ctx, keep), dshift = upload_vec(ctx, shift);
-            const int* KP = dkeep.get();
-            const int* SH = dshift.get();
-            const int* NS = P.nslot.get();
-            DCoo A2;
-            i64 m2 = 0;
-            for (int q : stay) m2 += P.s[q].nnz;
-            A2.nnz = m2;
-            A2.row = ctx.buf<int>(std::max<i64>(m2, 1));
-            A2.col = ctx.buf<int>(std::max<i64>(m2, 1));
-            A2.val = ctx.buf<float>(std::max<i64>(m2, 1));
-            {
-                const int* AR = P.A.row.get();
-                const int* AC = P.A.col.get();
-                const float* AW = P.A.val.get();
-                int* R2 = A2.row.get();
-                int* C2 = A2.col.get();
-                float* W2 = A2.val.get();
-                DBuf<int> cnt = ctx.buf<int>(1);
-                select_into(
-                    ctx, P.A.nnz, [=] __device__(i64 e) { return KP[NS[AR[e]]] != 0; },
-                    [=] __device__(i64 e, int p) {
-                        const int q = NS[AR[e]];
-                        R2[p] = AR[e] - SH[q];
-                        C2[p] = AC[e] - SH[q];
-                        W2[p] = AW[e];
-                    },
-                    cnt.get());
-            }
-            std::vector<i64> old_lab = P.lab;
-            DBuf<int> labels_old = std::move(P.labels);
-            std::vector<Slot> ns;
-            for (int q : stay) ns.push_back(P.s[q]);
-            P.s = ns;
-            P.A = std::move(A2);
-            pack_index(ctx, P);
-            {
-                std::vector<i64> off, n;
-                for (size_t c = 0; c < stay.size(); ++c) {
-                    off.push_back(old_lab[stay[c]]);
-                    n.push_back(P.s[c].n0);
-                }
-                P.labels = gather_ranges(ctx, labels_old.get(), off, n);
-            }
-        }
-        scan_check(ctx);
-        mark("pack empty", -1, 0);
-    } catch (...) {
-        finish_pool();
-        throw;
-    }
-    finish_pool();
-    mark("workers done", -1, 0);
-    if (times)
-        for (auto& kv : stage_ms) std::fprintf(stderr, "packed total %-14s %9.3f ms\n", kv.first.c_str(), kv.second);
+    BatchSink sink(B);
+    Lockstep L(ctx, B.o, workers, sink);
+    RAMA_REQUIRE(B.k <= Lockstep::kMaxSlots, "at most 4096 instances per packed batch");
+    std::vector<int> ids((size_t)B.k);
+    for (int q = 0; q < B.k; ++q) ids[q] = q;
+    L.admit(B.k, B.edge_off, B.i, B.j, B.c, ids.data());
+    while (!L.empty()) L.round();
+    scan_check(ctx);
+    L.finish();
 }
 
 }  // namespace rama
