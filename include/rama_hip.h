@@ -211,10 +211,14 @@ int rama_solve_batch_host(int32_t k, const int64_t* edge_off, const int32_t* i, 
                           int64_t* n_labels, double* lbs, int32_t* status);
 
 /* Serving loop with continuous admission: requests are queued in arrival
- * order and admitted by `workers` device workers while the device-memory
- * estimate of the requests in flight stays within hbm_budget_bytes; each
- * request's graph and dual state stay resident on the device until its
- * labels are back on the host. */
+ * order and admitted by `workers` lanes (threads with a stream each) while
+ * the device-memory estimate of the requests in flight stays within
+ * hbm_budget_bytes.  A lane solves a lone request with the plain solver and
+ * a group of queued requests as one lockstep pack, which admits further
+ * requests between its rounds while every lane is busy
+ * (RAMA_SERVE_MODE=workers: one request per lane).  Each request's graph
+ * and dual state stay resident on the device until its labels are back on
+ * the host. */
 typedef struct rama_server rama_server;
 rama_server* rama_server_create(int device, int workers, int64_t hbm_budget_bytes, const rama_options_c* opts);
 /* Copies the request; returns its ticket (>= 0) or -1. */

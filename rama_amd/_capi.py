@@ -341,11 +341,25 @@ def solve_batch(instances, opts=None, workers=8):
 class Server:
     """Serving loop (rama_server_*): submit instances as they arrive, wait for
     results; admission keeps the in-flight device-memory estimate within the
-    HBM budget."""
+    HBM budget.  mode "lockstep" (default where the options allow) packs the
+    admitted requests and admits new ones between rounds; "workers" solves
+    one request per worker thread (RAMA_SERVE_MODE)."""
 
-    def __init__(self, workers=8, hbm_budget_bytes=200 << 30, opts=None, device=0):
+    def __init__(self, workers=8, hbm_budget_bytes=200 << 30, opts=None, device=0, mode=None):
         self._opts = opts or options(verbose=0)
-        self._h = lib().rama_server_create(device, workers, hbm_budget_bytes, ctypes.byref(self._opts))
+        if mode not in (None, "lockstep", "workers"):
+            raise ValueError("mode must be 'lockstep' or 'workers'")
+        old = os.environ.get("RAMA_SERVE_MODE")
+        try:
+            if mode is not None:
+                os.environ["RAMA_SERVE_MODE"] = mode
+            self._h = lib().rama_server_create(device, workers, hbm_budget_bytes, ctypes.byref(self._opts))
+        finally:
+            if mode is not None:
+                if old is None:
+                    os.environ.pop("RAMA_SERVE_MODE", None)
+                else:
+                    os.environ["RAMA_SERVE_MODE"] = old
         if not self._h:
             raise RamaError(RAMA_ERR_INTERNAL_STATUS, lib().rama_last_error().decode())
         self._caps = {}

@@ -55,8 +55,9 @@ struct rama_server {
         int64_t ticket;
         std::vector<int32_t> i, j;
         std::vector<float> c;
-        Clock::time_point submitted;
+        Clock::time_point submitted, admitted;
         double est = 0.0;
+        int lock_id = -1;
         // result
         bool done = false;
         int status = 0;
@@ -80,9 +81,63 @@ struct rama_server {
     int64_t completed = 0;
     std::vector<std::thread> workers;
 
-    rama_server(int dev, int nworkers, double hbm_budget, const rama_options_c& o)
+    // Continuous batching over lanes.  Each lane is a thread with its own
+    // stream; lanes run side by side on the device.  An idle lane takes its
+    // share of the queue: a lone request runs the plain solver (the lowest
+    // latency for one instance), two or more become a lockstep pack that
+    // shares every round's launches.  While every lane is busy, a lane with
+    // a pack admits the queued requests between its rounds, where they run
+    // their initial dual and join the next round.  Admission always keeps
+    // the in-flight estimate within the HBM budget.  Options the lockstep
+    // path does not carry, or RAMA_SERVE_MODE=workers, give one request per
+    // lane.
+    bool pack = false;
+    int idle = 0;  // lanes waiting for work
+    std::unordered_map<int, std::shared_ptr<Request>> by_id;  // lockstep ids of packed requests
+    int next_id = 0;
+    std::string dead;  // set once a lane has failed
+
+    void finish(const std::shared_ptr<Request>& r) {
+        {
+            std::lock_guard<std::mutex> l(mu);
+            in_flight -= r->est;
+            r->done = true;
+            ++completed;
+        }
+        cv_done.notify_all();
+        cv_work.notify_all();
+    }
+
+    struct Sink : LockstepSink {
+        rama_server* s;
+        explicit Sink(rama_server* x) : s(x) {}
+        void done(int id, int status, const std::string& error, const int32_t* labels, i64 n, double lb) override {
+            std::shared_ptr<Request> r;
+            {
+                std::lock_guard<std::mutex> l(s->mu);
+                auto it = s->by_id.find(id);
+                if (it == s->by_id.end()) return;
+                r = it->second;
+                s->by_id.erase(it);
+            }
+            const auto t1 = Clock::now();
+            r->status = status;
+            r->error = error;
+            r->lb = lb;
+            if (status == 0 && n > 0) r->labels.assign(labels, labels + n);
+            r->latency_ms = std::chrono::duration<double, std::milli>(t1 - r->submitted).count();
+            r->solve_ms = std::chrono::duration<double, std::milli>(t1 - r->admitted).count();
+            s->finish(r);
+        }
+    };
+    Sink sink{this};
+
+    rama_server(int dev, int lanes, double hbm_budget, const rama_options_c& o)
         : device(dev), opts(o), budget(hbm_budget) {
-        for (int w = 0; w < nworkers; ++w) workers.emplace_back([this] { run(); });
+        const char* mode = std::getenv("RAMA_SERVE_MODE");
+        pack = !(mode && std::string(mode) == "workers") && o.sanitize_graph == 0 && o.dump_timeline == 0 &&
+               o.max_time_sec < 0 && o.verbose == 0;
+        for (int w = 0; w < lanes; ++w) workers.emplace_back([this] { lane(); });
     }
     ~rama_server() {
         {
@@ -93,37 +148,107 @@ struct rama_server {
         for (auto& t : workers) t.join();
     }
 
-    void run() {
+    // Fails every packed and queued request and refuses later ones (a
+    // lane's device state is lost).
+    void fail_all(const std::string& msg) {
+        std::vector<std::shared_ptr<Request>> rs;
+        {
+            std::lock_guard<std::mutex> l(mu);
+            dead = msg.empty() ? "serving stopped after a device error" : msg;
+            for (auto& kv : by_id) rs.push_back(kv.second);
+            by_id.clear();
+            for (auto& r : queue) rs.push_back(r);
+            queue.clear();
+            for (auto& r : rs) {
+                r->status = RAMA_ERR_INTERNAL;
+                r->error = dead;
+                r->done = true;
+            }
+        }
+        cv_done.notify_all();
+        cv_work.notify_all();
+    }
+
+    // Under mu: the head request may start now.
+    bool fits() const {
+        return !queue.empty() && (in_flight == 0.0 || in_flight + queue.front()->est <= budget);
+    }
+    std::shared_ptr<Request> pop_head() {
+        auto r = queue.front();
+        queue.pop_front();
+        in_flight += r->est;
+        r->admitted = Clock::now();
+        return r;
+    }
+
+    void lane() {
         std::unique_ptr<Ctx> ctx;
+        LockstepHandle* L = nullptr;
         try {
             ctx.reset(new Ctx(device, nullptr, false));
-        } catch (const std::exception&) {
+            if (pack) L = lockstep_create(*ctx, to_options(opts), 2, sink);
+        } catch (const std::exception& e) {
+            fail_all(e.what());
             return;
         }
-        for (;;) {
-            std::shared_ptr<Request> r;
-            {
-                std::unique_lock<std::mutex> l(mu);
-                // admit the head request once it fits the budget (or nothing else is running)
-                cv_work.wait(l, [&] {
-                    return stopping || (!queue.empty() && (in_flight == 0.0 || in_flight + queue.front()->est <= budget));
-                });
-                if (stopping && queue.empty()) return;
-                if (queue.empty()) continue;
-                r = queue.front();
-                queue.pop_front();
-                in_flight += r->est;
+        try {
+            for (;;) {
+                const int inpack = L ? lockstep_slots(L) : 0;
+                std::vector<std::shared_ptr<Request>> group;
+                {
+                    std::unique_lock<std::mutex> l(mu);
+                    if (inpack == 0) {
+                        ++idle;
+                        cv_work.wait(l, [&] { return fits() || (stopping && queue.empty()) || !dead.empty(); });
+                        --idle;
+                        if (!dead.empty() || !fits()) break;
+                        // this lane's share of the queue (the other idle lanes take theirs)
+                        size_t take = pack ? (queue.size() + (size_t)idle) / (size_t)(idle + 1) : 1;
+                        take = std::min<size_t>(take, (size_t)kLockstepMaxSlots);
+                        while (take-- > 0 && fits()) group.push_back(pop_head());
+                    } else if (idle == 0) {
+                        // every lane is busy: queued requests join this pack
+                        while (fits() && inpack + (int)group.size() < kLockstepMaxSlots) group.push_back(pop_head());
+                    }
+                    if (group.size() > 1 || (!group.empty() && inpack > 0))
+                        for (auto& r : group) {
+                            r->lock_id = next_id++;
+                            by_id[r->lock_id] = r;
+                        }
+                }
+                if (group.size() == 1 && inpack == 0) {
+                    process(*ctx, *group[0]);
+                    finish(group[0]);
+                    continue;
+                }
+                if (!group.empty()) admit(L, group);
+                if (L && lockstep_slots(L) > 0) lockstep_round(L);
             }
-            process(*ctx, *r);
-            {
-                std::lock_guard<std::mutex> l(mu);
-                in_flight -= r->est;
-                r->done = true;
-                ++completed;
-            }
-            cv_done.notify_all();
-            cv_work.notify_all();
+        } catch (const std::exception& e) {
+            fail_all(e.what());
         }
+        if (L) lockstep_destroy(L);  // waits for the handed-off requests
+    }
+
+    // Packs the group's edges one after another and admits them.
+    void admit(LockstepHandle* L, std::vector<std::shared_ptr<Request>>& group) {
+        const int k = (int)group.size();
+        std::vector<int64_t> off((size_t)k + 1, 0);
+        for (int q = 0; q < k; ++q) off[q + 1] = off[q] + (int64_t)group[q]->c.size();
+        std::vector<int32_t> gi((size_t)off[k]), gj((size_t)off[k]);
+        std::vector<float> gc((size_t)off[k]);
+        std::vector<int> ids((size_t)k);
+        for (int q = 0; q < k; ++q) {
+            Request& r = *group[q];
+            std::copy(r.i.begin(), r.i.end(), gi.begin() + off[q]);
+            std::copy(r.j.begin(), r.j.end(), gj.begin() + off[q]);
+            std::copy(r.c.begin(), r.c.end(), gc.begin() + off[q]);
+            ids[q] = r.lock_id;
+            std::vector<int32_t>().swap(r.i);
+            std::vector<int32_t>().swap(r.j);
+            std::vector<float>().swap(r.c);
+        }
+        lockstep_admit(L, k, off.data(), gi.data(), gj.data(), gc.data(), ids.data());
     }
 
     void process(Ctx& ctx, Request& r) {
@@ -184,6 +309,10 @@ int64_t rama_server_submit(rama_server* s, const int32_t* i, const int32_t* j, c
     r->submitted = Clock::now();
     {
         std::lock_guard<std::mutex> l(s->mu);
+        if (!s->dead.empty()) {
+            set_last_error(s->dead);
+            return -1;
+        }
         r->ticket = s->next_ticket++;
         s->queue.push_back(r);
         s->all[r->ticket] = r;

@@ -571,12 +571,6 @@ struct Handoff {
 
 // ------------------------------------------------------------- driver
 
-// Receives each instance's outcome (from the lockstep thread or a worker).
-struct LockstepSink {
-    virtual ~LockstepSink() = default;
-    virtual void done(int id, int status, const std::string& error, const int32_t* labels, i64 n, double lb) = 0;
-};
-
 // The lockstep solver: instances are admitted in groups (each group runs
 // its initial dual on arrival), and every call of round() advances all
 // admitted instances by one contraction round.  Instances leave when they
@@ -593,7 +587,7 @@ class Lockstep {
     }
     ~Lockstep() { finish(); }
 
-    static constexpr int kMaxSlots = 4096;
+    static constexpr int kMaxSlots = kLockstepMaxSlots;
     int slots() const { return P_.Q(); }
     bool empty() const { return P_.Q() == 0; }
 
@@ -1240,5 +1234,21 @@ void solve_packed(Ctx& ctx, PackedBatch& B, int workers) {
     scan_check(ctx);
     L.finish();
 }
+
+// Opaque handle for callers outside this file (the serving loop).
+struct LockstepHandle {
+    Lockstep engine;
+    LockstepHandle(Ctx& ctx, const Options& o, int workers, LockstepSink& sink) : engine(ctx, o, workers, sink) {}
+};
+LockstepHandle* lockstep_create(Ctx& ctx, const Options& o, int workers, LockstepSink& sink) {
+    return new LockstepHandle(ctx, o, workers, sink);
+}
+void lockstep_admit(LockstepHandle* h, int k, const int64_t* edge_off, const int32_t* i, const int32_t* j,
+                    const float* c, const int* ids) {
+    h->engine.admit(k, edge_off, i, j, c, ids);
+}
+void lockstep_round(LockstepHandle* h) { h->engine.round(); }
+int lockstep_slots(const LockstepHandle* h) { return h->engine.slots(); }
+void lockstep_destroy(LockstepHandle* h) { delete h; }
 
 }  // namespace rama

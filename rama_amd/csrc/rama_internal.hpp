@@ -185,6 +185,21 @@ struct PackedBatch {
     std::mutex* emu;
 };
 void solve_packed(Ctx& ctx, PackedBatch& B, int workers);
+// Receives each instance's outcome (from the lockstep thread or a worker).
+struct LockstepSink {
+    virtual ~LockstepSink() = default;
+    virtual void done(int id, int status, const std::string& error, const int32_t* labels, i64 n, double lb) = 0;
+};
+// The lockstep engine behind an opaque handle: admit groups of instances,
+// advance all admitted ones by one contraction round per call.
+struct LockstepHandle;
+constexpr int kLockstepMaxSlots = 4096;  // instances in one pack at a time
+LockstepHandle* lockstep_create(Ctx& ctx, const Options& o, int workers, LockstepSink& sink);
+void lockstep_admit(LockstepHandle* h, int k, const int64_t* edge_off, const int32_t* i, const int32_t* j,
+                    const float* c, const int* ids);
+void lockstep_round(LockstepHandle* h);
+int lockstep_slots(const LockstepHandle* h);
+void lockstep_destroy(LockstepHandle* h);  // waits for handed-off instances
 
 // ---- solver.hip
 struct SolveResult {

@@ -1,10 +1,13 @@
 """Batched solving and the serving loop: every instance's result in the mix
-is identical (labels and lower bound, bit for bit) to solving it alone,
-because each instance runs the unchanged plain solver on its own stream."""
+is identical (labels and lower bound, bit for bit) to solving it alone.  The
+lockstep path packs many instances into one graph but keeps each one's
+thresholds, stop rules and reduction order; the worker path runs the plain
+solver per instance on its own stream."""
 import numpy as np
 import pytest
 
 from rama_amd import instances as I
+from tests.instances_small import named
 
 pytestmark = pytest.mark.gpu
 
@@ -39,6 +42,45 @@ def test_server_mix_equals_alone(gpu):
             assert lat >= sms > 0
 
 
+def test_server_staggered_arrivals_equal_alone(gpu):
+    # requests arrive in waves while earlier ones are mid-solve; with one
+    # lane the first runs alone, the next queued ones form a pack and later
+    # waves join that pack between its rounds
+    import time
+
+    insts = mixed_instances(24, lo=500, hi=20000, seed=11)
+    opts = gpu.options("PD", verbose=0)
+    alone = _alone(gpu, insts, opts)
+    with gpu.Server(workers=1, opts=opts) as srv:
+        tickets = []
+        for w in range(4):
+            tickets += [srv.submit(*insts[q]) for q in range(6 * w, 6 * w + 6)]
+            time.sleep(0.003)
+        res = [srv.wait(t) for t in tickets]
+    _assert_same(alone, [r[:2] for r in res])
+
+
+@pytest.mark.parametrize("preset", ["P", "D"])
+def test_server_modes_equal_alone(gpu, preset):
+    insts = mixed_instances(10, seed=13)
+    opts = gpu.options(preset, verbose=0)
+    alone = _alone(gpu, insts, opts)
+    with gpu.Server(workers=2, opts=opts) as srv:
+        res = [srv.wait(t) for t in [srv.submit(*x) for x in insts]]
+    _assert_same(alone, [r[:2] for r in res])
+
+
+def test_server_worker_mode_equals_lockstep(gpu, monkeypatch):
+    insts = mixed_instances(8, seed=17)
+    opts = gpu.options("PD", verbose=0)
+    with gpu.Server(workers=3, opts=opts) as srv:
+        a = [srv.wait(t)[:2] for t in [srv.submit(*x) for x in insts]]
+    monkeypatch.setenv("RAMA_SERVE_MODE", "workers")
+    with gpu.Server(workers=3, opts=opts) as srv:
+        b = [srv.wait(t)[:2] for t in [srv.submit(*x) for x in insts]]
+    _assert_same(a, b)
+
+
 def test_server_budget_admits_one_at_a_time(gpu):
     # a 1-byte budget still runs every request (one in flight at a time)
     insts = mixed_instances(4, seed=9)
@@ -52,10 +94,19 @@ def test_server_budget_admits_one_at_a_time(gpu):
 
 
 def test_server_reports_errors(gpu):
-    with gpu.Server(workers=2, opts=gpu.options("PD", verbose=0)) as srv:
+    # a bad request fails alone; the ones around it are served
+    i, j, c = named("grid64")
+    opts = gpu.options("PD", verbose=0)
+    ref = gpu.solve(i, j, c, opts)
+    with gpu.Server(workers=2, opts=opts) as srv:
+        t0 = srv.submit(i, j, c)
         t = srv.submit([0, -5], [1, 2], [1.0, -1.0])
+        t1 = srv.submit(i, j, c)
         with pytest.raises(gpu.RamaError):
             srv.wait(t)
+        for tk in (t0, t1):
+            lab, lb, _, _ = srv.wait(tk)
+            assert lb == ref[1] and np.array_equal(lab, ref[0])
 
 
 # ---- the lockstep (packed) batch path: every instance of the batch shares one
