@@ -41,6 +41,10 @@ def parse():
     ap.add_argument("--H", type=int, default=1024)
     ap.add_argument("--W", type=int, default=2048)
     ap.add_argument("--preset", default="PD")
+    ap.add_argument("--workload", choices=["grid2d", "grid3d"], default="grid2d",
+                    help="grid2d: H x W grid with long-range offsets, 2000 seeds; "
+                         "grid3d: D^3 grid, 6-neighbourhood, 20000 seeds (run with --preset PD+)")
+    ap.add_argument("--D", type=int, default=192)
     ap.add_argument("--mode", choices=["plain", "batch", "serve", "copy"], default="plain",
                     help="plain: the default line; batch: 512 mixed instances batched vs one at a time; "
                          "serve: Poisson arrivals against the serving loop and the plain loop; "
@@ -212,7 +216,12 @@ def main():
 
     from rama_amd import _capi, instances, profiler
 
-    i, j, c, _ = instances.grid2d_voronoi(H=a.H, W=a.W, seed=0)
+    if a.workload == "grid3d":
+        i, j, c, _ = instances.grid3d_voronoi(D=a.D, seed=0)
+        workload = "grid3d_voronoi %d^3, 6-neighbourhood; 20000 seeds; %s" % (a.D, a.preset)
+    else:
+        i, j, c, _ = instances.grid2d_voronoi(H=a.H, W=a.W, seed=0)
+        workload = "grid2d_voronoi %dx%d + offsets (9,0),(0,9),(27,0),(0,27); 2000 seeds; %s" % (a.H, a.W, a.preset)
     E = len(c)
     dev = torch.device("cuda", 0)
     di = torch.from_numpy(i).to(dev)
@@ -258,11 +267,11 @@ def main():
         # HBM bytes per launch of the same kernel from the rocprofv3
         # FETCH_SIZE / WRITE_SIZE runs summarised under profiles/
         table = profiler.traffic_table(os.path.join(ROOT, "profiles", "traffic_pd_grid2d_1024x2048.md"))
-        if a.H == 1024 and a.W == 2048 and a.preset == "PD" and roof["kernel"] in table:
+        if a.workload == "grid2d" and a.H == 1024 and a.W == 2048 and a.preset == "PD" and roof["kernel"] in table:
             roof["traffic"] = table[roof["kernel"]]
         roof["measured_copy_peak"] = copy_peak(torch)
     out = {
-        "metric": "PD edges/s",
+        "metric": "%s edges/s" % a.preset,
         "value": E / dt,
         "unit": "edges/s",
         "n_gpus": 1,
@@ -274,7 +283,7 @@ def main():
         "vs_baseline": None,
         "dtype": "fp32",
         "data": "synthetic",
-        "config": {"workload": "grid2d_voronoi 1024x2048 + offsets (9,0),(0,9),(27,0),(0,27); 2000 seeds; PD",
+        "config": {"workload": workload,
                    "edges": E, "nodes": n_nodes, "mode": a.preset, "objective": obj, "lower_bound": lb,
                    "clusters": int(len(np.unique(lab)))},
         "roofline": roof,

@@ -45,8 +45,23 @@ struct Error : std::runtime_error {
         if (!(cond)) throw ::rama::Error(RAMA_ERR_INVALID, (msg));    \
     } while (0)
 
+// Host synchronisations counted per calling function and line when
+// RAMA_SYNC_SITES is set, printed at exit (diagnostic).
+bool sync_sites_on();
+void note_sync_site(const char* fn, int line);
+
+// Kernel launches made by this thread (diagnostic).
+inline int64_t& launch_count() {
+    static thread_local int64_t n = 0;
+    return n;
+}
+
 // Launch-error check after a kernel launch.
-#define RAMA_LAUNCHED() RAMA_HIP(hipGetLastError())
+#define RAMA_LAUNCHED()                  \
+    do {                                 \
+        ++::rama::launch_count();        \
+        RAMA_HIP(hipGetLastError());     \
+    } while (0)
 
 inline unsigned grid_for(i64 n, int block = 256) {
     i64 g = (n + block - 1) / block;
@@ -268,8 +283,9 @@ struct Ctx {
         *epoch = scan_epoch;
         return scan_status.get();
     }
-    void sync() {
+    void sync(const char* fn = __builtin_FUNCTION(), int line = __builtin_LINE()) {
         ++syncs;
+        if (sync_sites_on()) note_sync_site(fn, line);
         RAMA_HIP(hipStreamSynchronize(stream));
     }
 
@@ -311,31 +327,31 @@ struct Ctx {
 
     // Read `n` device scalars (contiguous) into host memory (synchronises).
     template <class T>
-    void read(const T* d, T* h, int n = 1) {
+    void read(const T* d, T* h, int n = 1, const char* fn = __builtin_FUNCTION(), int line = __builtin_LINE()) {
         RAMA_REQUIRE(sizeof(T) * (size_t)n <= pool::kPinnedBytes, "scalar read-back too large");
         RAMA_HIP(hipMemcpyAsync(pinned, d, sizeof(T) * n, hipMemcpyDeviceToHost, stream));
-        sync();
+        sync(fn, line);
         std::memcpy(h, pinned, sizeof(T) * n);
     }
     template <class T>
-    T read1(const T* d) {
+    T read1(const T* d, const char* fn = __builtin_FUNCTION(), int line = __builtin_LINE()) {
         T v;
-        read(d, &v, 1);
+        read(d, &v, 1, fn, line);
         return v;
     }
     template <class T>
-    void write1(T* d, T v) {
+    void write1(T* d, T v, const char* fn = __builtin_FUNCTION(), int line = __builtin_LINE()) {
         RAMA_HIP(hipMemcpyAsync(d, &v, sizeof(T), hipMemcpyHostToDevice, stream));
-        sync();
+        sync(fn, line);
     }
     template <class T>
     void h2d(T* d, const T* h, i64 n) {
         if (n > 0) RAMA_HIP(hipMemcpyAsync(d, h, sizeof(T) * (size_t)n, hipMemcpyHostToDevice, stream));
     }
     template <class T>
-    void d2h(T* h, const T* d, i64 n) {
+    void d2h(T* h, const T* d, i64 n, const char* fn = __builtin_FUNCTION(), int line = __builtin_LINE()) {
         if (n > 0) RAMA_HIP(hipMemcpyAsync(h, d, sizeof(T) * (size_t)n, hipMemcpyDeviceToHost, stream));
-        sync();
+        sync(fn, line);
     }
     template <class T>
     void d2d(T* dst, const T* src, i64 n) {

@@ -9,10 +9,11 @@ namespace rama {
 
 // dual_update_cycle_length (src/dual_solver.cu:7-35).  Returns the lower
 // bound; `num_tri` receives the raw triangle count of the search.
-static double dual_update_cycle_length(Ctx& ctx, DCoo& A, int c, int steps, float factor, float tol, i64* num_tri) {
+static double dual_update_cycle_length(Ctx& ctx, DCoo& A, int c, int steps, float factor, float tol, i64* num_tri,
+                                       bool want_lb) {
     Triangles tri = conflicted_cycles(ctx, A, c, factor, tol);
     *num_tri = tri.raw;
-    if (tri.raw == 0) return dcoo_lb(ctx, A.val.get(), A.nnz);
+    if (tri.raw == 0) return want_lb ? dcoo_lb(ctx, A.val.get(), A.nnz) : 0.0;
     MPState S;
     mp_build(ctx, A, tri, S);
     tri = Triangles();
@@ -36,14 +37,17 @@ static double dual_update_cycle_length(Ctx& ctx, DCoo& A, int c, int steps, floa
     return lb;
 }
 
-double dual_solver(Ctx& ctx, DCoo& A, int max_len, int num_iter, float factor, int num_outer, float tol) {
-    if (max_len < 3 || num_iter == 0 || num_outer == 0) return dcoo_lb(ctx, A.val.get(), A.nnz);
+double dual_solver(Ctx& ctx, DCoo& A, int max_len, int num_iter, float factor, int num_outer, float tol,
+                   bool want_lb) {
+    if (max_len < 3 || num_iter == 0 || num_outer == 0) return want_lb ? dcoo_lb(ctx, A.val.get(), A.nnz) : 0.0;
+    // the bound also drives the outer stall test when there are several passes
+    const bool need = want_lb || num_outer > 1;
     double fin = 0.0, prev = 0.0;
     i64 num_tri = 0;
     try {
         for (int o = 0; o < num_outer; ++o) {
             for (int c = 3; c <= max_len; ++c)
-                fin = dual_update_cycle_length(ctx, A, c, num_iter, factor, tol, &num_tri);
+                fin = dual_update_cycle_length(ctx, A, c, num_iter, factor, tol, &num_tri, need);
             if (num_tri == 0 || (o > 0 && (fin - prev) < 1e-3)) break;
             prev = fin;
         }

@@ -219,14 +219,29 @@ struct MPControl {
     int last_row;
 };
 
+// Thread t's share of a fixed-order sum: p[t] + p[t + kBlock] + ... in
+// increasing index, with eight loads in flight at a time (the additions keep
+// their order, so the result does not depend on the batching).
+__device__ __forceinline__ double strided_sum(const double* __restrict__ p, i64 n) {
+    double s = 0.0;
+    i64 k = threadIdx.x;
+    for (; k + 7 * kBlock < n; k += 8 * kBlock) {
+        double v[8];
+#pragma unroll
+        for (int u = 0; u < 8; ++u) v[u] = p[k + (i64)u * kBlock];
+#pragma unroll
+        for (int u = 0; u < 8; ++u) s += v[u];
+    }
+    for (; k < n; k += kBlock) s += p[k];
+    return s;
+}
+
 // After an iteration on a large grid: sum the lower-bound partials in the
 // same fixed order as the fused path and apply the stall test.
 __global__ __launch_bounds__(kBlock) void k_mp_finalize(const double* __restrict__ parts, i64 n, MPControl* ctl) {
     __shared__ double ws[kWaves];
     if (!ctl->active) return;
-    double s = 0.0;
-    for (i64 k = threadIdx.x; k < n; k += kBlock) s += parts[k];
-    const double lb = block_reduce(s, OpSum(), ws);
+    const double lb = block_reduce(strided_sum(parts, n), OpSum(), ws);
     if (threadIdx.x == 0) {
         ctl->prev = ctl->lb;
         ctl->lb = lb;
@@ -380,9 +395,7 @@ __global__ __launch_bounds__(kBlock) void k_mp_edges(i64 E, const int* __restric
     __syncthreads();
     if (!last) return;
     __threadfence();  // acquire: the other blocks' partials are visible to plain loads
-    double sum = 0.0;
-    for (i64 k = threadIdx.x; k < nparts; k += kBlock) sum += parts[k];
-    const double tot = block_reduce(sum, OpSum(), ws);
+    const double tot = block_reduce(strided_sum(parts, nparts), OpSum(), ws);
     if (threadIdx.x == 0) {
         ctl->prev = ctl->lb;
         ctl->lb = tot;
@@ -495,10 +508,7 @@ __global__ __launch_bounds__(kBlock) void k_mp_finalize_packed(MPPack pk, const 
     MPControl* c = ctl + q;
     if (!c->active) return;
     const i64 n = (i64)(pk.tblk[q + 1] - pk.tblk[q]) + (pk.eblk[q + 1] - pk.eblk[q]);
-    const double* p = parts + pk.pbeg[q];
-    double s = 0.0;
-    for (i64 k = threadIdx.x; k < n; k += kBlock) s += p[k];
-    const double lb = block_reduce(s, OpSum(), ws);
+    const double lb = block_reduce(strided_sum(parts + pk.pbeg[q], n), OpSum(), ws);
     if (threadIdx.x == 0) {
         c->prev = c->lb;
         c->lb = lb;

@@ -186,7 +186,15 @@ __global__ __launch_bounds__(kBlock) void k_radix_hist_all(const K* __restrict__
     __threadfence();  // acquire: the other blocks' rows are visible to plain loads
     for (int k = threadIdx.x; k < nb; k += kBlock) {
         int sum = 0;
-        for (unsigned b = 0; b < gridDim.x; ++b) sum += part[(i64)b * nb + k];
+        unsigned b = 0;
+        for (; b + 8 <= gridDim.x; b += 8) {
+            int v[8];  // eight loads in flight
+#pragma unroll
+            for (int u = 0; u < 8; ++u) v[u] = part[(i64)(b + u) * nb + k];
+#pragma unroll
+            for (int u = 0; u < 8; ++u) sum += v[u];
+        }
+        for (; b < gridDim.x; ++b) sum += part[(i64)b * nb + k];
         hist[k] = sum;
     }
     if (threadIdx.x == 0) *ticket = 0;
@@ -335,6 +343,114 @@ __global__ __launch_bounds__(kBlock) void k_onesweep(const K* __restrict__ kin, 
     }
 }
 
+// Arrays of at most kTsTile pairs: every pass in one block, in place.  The
+// pairs stay in registers (kTsItems per thread, in (wave, item, lane)
+// order); each pass ranks them stably by ballot multi-split, reorders them
+// through LDS and reloads them in the same order, so each pass is stable.
+// One launch replaces the histogram launch and one launch per pass.
+constexpr int kTsThreads = 1024;
+constexpr int kTsItems = 8;
+constexpr int kTsTile = kTsThreads * kTsItems;
+constexpr int kTsWaves = kTsThreads / 64;
+
+template <class K>
+__global__ __launch_bounds__(kTsThreads) void k_tile_sort(K* __restrict__ keys, u32* __restrict__ vals, int n,
+                                                          PassPlan plan) {
+    __shared__ K sk[kTsTile];
+    __shared__ u32 sv[kTsTile];
+    __shared__ int wcnt[kTsWaves][256];
+    __shared__ int dstart[256];
+    const int tid = threadIdx.x, lane = lane_id(), w = tid >> 6;
+    const int wbase = w * 64 * kTsItems;
+    K key[kTsItems];
+    u32 val[kTsItems];
+    int rnk[kTsItems];
+#pragma unroll
+    for (int k = 0; k < kTsItems; ++k) {
+        const int i = wbase + k * 64 + lane;
+        const bool ok = i < n;
+        key[k] = ok ? keys[i] : ~K(0);  // padding sorts after every real key (it starts after them)
+        val[k] = ok ? vals[i] : 0u;
+    }
+    const unsigned long long lt = (1ull << lane) - 1ull;
+    for (int p = 0; p < plan.passes; ++p) {
+        const int shift = plan.shift[p], bits = plan.width[p];
+        const u32 mask = (1u << bits) - 1u;
+        for (int k = tid; k < kTsWaves * 256; k += kTsThreads) (&wcnt[0][0])[k] = 0;
+        __syncthreads();
+#pragma unroll
+        for (int k = 0; k < kTsItems; ++k) {
+            const u32 d = (u32)(key[k] >> shift) & mask;
+            const unsigned long long peers = digit_peers(d, bits);
+            const int r = __popcll(peers & lt);
+            const int c = __popcll(peers);
+            const int before = wcnt[w][d];
+            __builtin_amdgcn_wave_barrier();
+            if (r == 0) wcnt[w][d] = before + c;
+            __builtin_amdgcn_wave_barrier();
+            rnk[k] = before + r;
+        }
+        __syncthreads();
+        if (tid < 256) {
+            // per digit: earlier waves' counts, and the digit's total
+            int tot = 0;
+#pragma unroll
+            for (int q = 0; q < kTsWaves; ++q) {
+                const int c = wcnt[q][tid];
+                wcnt[q][tid] = tot;
+                tot += c;
+            }
+            dstart[tid] = tot;
+        }
+        __syncthreads();
+        if (w == 0) {
+            // exclusive scan of the 256 digit totals, four per lane
+            int a[4], s = 0;
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+                a[q] = dstart[lane * 4 + q];
+                s += a[q];
+            }
+            int inc = s;
+#pragma unroll
+            for (int o = 1; o < 64; o <<= 1) {
+                const int t = __shfl_up(inc, o);
+                if (lane >= o) inc += t;
+            }
+            int run = inc - s;
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+                dstart[lane * 4 + q] = run;
+                run += a[q];
+            }
+        }
+        __syncthreads();
+#pragma unroll
+        for (int k = 0; k < kTsItems; ++k) {
+            const u32 d = (u32)(key[k] >> shift) & mask;
+            const int pos = dstart[d] + wcnt[w][d] + rnk[k];
+            sk[pos] = key[k];
+            sv[pos] = val[k];
+        }
+        __syncthreads();
+#pragma unroll
+        for (int k = 0; k < kTsItems; ++k) {
+            const int i = wbase + k * 64 + lane;
+            key[k] = sk[i];
+            val[k] = sv[i];
+        }
+        __syncthreads();
+    }
+#pragma unroll
+    for (int k = 0; k < kTsItems; ++k) {
+        const int i = wbase + k * 64 + lane;
+        if (i < n) {
+            keys[i] = key[k];
+            vals[i] = val[k];
+        }
+    }
+}
+
 // Sorts and returns whether the result ended in (tk, tv) instead of
 // (keys, vals) (odd number of passes).
 template <class K>
@@ -356,6 +472,12 @@ static bool radix_sort_core(Ctx& ctx, K* keys, u32* vals, K* tk, u32* tv, i64 n,
     K* kb = tk;
     u32* vb = tv;
     const double pair_bytes = (double)(sizeof(K) + sizeof(u32));
+    if (n <= kTsTile) {
+        prof::Scope ps(ctx, "radix_tile_sort", 2.0 * pair_bytes * n);
+        k_tile_sort<K><<<1, kTsThreads, 0, ctx.stream>>>(keys, vals, (int)n, plan);
+        RAMA_LAUNCHED();
+        return false;
+    }
     if (ntiles > kOnesweepMaxTiles) {
         // large arrays: three launches per pass (look-back chains would be long)
         DBuf<int> th = ctx.buf<int>((i64)256 * ntiles + 1);

@@ -2,12 +2,39 @@
 //   rama_profile_enable(on), rama_profile_reset(),
 //   rama_profile_count(), rama_profile_entry(k, name, launches, ms, bytes).
 #include <atomic>
+#include <cstdio>
+#include <map>
+#include <string>
 #include <cstdlib>
 #include <mutex>
 
 #include "profiler.hpp"
 
 namespace rama {
+
+namespace {
+const bool g_sync_sites = std::getenv("RAMA_SYNC_SITES") != nullptr;
+struct SyncSites {
+    std::mutex mu;
+    std::map<std::pair<std::string, int>, long long> n;
+    ~SyncSites() {
+        for (auto& kv : n)
+            std::fprintf(stderr, "sync %-40s line %5d  %lld\n", kv.first.first.c_str(), kv.first.second, kv.second);
+    }
+};
+SyncSites& sync_sites() {
+    static SyncSites s;
+    return s;
+}
+}  // namespace
+
+bool sync_sites_on() { return g_sync_sites; }
+void note_sync_site(const char* fn, int line) {
+    auto& s = sync_sites();
+    std::lock_guard<std::mutex> l(s.mu);
+    ++s.n[{fn, line}];
+}
+
 namespace prof {
 
 namespace {

@@ -155,8 +155,10 @@ double mp_run(Ctx& ctx, MPState& S, int steps);
 DCoo mp_reparametrized(Ctx& ctx, MPState& S);
 
 // ---- dual.hip
+// want_lb = false: the caller ignores the returned bound, so it is not
+// computed where nothing else needs it (0 is returned then).
 double dual_solver(Ctx& ctx, DCoo& A, int max_cycle_length, int num_iter, float tri_memory_factor, int num_outer,
-                   float tol_ratio);
+                   float tol_ratio, bool want_lb = true);
 
 // ---- forest.hip
 void connected_components(Ctx& ctx, const int* off, const int* adj, int n, int* labels);

@@ -83,9 +83,12 @@ void solve_rounds(Ctx& ctx, DCoo& A, SolveResult& out, const Options& o, int sta
     const auto t0 = out.t0;  // the time budget counts from the start of the solve
     const i64 N = out.num_labels;
     for (int iter = start_iter; A.nnz > 0; ++iter) {
+        const auto r0 = std::chrono::steady_clock::now();
+        const int64_t l0 = launch_count(), s0 = ctx.syncs;
+        const i64 e0 = A.nnz;
         st.begin(ctx);
         if (iter > 0)
-            dual_solver(ctx, A, o.max_cycle_length_primal, o.num_dual_itr_primal, 1.0f, 1, 1e-4f);
+            dual_solver(ctx, A, o.max_cycle_length_primal, o.num_dual_itr_primal, 1.0f, 1, 1e-4f, false);
         st.end(ctx, "dual_primal");
         st.begin(ctx);
         DBuf<int> cur;
@@ -93,7 +96,6 @@ void solve_rounds(Ctx& ctx, DCoo& A, SolveResult& out, const Options& o, int sta
         int n_contract = 0;
         int max_label = 0;
         const bool mm_branch = use_mm;
-        if (st.on) std::fprintf(stderr, "round %d: %lld edges, %s\n", iter, (long long)A.nnz, use_mm ? "matching" : "forest");
         if (use_mm) {
             // export_undirected: node count = largest endpoint + 1
             const int Un = A.max_node + 1;
@@ -122,7 +124,7 @@ void solve_rounds(Ctx& ctx, DCoo& A, SolveResult& out, const Options& o, int sta
         }
         bool bad = false;
         double diag = 0.0;
-        DCoo B = contract(ctx, A, cur.get(), max_label, &bad, &diag);
+        DCoo B = contract(ctx, A, cur.get(), max_label, &bad, o.verbose ? &diag : nullptr);
         if (o.verbose) {
             std::printf("original A size %dx%d\ncontracted A size %dx%d\nenergy reduction %g\n", A.cols, A.rows, B.cols,
                         B.rows, diag);
@@ -131,6 +133,11 @@ void solve_rounds(Ctx& ctx, DCoo& A, SolveResult& out, const Options& o, int sta
         A = std::move(B);
         map_node_labels(ctx, cur.get(), cur_size, out.labels.get(), N);
         st.end(ctx, "contract");
+        if (st.on)
+            std::fprintf(stderr, "round %d: %lld edges, %s, %.3f ms, %lld launches, %lld syncs\n", iter,
+                         (long long)e0, mm_branch ? "matching" : "forest",
+                         std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - r0).count(),
+                         (long long)(launch_count() - l0), (long long)(ctx.syncs - s0));
         out.iterations = iter + 1;
         if (o.dump_timeline) {
             std::vector<int> snap(N);

@@ -163,3 +163,35 @@ def test_full_size_properties(gpu, full_grid, preset):
     assert 1000 < k < 8000
     lab2, lb2, _, _ = gpu.solve(i, j, c, gpu.options(preset, verbose=0))
     assert lb2 == lb and np.array_equal(lab, lab2)
+
+
+def _full_size_module():
+    import importlib.util
+    import os
+
+    path = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "make_full_size.py")
+    spec = importlib.util.spec_from_file_location("make_full_size", path)
+    mod = importlib.util.module_from_spec(spec)
+    spec.loader.exec_module(mod)
+    return mod
+
+
+@pytest.mark.parametrize("name", ["grid2d_voronoi_1024x2048_PD", "grid3d_voronoi_192_PDplus"])
+def test_full_size_equals_oracle(gpu, name):
+    """The whole partition of each full-size workload equals the oracle's
+    (hash of the labels renumbered by first occurrence), with the same
+    objective; the bound agrees to 1e-9 relative (reduction order)."""
+    import json
+    import os
+
+    mk = _full_size_module()
+    with open(os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "full_size_oracle.json")) as f:
+        ref = json.load(f)[name]
+    gen, preset = mk.WORKLOADS[name]
+    i, j, c, _ = gen()
+    assert len(c) == ref["edges"]
+    lab, lb, _, _ = gpu.solve(i, j, c, gpu.options(preset, verbose=0))
+    assert len(np.unique(lab)) == ref["clusters"]
+    assert mk.partition_hash(lab) == ref["partition_sha256"]
+    assert mk.objective(lab, i, j, c) == ref["objective"]
+    assert abs(lb - ref["lower_bound"]) <= 1e-9 * abs(ref["lower_bound"])

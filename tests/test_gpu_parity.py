@@ -27,6 +27,23 @@ def test_dcoo_init(gpu, oracle, name):
     assert g[3] == o[3]
 
 
+@pytest.mark.parametrize("m", [2, 3, 100, 4095, 4097, 8191, 8192, 8193, 65536, 300000])
+def test_dcoo_init_sort_sizes(gpu, oracle, m):
+    # sizes on both sides of the one-block, onesweep and three-launch sort
+    # paths; repeated (i, j) pairs with distinct costs expose any unstable pass
+    rng = np.random.Generator(np.random.PCG64(m))
+    n = max(2, int(np.sqrt(m)) + 3)
+    i = rng.integers(0, n, size=m).astype(np.int32)
+    j = rng.integers(0, n, size=m).astype(np.int32)
+    j = np.where(i == j, (j + 1) % n, j).astype(np.int32)
+    c = rng.standard_normal(m).astype(np.float32)
+    g = gpu.dcoo_init(i, j, c)
+    o = oracle.dcoo_init(i, j, c)
+    for a, b in zip(g[:3], o[:3]):
+        assert np.array_equal(a, b)
+    assert g[3] == o[3]
+
+
 @pytest.mark.parametrize("name", NAMES)
 @pytest.mark.parametrize("groups", [2, 7, 50])
 def test_contract(gpu, oracle, name, groups):

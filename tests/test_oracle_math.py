@@ -191,3 +191,18 @@ def test_message_passing_bound_is_monotone_and_valid(oracle):
         _, _, _, lbs = oracle.message_passing(r, co, v, t1, t2, t3, 0, 8)
         assert np.all(np.diff(lbs) >= -1e-6)
         assert lbs[-1] <= brute_optimum(n, i, j, c) + 1e-5
+
+
+def test_partition_hash_is_relabel_invariant():
+    import importlib.util
+    import os
+
+    path = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "make_full_size.py")
+    spec = importlib.util.spec_from_file_location("make_full_size", path)
+    mk = importlib.util.module_from_spec(spec)
+    spec.loader.exec_module(mk)
+    a = np.array([5, 5, 2, 7, 2, 9], np.int32)
+    perm = np.array([3, 8, 1, 0, 4, 6, 2, 5, 7, 9])
+    assert mk.partition_hash(a) == mk.partition_hash(perm[a])
+    assert mk.partition_hash(a) != mk.partition_hash(np.array([5, 5, 2, 7, 7, 9]))
+    assert mk.partition_hash(a) != mk.partition_hash(np.array([5, 5, 2, 2, 2, 9]))
