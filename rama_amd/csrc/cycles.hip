@@ -41,7 +41,24 @@ struct CycArgs {
     const i64* slot_base = nullptr;
     const i64* slot_budget = nullptr;
     const i64* slot_out = nullptr;
+    // Counting in chunks of blocks (single instance): this launch's first
+    // block, and where the blocks add their attempted writes ([0]) and the
+    // end of the expansions they covered ([1], a maximum).
+    unsigned blk0 = 0;
+    i64* counted = nullptr;
 };
+
+// Adds the block's attempted writes to counted[0] and raises counted[1] to
+// the end of its expansions.  Block-uniform call.
+__device__ __forceinline__ void block_counted(i64* counted, i64 mine, i64 xend) {
+    __shared__ i64 ws[kWaves];
+    const i64 s = block_reduce(mine, OpSum(), ws);
+    const i64 m = block_reduce(xend, OpMax(), ws);
+    if (threadIdx.x == 0) {
+        if (s) atomicAdd(reinterpret_cast<unsigned long long*>(counted), (unsigned long long)s);
+        atomicMax(reinterpret_cast<unsigned long long*>(counted + 1), (unsigned long long)m);
+    }
+}
 
 // Adds v to base[key] for every lane with `on`, one atomic per distinct key
 // in the wave (lanes of a wave mostly share their instance).  Wave-uniform
@@ -150,6 +167,41 @@ __device__ __forceinline__ int count_vs_regs(const int* __restrict__ adj, int ia
     return c;
 }
 
+// count_vs_regs when the register list has no repeated value (rdup false):
+// then a streamed value counts on its first copy only, when some register
+// entry equals it.  The 16 comparisons per streamed value become lane
+// masks combined on the scalar unit, and the walk stops at the first value
+// above the register list's largest entry (rmax; both lists are sorted).
+// With repeated register entries it falls back to count_vs_regs.
+__device__ __forceinline__ int count_vs_regs_fast(const int* __restrict__ adj, int ia, int ie,
+                                                  const int (&rb)[kRegList], int rmax, bool rdup) {
+    if (rdup) return count_vs_regs(adj, ia, ie, rb);
+    const int lane = lane_id();
+    int c = 0, prev = -1;
+    for (int k = ia; k < ie; ++k) {
+        const int x = adj[k];
+        if (x > rmax) break;
+        unsigned long long hit = 0;
+#pragma unroll
+        for (int q = 0; q < kRegList; ++q) hit |= __ballot(rb[q] == x);
+        c += (int)((hit >> lane) & 1ull) & (int)(x != prev);
+        prev = x;
+    }
+    return c;
+}
+
+// Largest entry and whether a value repeats, for a register list of nb
+// sorted entries.
+__device__ __forceinline__ void reg_list_props(const int (&rb)[kRegList], int nb, int& rmax, bool& rdup) {
+    rmax = -1;
+    rdup = false;
+#pragma unroll
+    for (int q = 0; q < kRegList; ++q) {
+        if (q < nb) rmax = rb[q];
+        if (q > 0 && q < nb) rdup |= rb[q] == rb[q - 1];
+    }
+}
+
 // Adds `v` to the total of the tile holding expansion x, flushing the
 // running sum whenever the thread's expansions cross into the next tile.
 struct TileAcc {
@@ -184,7 +236,10 @@ __global__ __launch_bounds__(kBlock) void k_cycle_count3(CycArgs a, unsigned cha
         if (n2 <= kRegList) {
             int rb[kRegList];
             load_reg_list(a.adj, b2, n2, rb);
-            c = count_vs_regs(a.adj, a.off[v1], a.off[v1 + 1], rb);
+            int rmax;
+            bool rdup;
+            reg_list_props(rb, n2, rmax, rdup);
+            c = count_vs_regs_fast(a.adj, a.off[v1], a.off[v1 + 1], rb, rmax, rdup);
         } else {
             c = count_common(a.off, a.adj, v1, v2);
         }
@@ -200,9 +255,9 @@ __global__ __launch_bounds__(kBlock) void k_cycle_count3(CycArgs a, unsigned cha
 // (held in registers).  tile_tot must be zero on entry.
 __global__ __launch_bounds__(kBlock) void k_cycle_count4(CycArgs a, unsigned char* __restrict__ cnt8,
                                                          i64* __restrict__ tile_tot) {
-    const i64 r = (i64)xcd_block() * kBlock + threadIdx.x;
+    const i64 r = (i64)(a.blk0 + xcd_block()) * kBlock + threadIdx.x;
     const bool on = r < a.R;
-    i64 mine = 0;
+    i64 mine = 0, xend = 0;
     if (on) {
         const int v1 = a.rv1[r], v2 = a.rv2[r];
         const int b1 = a.off[v1], d1 = a.off[v1 + 1] - b1;
@@ -210,19 +265,26 @@ __global__ __launch_bounds__(kBlock) void k_cycle_count4(CycArgs a, unsigned cha
         const i64 x0 = a.xoff[r];
         int rb[kRegList];
         const bool regs = n2 <= kRegList;
-        if (regs) load_reg_list(a.adj, b2, n2, rb);
+        int rmax = -1;
+        bool rdup = false;
+        if (regs) {
+            load_reg_list(a.adj, b2, n2, rb);
+            reg_list_props(rb, n2, rmax, rdup);
+        }
         TileAcc acc;
         for (int l = 0; l < d1; ++l) {
             const int n1 = a.adj[b1 + l];
-            const int c =
-                regs ? count_vs_regs(a.adj, a.off[n1], a.off[n1 + 1], rb) : count_common(a.off, a.adj, n1, v2);
+            const int c = regs ? count_vs_regs_fast(a.adj, a.off[n1], a.off[n1 + 1], rb, rmax, rdup)
+                               : count_common(a.off, a.adj, n1, v2);
             cnt8[x0 + l] = (unsigned char)(c < 255 ? c : 255);
             acc.add(tile_tot, x0 + l, 2 * (i64)c);
             mine += 2 * (i64)c;
         }
         acc.flush(tile_tot);
+        xend = x0 + d1;
     }
     if (a.nslot) wave_add_keyed(a.slot_total, on ? a.nslot[a.rv1[r]] : 0, mine, on);
+    if (a.counted) block_counted(a.counted, mine, xend);
 }
 
 // 5-cycles: one thread per (repulsive edge r, n2 = l2-th neighbour of v2);
@@ -235,10 +297,10 @@ __global__ __launch_bounds__(kBlock) void k_cycle_count5(CycArgs a, const i64* _
                                                          const int* __restrict__ ytile_rep, i64 Y,
                                                          unsigned char* __restrict__ cnt8,
                                                          i64* __restrict__ tile_tot) {
-    const unsigned blk = xcd_block();
+    const unsigned blk = a.blk0 + xcd_block();
     const i64 y = (i64)blk * kCycTile + threadIdx.x;
     const bool on = y < Y;
-    i64 mine = 0;
+    i64 mine = 0, xend = 0;
     int slot = 0;
     if (on) {
     // rep owning thread y (galloping from the block's first rep)
@@ -261,21 +323,29 @@ __global__ __launch_bounds__(kBlock) void k_cycle_count5(CycArgs a, const i64* _
     const i64 x0 = a.xoff[r] + (i64)l2 * d1;
     int rb[kRegList];
     const bool regs = nq <= kRegList;
-    if (regs) load_reg_list(a.adj, bq, nq, rb);
+    int rmax = -1;
+    bool rdup = false;
+    if (regs) {
+        load_reg_list(a.adj, bq, nq, rb);
+        reg_list_props(rb, nq, rmax, rdup);
+    }
     TileAcc acc;
     for (int l = 0; l < d1; ++l) {
         const int n1 = a.adj[b1 + l];
         int c = 0;
         if (n1 != n2)
-            c = regs ? count_vs_regs(a.adj, a.off[n1], a.off[n1 + 1], rb) : count_common(a.off, a.adj, n1, n2);
+            c = regs ? count_vs_regs_fast(a.adj, a.off[n1], a.off[n1 + 1], rb, rmax, rdup)
+                     : count_common(a.off, a.adj, n1, n2);
         cnt8[x0 + l] = (unsigned char)(c < 255 ? c : 255);
         acc.add(tile_tot, x0 + l, 3 * (i64)c);
         mine += 3 * (i64)c;
     }
     acc.flush(tile_tot);
+    xend = x0 + d1;
     if (a.nslot) slot = a.nslot[v1];
     }
     if (a.nslot) wave_add_keyed(a.slot_total, slot, mine, on);
+    if (a.counted) block_counted(a.counted, mine, xend);
 }
 
 __device__ __forceinline__ void put_tri(int* t1, int* t2, int* t3, i64 g, i64 budget, int a, int b, int c) {
@@ -298,6 +368,7 @@ __global__ __launch_bounds__(kBlock) void k_cycle_emit(CycArgs a, const unsigned
     // working tiles spread over all XCDs balances them
     const unsigned blk = blockIdx.x;
     const bool packed = a.nslot != nullptr;
+    if (tile_off[blk + 1] == tile_off[blk]) return;  // uniform: the tile writes nothing
     const i64 tstart = (packed ? 0 : base) + tile_off[blk];
     if (!packed && tstart >= budget) return;  // uniform per block
     const i64 x = (i64)blk * kCycTile + threadIdx.x;
@@ -389,36 +460,84 @@ static i64 run_stage(Ctx& ctx, const Csr& G, const int* rv1, const int* rv2, i64
         prof::Scope ps(ctx, "cycles3_count", 0.0);
         k_cycle_count3<<<(unsigned)ntiles, kBlock, 0, ctx.stream>>>(a, cnt8.get(), tile.get());
         RAMA_LAUNCHED();
-    } else if (S == 4) {
+    }
+    // Stages 4 and 5 count in chunks of blocks, each a prefix of the
+    // expansion order, and stop once the attempted writes reach the budget
+    // left (budget - base): expansions past that point write nothing, so
+    // the triangles, the stage's outcome (total >= budget left) and every
+    // later decision are those of a full count.  Only the returned total is
+    // then a lower bound of the full one; verbose runs count everything, as
+    // they print it.
+    i64 tiles_used = ntiles;
+    if (S > 3) {
         ctx.zero(tile.get(), ntiles);
-        prof::Scope ps(ctx, "cycles4_count", 0.0);
-        k_cycle_count4<<<grid_for(R), kBlock, 0, ctx.stream>>>(a, cnt8.get(), tile.get());
-        RAMA_LAUNCHED();
-    } else {
-        // thread space: one thread per (rep edge, neighbour of v2)
-        DBuf<i64> yoff = ctx.buf<i64>(R + 1);
-        scan_exclusive<i64>(ctx, [=] __device__(i64 r) { const int u = rv2[r]; return (i64)(OFF[u + 1] - OFF[u]); },
-                            yoff.get(), R);
-        const i64 Y = ctx.read1(yoff.get() + R);
-        ctx.zero(tile.get(), ntiles);
-        if (Y > 0) {
-            const i64 ytiles = (Y + kCycTile - 1) / kCycTile;
-            RAMA_REQUIRE(ytiles < ((i64)1 << 31), "too many cycle expansions");
-            DBuf<int> ytile_rep = ctx.buf<int>(ytiles);
-            k_tile_rep<<<grid_for(R), kBlock, 0, ctx.stream>>>(yoff.get(), R, ytile_rep.get());
-            RAMA_LAUNCHED();
-            prof::Scope ps(ctx, "cycles5_count", 0.0);
-            k_cycle_count5<<<(unsigned)ytiles, kBlock, 0, ctx.stream>>>(a, yoff.get(), ytile_rep.get(), Y,
-                                                                        cnt8.get(), tile.get());
-            RAMA_LAUNCHED();
+        DBuf<i64> cnt = ctx.buf<i64>(2);
+        ctx.zero(cnt.get(), 2);
+        a.counted = cnt.get();
+        DBuf<i64> yoff;
+        DBuf<int> ytile_rep;
+        i64 Y = 0;
+        unsigned nblk = 0;
+        if (S == 4) {
+            nblk = grid_for(R);
+        } else {
+            // thread space: one thread per (rep edge, neighbour of v2)
+            yoff = ctx.buf<i64>(R + 1);
+            scan_exclusive<i64>(ctx,
+                                [=] __device__(i64 r) { const int u = rv2[r]; return (i64)(OFF[u + 1] - OFF[u]); },
+                                yoff.get(), R);
+            Y = ctx.read1(yoff.get() + R);
+            if (Y > 0) {
+                const i64 ytiles = (Y + kCycTile - 1) / kCycTile;
+                RAMA_REQUIRE(ytiles < ((i64)1 << 31), "too many cycle expansions");
+                ytile_rep = ctx.buf<int>(ytiles);
+                k_tile_rep<<<grid_for(R), kBlock, 0, ctx.stream>>>(yoff.get(), R, ytile_rep.get());
+                RAMA_LAUNCHED();
+                nblk = (unsigned)ytiles;
+            }
+        }
+        const bool early = !ctx.verbose && budget > base;
+        const i64 need = budget - base;
+        unsigned done = 0, chunk = early ? std::max<unsigned>(nblk / 64, 256u) : nblk;
+        i64 hc[2] = {0, X};
+        while (done < nblk) {
+            const unsigned n = std::min(chunk, nblk - done);
+            a.blk0 = done;
+            if (S == 4) {
+                prof::Scope ps(ctx, "cycles4_count", 0.0);
+                k_cycle_count4<<<n, kBlock, 0, ctx.stream>>>(a, cnt8.get(), tile.get());
+                RAMA_LAUNCHED();
+            } else {
+                prof::Scope ps(ctx, "cycles5_count", 0.0);
+                k_cycle_count5<<<n, kBlock, 0, ctx.stream>>>(a, yoff.get(), ytile_rep.get(), Y, cnt8.get(),
+                                                             tile.get());
+                RAMA_LAUNCHED();
+            }
+            done += n;
+            if (!early || done >= nblk) break;
+            ctx.read(cnt.get(), hc, 2);
+            if (hc[0] >= need) break;
+            // next chunk: 1.25x the blocks that the rate so far says reach
+            // the budget (all the rest when it says they will not)
+            const double per_block = (double)hc[0] / (double)done;
+            const double want = per_block > 0.0 ? 1.25 * (double)(need - hc[0]) / per_block + 64.0 : 1e30;
+            chunk = want >= (double)(nblk - done) ? nblk - done : (unsigned)want;
+        }
+        if (done < nblk) {
+            // counted expansions [0, x_end): emit the tiles they touch; the
+            // rest of the last tile contributes nothing
+            const i64 x_end = hc[1];
+            tiles_used = (x_end + kCycTile - 1) / kCycTile;
+            const i64 tail = std::min<i64>(tiles_used * kCycTile, X) - x_end;
+            if (tail > 0) ctx.zero(cnt8.get() + x_end, tail);
         }
     }
-    scan_exclusive_arr<i64>(ctx, tile.get(), tile.get(), ntiles);
-    const i64 W = ctx.read1(tile.get() + ntiles);
+    scan_exclusive_arr<i64>(ctx, tile.get(), tile.get(), tiles_used);
+    const i64 W = ctx.read1(tile.get() + tiles_used);
     if (W > 0 && base < budget) {
         prof::Scope ps(ctx, S == 3 ? "cycles3_emit" : (S == 4 ? "cycles4_emit" : "cycles5_emit"), 0.0);
-        k_cycle_emit<S><<<(unsigned)ntiles, kBlock, 0, ctx.stream>>>(a, cnt8.get(), tile.get(), base, budget,
-                                                                     T.t1.get(), T.t2.get(), T.t3.get());
+        k_cycle_emit<S><<<(unsigned)tiles_used, kBlock, 0, ctx.stream>>>(a, cnt8.get(), tile.get(), base, budget,
+                                                                         T.t1.get(), T.t2.get(), T.t3.get());
         RAMA_LAUNCHED();
     }
     return W;

@@ -9,10 +9,11 @@
 //   k_mp_triangles  per triangle: gather the three edge costs / counts, add
 //                   c/count to the triangle costs (send_messages_to_triplets),
 //                   run the six min-marginal steps, store the three messages
-//                   per slot and the triangle's lower-bound term;
-//   k_mp_edges      per edge: cost = sum of its slot messages in fixed order
-//                   (replaces float atomicAdd), plus the edge's
-//                   lower-bound term.
+//                   at their places in the edge-major message array and add
+//                   the triangle's lower-bound term;
+//   k_mp_edges      per edge: cost = sum of its run of messages, read
+//                   contiguously in (triangle, slot) order (replaces float
+//                   atomicAdd), plus the edge's lower-bound term.
 // Lower-bound partials are per block in double.  In the device-driven loop
 // the edge kernel's last block sums them and applies the stall test, so the
 // host reads the outcome once per loop.
@@ -139,6 +140,7 @@ static void mp_build_impl(Ctx& ctx, const DCoo& A, Triangles& tri, MPState& S, i
     S.t13 = ctx.buf<float>(T);
     S.t23 = ctx.buf<float>(T);
     S.delta = ctx.buf<float>(N3 > 0 ? N3 : 1);
+    S.pos = ctx.buf<u32>(N3 > 0 ? N3 : 1);
     int* EI = S.ei.get();
     int* EJ = S.ej.get();
     float* EC = S.ec.get();
@@ -177,15 +179,15 @@ static void mp_build_impl(Ctx& ctx, const DCoo& A, Triangles& tri, MPState& S, i
     float* TC12 = S.t12.get();
     float* TC13 = S.t13.get();
     float* TC23 = S.t23.get();
+    u32* POS = S.pos.get();
     for_each(ctx, N3, [=] __device__(i64 q) {
         const K k = KT[q];
         const int u = (int)(PA[q] + (Nc > 0 ? lower_bound_dev(CK, Nc, k) : 0));
         const u32 v = VT[q], t = v / 3, s = v % 3;
         (s == 0 ? C12 : (s == 1 ? C13 : C23))[t] = u;
         (s == 0 ? TC12 : (s == 1 ? TC13 : TC23))[t] = 0.0f;  // triangle costs start at zero
+        POS[v] = (u32)q;  // slot 3t+s sends to place q, inside its edge's run [SOFF[u], SOFF[u+1])
     });
-    // the sorted slot ids are the edges' slot lists
-    S.slots = std::move(vt);
 }
 
 void mp_build(Ctx& ctx, const DCoo& A, Triangles& tri, MPState& S) {
@@ -277,7 +279,8 @@ template <bool kSweep>
 __device__ __forceinline__ double mp_triangle(i64 t, const int* __restrict__ c12, const int* __restrict__ c13,
                                               const int* __restrict__ c23, const float* __restrict__ share,
                                               float* __restrict__ t12, float* __restrict__ t13,
-                                              float* __restrict__ t23, float* __restrict__ delta) {
+                                              float* __restrict__ t23, const u32* __restrict__ pos,
+                                              float* __restrict__ delta) {
     float a = t12[t], b = t13[t], d = t23[t];
     a += share[c12[t]];
     b += share[c13[t]];
@@ -316,9 +319,9 @@ __device__ __forceinline__ double mp_triangle(i64 t, const int* __restrict__ c12
             a -= mm;
             e12 += mm;
         }
-        delta[3 * t + 0] = e12;
-        delta[3 * t + 1] = e13;
-        delta[3 * t + 2] = e23;
+        delta[pos[3 * t + 0]] = e12;
+        delta[pos[3 * t + 1]] = e13;
+        delta[pos[3 * t + 2]] = e23;
     }
     t12[t] = a;
     t13[t] = b;
@@ -330,15 +333,15 @@ __device__ __forceinline__ double mp_triangle(i64 t, const int* __restrict__ c12
 // no triangle keep their cost); without, edges in a triangle are zeroed
 // (decrease_edge_costs_func, :231-239).  Returns its lower-bound term.
 template <bool kSweep>
-__device__ __forceinline__ double mp_edge(i64 e, const int* __restrict__ soff, const u32* __restrict__ slots,
-                                          const float* __restrict__ delta, float* __restrict__ ec,
+__device__ __forceinline__ double mp_edge(i64 e, const int* __restrict__ soff, const float* __restrict__ delta,
+                                          float* __restrict__ ec,
                                           float* __restrict__ share) {
     float c = ec[e];
     const int s0 = soff[e], s1 = soff[e + 1];
     if (s1 > s0) {
         if (kSweep) {
             double s = 0.0;
-            for (int k = s0; k < s1; ++k) s += (double)delta[slots[k]];
+            for (int k = s0; k < s1; ++k) s += (double)delta[k];
             c = (float)s;
         } else {
             c = 0.0f;
@@ -353,14 +356,14 @@ template <bool kSweep>
 __global__ __launch_bounds__(kBlock) void k_mp_triangles(i64 T, const int* __restrict__ c12, const int* __restrict__ c13,
                                                          const int* __restrict__ c23, const float* __restrict__ share,
                                                          float* __restrict__ t12, float* __restrict__ t13,
-                                                         float* __restrict__ t23, float* __restrict__ delta,
-                                                         double* __restrict__ lbpart,
+                                                         float* __restrict__ t23, const u32* __restrict__ pos,
+                                                         float* __restrict__ delta, double* __restrict__ lbpart,
                                                          const MPControl* __restrict__ ctl) {
     __shared__ double ws[kWaves];
     if (ctl && !ctl->active) return;  // uniform: the loop has stopped
     const unsigned blk = xcd_block();
     const i64 t = (i64)blk * kBlock + threadIdx.x;
-    const double lb = t < T ? mp_triangle<kSweep>(t, c12, c13, c23, share, t12, t13, t23, delta) : 0.0;
+    const double lb = t < T ? mp_triangle<kSweep>(t, c12, c13, c23, share, t12, t13, t23, pos, delta) : 0.0;
     const double r = block_reduce(lb, OpSum(), ws);
     if (threadIdx.x == 0) lbpart[blk] = r;
 }
@@ -374,7 +377,7 @@ __global__ __launch_bounds__(kBlock) void k_mp_triangles(i64 T, const int* __res
 // Each block's ticket carries a release, whose cost grows with the grid, so
 // large grids finalise in a separate one-block launch instead.
 template <bool kSweep>
-__global__ __launch_bounds__(kBlock) void k_mp_edges(i64 E, const int* __restrict__ soff, const u32* __restrict__ slots,
+__global__ __launch_bounds__(kBlock) void k_mp_edges(i64 E, const int* __restrict__ soff,
                                                      const float* __restrict__ delta, float* __restrict__ ec,
                                                      float* __restrict__ share, double* __restrict__ lbpart,
                                                      MPControl* ctl, const double* parts, i64 nparts,
@@ -384,7 +387,7 @@ __global__ __launch_bounds__(kBlock) void k_mp_edges(i64 E, const int* __restric
     if (ctl && !ctl->active) return;
     const unsigned blk = xcd_block();
     const i64 e = (i64)blk * kBlock + threadIdx.x;
-    const double lb = e < E ? mp_edge<kSweep>(e, soff, slots, delta, ec, share) : 0.0;
+    const double lb = e < E ? mp_edge<kSweep>(e, soff, delta, ec, share) : 0.0;
     const double r = block_reduce(lb, OpSum(), ws);
     if (threadIdx.x == 0) lbpart[blk] = r;
     if (!parts) return;
@@ -418,16 +421,16 @@ static void launch_round(Ctx& ctx, MPState& S, bool sweep, MPControl* ctl = null
     double* P = S.lbparts.get();
     {
         // per triangle: 3 edge ids, 3 share gathers, 3 costs in and out,
-        // 3 messages out = 60 B
-        prof::Scope ps(ctx, sweep ? "mp_triangles" : "mp_triangles_absorb", 60.0 * T);
+        // 3 message places, 3 messages out = 72 B
+        prof::Scope ps(ctx, sweep ? "mp_triangles" : "mp_triangles_absorb", (sweep ? 72.0 : 48.0) * T);
         if (sweep)
             k_mp_triangles<true><<<gt, kBlock, 0, ctx.stream>>>(T, S.c12.get(), S.c13.get(), S.c23.get(),
                                                                  S.share.get(), S.t12.get(), S.t13.get(),
-                                                                 S.t23.get(), S.delta.get(), P, ctl);
+                                                                 S.t23.get(), S.pos.get(), S.delta.get(), P, ctl);
         else
             k_mp_triangles<false><<<gt, kBlock, 0, ctx.stream>>>(T, S.c12.get(), S.c13.get(), S.c23.get(),
                                                                   S.share.get(), S.t12.get(), S.t13.get(),
-                                                                  S.t23.get(), S.delta.get(), P, ctl);
+                                                                  S.t23.get(), S.pos.get(), S.delta.get(), P, ctl);
         RAMA_LAUNCHED();
     }
     // the device-driven loop (ctl set) finalises each iteration in the edge
@@ -438,16 +441,14 @@ static void launch_round(Ctx& ctx, MPState& S, bool sweep, MPControl* ctl = null
     const i64 nparts = (i64)gt + (i64)ge;
     unsigned* ticket = fused ? ctx.reduce_ticket() : nullptr;
     {
-        // per edge: offsets, cost in/out, share out; per slot: slot id + message
-        prof::Scope ps(ctx, sweep ? "mp_edges" : "mp_edges_zero", 16.0 * E + 24.0 * T);
+        // per edge: offsets, cost in/out, share out; per slot: its message
+        prof::Scope ps(ctx, sweep ? "mp_edges" : "mp_edges_zero", 16.0 * E + (sweep ? 12.0 * T : 0.0));
         if (sweep)
-            k_mp_edges<true><<<ge, kBlock, 0, ctx.stream>>>(E, S.soff.get(), S.slots.get(), S.delta.get(),
-                                                             S.ec.get(), S.share.get(), P + gt, ctl, parts, nparts,
-                                                             ticket);
+            k_mp_edges<true><<<ge, kBlock, 0, ctx.stream>>>(E, S.soff.get(), S.delta.get(), S.ec.get(),
+                                                             S.share.get(), P + gt, ctl, parts, nparts, ticket);
         else
-            k_mp_edges<false><<<ge, kBlock, 0, ctx.stream>>>(E, S.soff.get(), S.slots.get(), S.delta.get(),
-                                                              S.ec.get(), S.share.get(), P + gt, ctl, parts, nparts,
-                                                              ticket);
+            k_mp_edges<false><<<ge, kBlock, 0, ctx.stream>>>(E, S.soff.get(), S.delta.get(), S.ec.get(),
+                                                              S.share.get(), P + gt, ctl, parts, nparts, ticket);
         RAMA_LAUNCHED();
     }
     if (ctl && !fused) {
@@ -471,7 +472,9 @@ __global__ __launch_bounds__(kBlock) void k_mp_triangles_packed(MPPack pk, const
                                                                 const int* __restrict__ c23,
                                                                 const float* __restrict__ share,
                                                                 float* __restrict__ t12, float* __restrict__ t13,
-                                                                float* __restrict__ t23, float* __restrict__ delta,
+                                                                float* __restrict__ t23,
+                                                                const u32* __restrict__ pos,
+                                                                float* __restrict__ delta,
                                                                 double* __restrict__ parts,
                                                                 const MPControl* __restrict__ ctl) {
     __shared__ double ws[kWaves];
@@ -479,14 +482,14 @@ __global__ __launch_bounds__(kBlock) void k_mp_triangles_packed(MPPack pk, const
     if (!ctl[q].active) return;  // uniform per block
     const int lb = (int)blockIdx.x - pk.tblk[q];
     const i64 t = pk.tbeg[q] + (i64)lb * kBlock + threadIdx.x;
-    const double v = t < pk.tbeg[q + 1] ? mp_triangle<kSweep>(t, c12, c13, c23, share, t12, t13, t23, delta) : 0.0;
+    const double v =
+        t < pk.tbeg[q + 1] ? mp_triangle<kSweep>(t, c12, c13, c23, share, t12, t13, t23, pos, delta) : 0.0;
     const double r = block_reduce(v, OpSum(), ws);
     if (threadIdx.x == 0) parts[pk.pbeg[q] + lb] = r;
 }
 
 template <bool kSweep>
 __global__ __launch_bounds__(kBlock) void k_mp_edges_packed(MPPack pk, const int* __restrict__ soff,
-                                                            const u32* __restrict__ slots,
                                                             const float* __restrict__ delta, float* __restrict__ ec,
                                                             float* __restrict__ share, double* __restrict__ parts,
                                                             const MPControl* __restrict__ ctl) {
@@ -495,7 +498,7 @@ __global__ __launch_bounds__(kBlock) void k_mp_edges_packed(MPPack pk, const int
     if (!ctl[q].active) return;
     const int lb = (int)blockIdx.x - pk.eblk[q];
     const i64 e = pk.ebeg[q] + (i64)lb * kBlock + threadIdx.x;
-    const double v = e < pk.ebeg[q + 1] ? mp_edge<kSweep>(e, soff, slots, delta, ec, share) : 0.0;
+    const double v = e < pk.ebeg[q + 1] ? mp_edge<kSweep>(e, soff, delta, ec, share) : 0.0;
     const double r = block_reduce(v, OpSum(), ws);
     if (threadIdx.x == 0) parts[pk.pbeg[q] + (pk.tblk[q + 1] - pk.tblk[q]) + lb] = r;
 }
@@ -554,10 +557,10 @@ void mp_run_packed(Ctx& ctx, MPState& S, const MPPack& pk, int total_tblocks, in
     for (int it = 0; it < steps; ++it) {
         k_mp_triangles_packed<true><<<(unsigned)total_tblocks, kBlock, 0, ctx.stream>>>(
             pk, S.c12.get(), S.c13.get(), S.c23.get(), S.share.get(), S.t12.get(), S.t13.get(), S.t23.get(),
-            S.delta.get(), parts.get(), ctl.get());
+            S.pos.get(), S.delta.get(), parts.get(), ctl.get());
         RAMA_LAUNCHED();
         k_mp_edges_packed<true><<<(unsigned)total_eblocks, kBlock, 0, ctx.stream>>>(
-            pk, S.soff.get(), S.slots.get(), S.delta.get(), S.ec.get(), S.share.get(), parts.get(), ctl.get());
+            pk, S.soff.get(), S.delta.get(), S.ec.get(), S.share.get(), parts.get(), ctl.get());
         RAMA_LAUNCHED();
         k_mp_finalize_packed<<<(unsigned)Q, kBlock, 0, ctx.stream>>>(pk, parts.get(), ctl.get());
         RAMA_LAUNCHED();

@@ -121,11 +121,11 @@ struct MPState {
     DBuf<float> ec;              // edge costs
     DBuf<float> share;           // ec / ecnt for edges in a triangle (what each triangle absorbs)
     DBuf<int> ecnt;              // #triangles per edge
-    DBuf<int> soff;              // edge -> slot list offsets (E+1)
-    DBuf<u32> slots;             // slot ids 3t+k, grouped by edge in (triangle, slot) order
+    DBuf<int> soff;              // edge -> its run of messages in delta (E+1)
+    DBuf<u32> pos;               // slot 3t+k -> its place in delta (runs by edge, (triangle, slot) order)
     DBuf<int> c12, c13, c23;     // triangle -> edge index
     DBuf<float> t12, t13, t23;   // triangle costs
-    DBuf<float> delta;           // per-slot messages (3T)
+    DBuf<float> delta;           // per-slot messages (3T), edge-major
     DBuf<double> lbparts;        // partial lower bounds
     DBuf<int> dims;              // [0] max col of (ei, ej), on device
     int max_col = -1, last_row = -1;  // host copies once read back (-1: not yet)
