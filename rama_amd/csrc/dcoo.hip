@@ -121,9 +121,10 @@ DCoo dcoo_copy(Ctx& ctx, const DCoo& A) {
     return B;
 }
 
-// Per-segment reduction of the sorted contraction keys: one thread per
-// (row, col) segment sums its costs in double in sorted order and rounds
-// once; self-loops are flagged (negative -> bad contraction) and dropped.
+// Per-segment reduction of the sorted contraction keys (packed instances,
+// which need the segments for their per-instance statistics): one thread
+// per (row, col) segment sums its costs in double in sorted order and
+// rounds once; self-loops are flagged (negative -> bad contraction).
 // The segment count S is read from the device; the grid covers the bound n.
 __global__ __launch_bounds__(kBlock) void k_contract_reduce(const u64* __restrict__ K, const u32* __restrict__ V,
                                                            const int* __restrict__ seg_start, const int* dS, i64 n,
@@ -161,28 +162,44 @@ __global__ __launch_bounds__(kBlock) void k_contract_reduce(const u64* __restric
     }
 }
 
-// Dimensions of a contraction in one launch: over the S = *dS segments,
-// out[1] = largest col, out[0] = largest col of a kept (off-diagonal)
-// segment, out[2] = row of the last segment.  Block maxima go to `part`; the
-// last block (atomic ticket, reset afterwards) combines them.
-__global__ __launch_bounds__(kBlock) void k_contract_dims(const u64* __restrict__ okey, const int* __restrict__ keep,
-                                                         const int* dS, int kb, int* part, unsigned* ticket,
-                                                         int* out) {
+// Statistics of the sorted contraction keys in one launch (grid-stride
+// over the n sorted edges): out[1] = largest col, out[0] = largest col of an
+// off-diagonal edge, out[2] = row of the last key; each self-loop run
+// (the diagonal) is summed in sorted order and rounded to float, a negative
+// one raises out[3] (bad contraction), and they are added to *diag.  Block results go to
+// `part`; the last block (atomic ticket, reset afterwards) combines them.
+__global__ __launch_bounds__(kBlock) void k_contract_stats(const u64* __restrict__ K, const u32* __restrict__ V,
+                                                          i64 n, int kb, int* part, unsigned* ticket, int* out,
+                                                          double* diag) {
     __shared__ int ws[kWaves];
+    __shared__ double wd[kWaves];
     __shared__ bool last;
-    const i64 S = *dS;
     const u64 mask = ((u64)1 << kb) - 1;
-    int all = -1, kept = -1;
-    for (i64 s = (i64)blockIdx.x * kBlock + threadIdx.x; s < S; s += (i64)gridDim.x * kBlock) {
-        const int c = (int)(okey[s] & mask);
+    int all = -1, kept = -1, bad = 0;
+    double dsum = 0.0;
+    for (i64 e = (i64)blockIdx.x * kBlock + threadIdx.x; e < n; e += (i64)gridDim.x * kBlock) {
+        const u64 k = K[e];
+        const int r = (int)(k >> kb), c = (int)(k & mask);
         all = c > all ? c : all;
-        if (keep[s] && c > kept) kept = c;
+        if (r != c) {
+            kept = c > kept ? c : kept;
+        } else if (e == 0 || K[e - 1] != k) {
+            double acc = 0.0;
+            for (i64 x = e; x < n && K[x] == k; ++x) acc += (double)__uint_as_float(V[x]);
+            const float f = (float)acc;  // the merged self-loop cost, as the edge would carry it
+            bad |= f < 0.0f ? 1 : 0;
+            dsum += (double)f;
+        }
     }
     all = block_reduce(all, OpMax(), ws);
     kept = block_reduce(kept, OpMax(), ws);
+    bad = block_reduce(bad, OpMax(), ws);
+    dsum = block_reduce(dsum, OpSum(), wd);
     if (threadIdx.x == 0) {
-        __hip_atomic_store(&part[2 * blockIdx.x], all, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        __hip_atomic_store(&part[2 * blockIdx.x + 1], kept, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        if (dsum != 0.0) atomicAdd(diag, dsum);
+        __hip_atomic_store(&part[3 * blockIdx.x], all, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        __hip_atomic_store(&part[3 * blockIdx.x + 1], kept, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        __hip_atomic_store(&part[3 * blockIdx.x + 2], bad, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         const unsigned t = __hip_atomic_fetch_add(ticket, 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
         last = t == gridDim.x - 1;
     }
@@ -191,18 +208,21 @@ __global__ __launch_bounds__(kBlock) void k_contract_dims(const u64* __restrict_
     __threadfence();  // acquire: the other blocks' partials are visible to plain loads
     all = -1;
     kept = -1;
+    bad = 0;
     for (unsigned b = threadIdx.x; b < gridDim.x; b += kBlock) {
-        const int a = part[2 * b];
-        const int k = part[2 * b + 1];
-        all = a > all ? a : all;
-        kept = k > kept ? k : kept;
+        const int x = part[3 * b], y = part[3 * b + 1], z = part[3 * b + 2];
+        all = x > all ? x : all;
+        kept = y > kept ? y : kept;
+        bad = z > bad ? z : bad;
     }
     all = block_reduce(all, OpMax(), ws);
     kept = block_reduce(kept, OpMax(), ws);
+    bad = block_reduce(bad, OpMax(), ws);
     if (threadIdx.x == 0) {
         out[1] = all;
         out[0] = kept;
-        out[2] = S > 0 ? (int)(okey[S - 1] >> kb) : -1;
+        out[2] = (int)(K[n - 1] >> kb);
+        out[3] = bad;
         *ticket = 0;
     }
 }
@@ -226,7 +246,7 @@ DCoo contract(Ctx& ctx, const DCoo& A, const int* node_map, int max_label, bool*
     const int* R = A.row.get();
     const int* C = A.col.get();
     const float* W = A.val.get();
-    // scalars on device: [0] S, [1] S2, [2] max kept col, [3] max col, [4] last row, [5] bad; diag as double
+    // scalars on device: [1] kept edges, [2] max kept col, [3] max col, [4] last row, [5] bad; diag as double
     DBuf<int> sc = ctx.buf<int>(8);
     DBuf<double> dg = ctx.buf<double>(1);
     int* SC = sc.get();
@@ -246,46 +266,42 @@ DCoo contract(Ctx& ctx, const DCoo& A, const int* node_map, int max_label, bool*
     K = keys.get();  // an odd pass count hands back the scratch buffers
     V = vals.get();
 
-    DBuf<int> seg_start = ctx.buf<int>(n);
-    int* SS = seg_start.get();
-    select_into(
-        ctx, n, [=] __device__(i64 e) { return e == 0 || K[e] != K[e - 1]; },
-        [=] __device__(i64 e, int pos) { SS[pos] = (int)e; }, SC + 0);
-    DBuf<u64> okey = ctx.buf<u64>(n);
-    DBuf<float> oval = ctx.buf<float>(n);
-    DBuf<int> keep = ctx.buf<int>(n);
-    {
-        // 12 B in per sorted edge; seg_start in + key, cost, keep flag out per segment (bounded by n)
-        prof::Scope ps(ctx, "contract_reduce", 32.0 * n);
-        k_contract_reduce<<<grid_for(n), kBlock, 0, ctx.stream>>>(K, V, SS, SC + 0, n, kb, okey.get(), oval.get(),
-                                                                    keep.get(), SC + 5, dg.get());
-        RAMA_LAUNCHED();
-    }
     const u64 mask = ((u64)1 << kb) - 1;
-    const u64* OK = okey.get();
-    const float* OV = oval.get();
-    const int* KP = keep.get();
     B.row = ctx.buf<int>(n);
     B.col = ctx.buf<int>(n);
     B.val = ctx.buf<float>(n);
     int* BR = B.row.get();
     int* BC = B.col.get();
     float* BW = B.val.get();
-    // drop the self-loops (remove_diagonal keeps rows_/cols_)
-    select_into(
-        ctx, n, [=] __device__(i64 s) { return s < SC[0] && KP[s] != 0; },
-        [=] __device__(i64 s, int q) {
-            BR[q] = (int)(OK[s] >> kb);
-            BC[q] = (int)(OK[s] & mask);
-            BW[q] = OV[s];
-        },
-        SC + 1);
+    {
+        // reduce_by_key + remove_diagonal in one pass: the head of each run
+        // of equal keys whose ends differ sums the run's costs in sorted
+        // order (double, rounded once) and writes the merged edge
+        prof::Site site2_("contract_merge");
+        select_into(
+            ctx, n,
+            [=] __device__(i64 e) {
+                const u64 k = K[e];
+                return (e == 0 || K[e - 1] != k) && (k >> kb) != (k & mask);
+            },
+            [=] __device__(i64 e, int q) {
+                const u64 k = K[e];
+                double acc = 0.0;
+                for (i64 x = e; x < n && K[x] == k; ++x) acc += (double)__uint_as_float(V[x]);
+                BR[q] = (int)(k >> kb);
+                BC[q] = (int)(k & mask);
+                BW[q] = (float)acc;
+            },
+            SC + 1);
+    }
     // dims as the reference computes them before removing the diagonal
-    // (src/dCOO.cu:91-92), and the largest endpoint among the kept edges
+    // (src/dCOO.cu:91-92), the largest endpoint among the kept edges, and
+    // the self-loop check
     {
         const unsigned nb = (unsigned)std::min<i64>(std::max<i64>((n + kBlock - 1) / kBlock, 1), kReduceMaxBlocks);
-        int* part = static_cast<int*>(ctx.reduce_scratch((size_t)kReduceMaxBlocks * 8));
-        k_contract_dims<<<nb, kBlock, 0, ctx.stream>>>(OK, KP, SC + 0, kb, part, ctx.reduce_ticket(), SC + 2);
+        int* part = static_cast<int*>(ctx.reduce_scratch((size_t)kReduceMaxBlocks * 12));
+        prof::Scope ps(ctx, "contract_stats", 12.0 * n);
+        k_contract_stats<<<nb, kBlock, 0, ctx.stream>>>(K, V, n, kb, part, ctx.reduce_ticket(), SC + 2, DG);
         RAMA_LAUNCHED();
     }
     int h[8];
