@@ -217,6 +217,9 @@ struct Ctx {
     Arena arena;
     bool verbose = false;
     void* pinned = nullptr;  // small pinned host area for scalar read-back
+    // The last conflicted-cycle search on this context found no cycle (a
+    // hint that the next dual pass leaves its graph unchanged).
+    bool last_cycles_empty = false;
 
     Ctx(int dev, hipStream_t s, bool verb) : device(dev), arena(dev), verbose(verb) {
         RAMA_HIP(hipSetDevice(dev));

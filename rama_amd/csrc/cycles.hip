@@ -755,8 +755,20 @@ Triangles conflicted_cycles(Ctx& ctx, const DCoo& A, int max_len, float factor, 
     const float thr = h.mn * tol;
     const i64 npos = (i64)(h.counts >> 32);
     const i64 R = (i64)(h.counts & 0xffffffffull);
-    if (npos == 0 || R == 0) return T;
-    Csr G = symmetric_csr(ctx, A.row.get(), A.col.get(), W, A.nnz, N, true, -1.0f * thr, npos);
+    if (npos == 0 || R == 0) {
+        ctx.last_cycles_empty = true;
+        return T;
+    }
+    Csr G;
+    if (ctx.last_cycles_empty) {
+        // the last search found nothing, so this dual pass likely leaves A
+        // as it is: build A's full symmetric CSR once (the matching reuses
+        // it) and take the attractive part out of it
+        if (!A.sym) A.sym = std::make_shared<Csr>(symmetric_csr(ctx, A.row.get(), A.col.get(), W, A.nnz, N, false, 0.0f));
+        G = csr_filter(ctx, *A.sym, -1.0f * thr, 2 * npos);
+    } else {
+        G = symmetric_csr(ctx, A.row.get(), A.col.get(), W, A.nnz, N, true, -1.0f * thr, npos);
+    }
     DBuf<int> rv1 = ctx.buf<int>(R), rv2 = ctx.buf<int>(R);
     {
         int* X1 = rv1.get();
@@ -790,6 +802,7 @@ Triangles conflicted_cycles(Ctx& ctx, const DCoo& A, int max_len, float factor, 
     T.raw = count < budget ? count : budget;
     T.n = rearrange_triangles(ctx, T, T.raw, N);
     T.max_id = N - 1;
+    ctx.last_cycles_empty = T.raw == 0;
     return T;
 }
 

@@ -177,18 +177,32 @@ __global__ __launch_bounds__(kBlock) void k_contract_stats(const u64* __restrict
     const u64 mask = ((u64)1 << kb) - 1;
     int all = -1, kept = -1, bad = 0;
     double dsum = 0.0;
-    for (i64 e = (i64)blockIdx.x * kBlock + threadIdx.x; e < n; e += (i64)gridDim.x * kBlock) {
-        const u64 k = K[e];
-        const int r = (int)(k >> kb), c = (int)(k & mask);
-        all = c > all ? c : all;
-        if (r != c) {
-            kept = c > kept ? c : kept;
-        } else if (e == 0 || K[e - 1] != k) {
-            double acc = 0.0;
-            for (i64 x = e; x < n && K[x] == k; ++x) acc += (double)__uint_as_float(V[x]);
-            const float f = (float)acc;  // the merged self-loop cost, as the edge would carry it
-            bad |= f < 0.0f ? 1 : 0;
-            dsum += (double)f;
+    const i64 stride = (i64)gridDim.x * kBlock;
+    for (i64 e0 = (i64)blockIdx.x * kBlock + threadIdx.x; e0 < n; e0 += 4 * stride) {
+        // four keys (and their predecessors) in flight per thread
+        u64 kk[4], pk[4];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            const i64 e = e0 + u * stride;
+            kk[u] = e < n ? K[e] : 0;
+            pk[u] = e < n && e > 0 ? K[e - 1] : ~(u64)0;
+        }
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            const i64 e = e0 + u * stride;
+            if (e >= n) break;
+            const u64 k = kk[u];
+            const int r = (int)(k >> kb), c = (int)(k & mask);
+            all = c > all ? c : all;
+            if (r != c) {
+                kept = c > kept ? c : kept;
+            } else if (pk[u] != k) {
+                double acc = 0.0;
+                for (i64 x = e; x < n && K[x] == k; ++x) acc += (double)__uint_as_float(V[x]);
+                const float f = (float)acc;  // the merged self-loop cost, as the edge would carry it
+                bad |= f < 0.0f ? 1 : 0;
+                dsum += (double)f;
+            }
         }
     }
     all = block_reduce(all, OpMax(), ws);
@@ -444,6 +458,42 @@ Csr symmetric_csr(Ctx& ctx, const int* row, const int* col, const float* val, i6
                   float keep_above, i64 known_count) {
     return symmetric_csr_impl(ctx, row, col, val, m, n_nodes, filter,
                               [=] __device__(i64 e) { return val[e] > keep_above; }, known_count);
+}
+
+// Stable filter of a CSR's entries: position of every kept entry by one
+// scan, then row offsets and kept entries in one launch.  Each row keeps its
+// order, so filtering the full symmetric CSR by cost gives exactly the
+// symmetric CSR of the edges that pass.
+Csr csr_filter(Ctx& ctx, const Csr& F, float keep_above, i64 kept) {
+    Csr G;
+    G.n = F.n;
+    G.nnz = kept;
+    G.off = ctx.buf<int>((i64)F.n + 1);
+    G.adj = ctx.buf<int>(kept > 0 ? kept : 1);
+    G.w = ctx.buf<float>(kept > 0 ? kept : 1);
+    if (F.nnz == 0) {
+        ctx.zero(G.off.get(), (i64)F.n + 1);
+        return G;
+    }
+    const i64 nnz = F.nnz;
+    const int n = F.n;
+    DBuf<int> pos = ctx.buf<int>(nnz + 1);
+    int* P = pos.get();
+    const int* FO = F.off.get();
+    const int* FA = F.adj.get();
+    const float* FW = F.w.get();
+    int* GO = G.off.get();
+    int* GA = G.adj.get();
+    float* GW = G.w.get();
+    scan_exclusive<int>(ctx, [=] __device__(i64 i) { return (int)(FW[i] > keep_above); }, P, nnz);
+    for_each(ctx, std::max<i64>(nnz, (i64)n + 1), [=] __device__(i64 i) {
+        if (i <= n) GO[i] = P[FO[i]];
+        if (i < nnz && FW[i] > keep_above) {
+            GA[P[i]] = FA[i];
+            GW[P[i]] = FW[i];
+        }
+    });
+    return G;
 }
 
 // Packed instances: edge e is kept when its cost exceeds the threshold of

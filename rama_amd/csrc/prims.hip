@@ -7,6 +7,8 @@
 // so each digit run is written out contiguously.  Items are ranked in
 // (wave, item, lane) order, which is their input order, so each pass is
 // stable and the whole sort is stable.
+#include <cstdlib>
+
 #include "prims.cuh"
 
 namespace rama {
@@ -478,7 +480,11 @@ static bool radix_sort_core(Ctx& ctx, K* keys, u32* vals, K* tk, u32* tv, i64 n,
         RAMA_LAUNCHED();
         return false;
     }
-    if (ntiles > kOnesweepMaxTiles) {
+    static const int onesweep_max = [] {
+        const char* e = std::getenv("RAMA_ONESWEEP_MAX_TILES");
+        return e ? std::atoi(e) : kOnesweepMaxTiles;
+    }();
+    if (ntiles > onesweep_max) {
         // large arrays: three launches per pass (look-back chains would be long)
         DBuf<int> th = ctx.buf<int>((i64)256 * ntiles + 1);
         for (int p = 0; p < plan.passes; ++p) {

@@ -4,10 +4,13 @@
 #pragma once
 
 #include <chrono>
+#include <memory>
 
 #include "prims.cuh"
 
 namespace rama {
+
+struct Csr;
 
 // Directed multicut graph in COO form, sorted by (row, col) with row < col —
 // the reference's dCOO (include/dCOO.h:13-71).  rows/cols follow the
@@ -23,6 +26,9 @@ struct DCoo {
     // have been removed
     int max_node = -1;
     int max_dim() const { return rows > cols ? rows : cols; }
+    // Symmetric CSR of exactly these arrays (every edge, max_dim() rows),
+    // when a stage built it for a later one to reuse.
+    mutable std::shared_ptr<Csr> sym;
 };
 
 // Symmetric adjacency in CSR form (both directions, sorted by (row, col)).
@@ -60,6 +66,8 @@ DCoo contract(Ctx& ctx, const DCoo& A, const int* node_map, int max_label, bool*
 // `known_count`: number of edges passing the filter if the caller knows it (-1: count here).
 Csr symmetric_csr(Ctx& ctx, const int* row, const int* col, const float* val, i64 m, int n_nodes,
                   bool filter, float keep_above, i64 known_count = -1);
+// The entries of F with weight > keep_above (kept of them), row order kept.
+Csr csr_filter(Ctx& ctx, const Csr& F, float keep_above, i64 kept);
 Csr symmetric_csr_slots(Ctx& ctx, const int* row, const int* col, const float* val, i64 m, int n_nodes,
                         const int* nslot, const float* keep_above, i64 known_count);
 void compute_offsets(Ctx& ctx, const int* sorted_ids, i64 n, int max_value, int* out);

@@ -99,7 +99,16 @@ void solve_rounds(Ctx& ctx, DCoo& A, SolveResult& out, const Options& o, int sta
         if (use_mm) {
             // export_undirected: node count = largest endpoint + 1
             const int Un = A.max_node + 1;
-            Csr U = symmetric_csr(ctx, A.row.get(), A.col.get(), A.val.get(), A.nnz, Un, false, 0.0f);
+            Csr U;
+            if (A.sym && A.sym->n >= Un) {
+                // built by this round's cycle search on the same arrays; rows
+                // from Un on are empty, so its first Un + 1 offsets are U's
+                U = std::move(*A.sym);
+                A.sym.reset();
+                U.n = Un;
+            } else {
+                U = symmetric_csr(ctx, A.row.get(), A.col.get(), A.val.get(), A.nnz, Un, false, 0.0f);
+            }
             DBuf<int> m;
             n_contract = matching_vertex_based(ctx, U, o.mean_multiplier_mm, m);
             cur = ctx.buf<int>(Un);
