@@ -141,6 +141,75 @@ __device__ __forceinline__ int count_common(const int* __restrict__ off, const i
     return c;
 }
 
+// Pairs whose two lists hold more than kHeavy entries together are
+// intersected by the whole wave instead of by their own lane: after
+// contraction a supernode can have thousands of neighbours, and one lane
+// walking two such lists would hold up its wave, its block and the launch.
+constexpr int kHeavy = 96;
+
+__device__ __forceinline__ int lower_in(const int* __restrict__ adj, int lo, int hi, int x) {
+    while (lo < hi) {
+        const int mid = (lo + hi) >> 1;
+        if (adj[mid] < x) lo = mid + 1; else hi = mid;
+    }
+    return lo;
+}
+
+// Whether entry i of the sorted list [sb, se) (value x) has a partner in the
+// sorted list [lb, le): its rank among the equal values of its own list is
+// below their count in the other, so a value held a and b times matches
+// min(a, b) times — the merge walk's rule.
+__device__ __forceinline__ bool has_partner(const int* __restrict__ adj, int sb, int i, int x, int lb, int le) {
+    const int rank = (i > sb && adj[i - 1] == x) ? i - lower_in(adj, sb, i, x) : 0;
+    const int p = lower_in(adj, lb, le, x);
+    if (p >= le || adj[p] != x) return false;
+    const int cnt = (p + 1 < le && adj[p + 1] == x) ? lower_in(adj, p, le, x + 1) - p : 1;
+    return rank < cnt;
+}
+
+// The shorter of the lists of qa and qb as [sb, se), the other as [lb, le).
+__device__ __forceinline__ void short_long(const int* __restrict__ off, int qa, int qb, int& sb, int& se, int& lb,
+                                           int& le) {
+    const int ab = off[qa], ae = off[qa + 1], bb = off[qb], be = off[qb + 1];
+    if (ae - ab <= be - bb) {
+        sb = ab, se = ae, lb = bb, le = be;
+    } else {
+        sb = bb, se = be, lb = ab, le = ae;
+    }
+}
+
+// Lane h's 64-bit value, as two 32-bit lane reads (h wave-uniform).
+__device__ __forceinline__ i64 readlane64(i64 v, int h) {
+    const unsigned lo = (unsigned)__builtin_amdgcn_readlane((int)(unsigned)(v & 0xffffffffll), h);
+    const unsigned hi = (unsigned)__builtin_amdgcn_readlane((int)(unsigned)((unsigned long long)v >> 32), h);
+    return (i64)(((unsigned long long)hi << 32) | lo);
+}
+
+// Lists at most this long are probed entry by entry into the other list by
+// their own lane (binary searches) when the pair is heavy; pairs of two
+// longer lists go to the whole wave.
+constexpr int kProbe = 16;
+
+// count_common by one lane from the shorter list [sb, se) into [lb, le).
+__device__ __forceinline__ int probe_count(const int* __restrict__ adj, int sb, int se, int lb, int le) {
+    int c = 0;
+    for (int i = sb; i < se; ++i) c += has_partner(adj, sb, i, adj[i], lb, le) ? 1 : 0;
+    return c;
+}
+
+// count_common by the whole wave (qa, qb wave-uniform): each lane takes
+// entries of the shorter list and binary-searches the longer one.
+__device__ __forceinline__ int wave_count_common(const int* __restrict__ off, const int* __restrict__ adj, int qa,
+                                                 int qb) {
+    int sb, se, lb, le;
+    short_long(off, qa, qb, sb, se, lb, le);
+    int c = 0;
+    for (int i = sb + lane_id(); i < se; i += 64) c += has_partner(adj, sb, i, adj[i], lb, le) ? 1 : 0;
+#pragma unroll
+    for (int d = 32; d >= 1; d >>= 1) c += __shfl_xor(c, d, 64);
+    return c;
+}
+
 // Register-list intersection: one sorted list (at most kRegList entries)
 // is held in registers, padded with -1, and the other is streamed from
 // memory with independent loads.  The k-th copy of a value in the streamed
@@ -230,10 +299,21 @@ __global__ __launch_bounds__(kBlock) void k_cycle_count3(CycArgs a, unsigned cha
     const unsigned blk = xcd_block();
     const i64 r = (i64)blk * kCycTile + threadIdx.x;
     int c = 0;
+    int v1 = 0, v2 = 0;
+    bool heavy = false;
     if (r < a.R) {
-        const int v1 = a.rv1[r], v2 = a.rv2[r];
+        v1 = a.rv1[r], v2 = a.rv2[r];
         const int b2 = a.off[v2], n2 = a.off[v2 + 1] - b2;
-        if (n2 <= kRegList) {
+        const int d1 = a.off[v1 + 1] - a.off[v1];
+        if (d1 + n2 > kHeavy) {
+            if (min(d1, n2) <= kProbe) {
+                int sb, se, lb, le;
+                short_long(a.off, v1, v2, sb, se, lb, le);
+                c = probe_count(a.adj, sb, se, lb, le);
+            } else {
+                heavy = true;  // counted by the whole wave below
+            }
+        } else if (n2 <= kRegList) {
             int rb[kRegList];
             load_reg_list(a.adj, b2, n2, rb);
             int rmax;
@@ -243,8 +323,14 @@ __global__ __launch_bounds__(kBlock) void k_cycle_count3(CycArgs a, unsigned cha
         } else {
             c = count_common(a.off, a.adj, v1, v2);
         }
-        cnt8[r] = (unsigned char)(c < 255 ? c : 255);
     }
+    for (unsigned long long hm = __ballot(heavy); hm; hm &= hm - 1) {
+        const int h = __builtin_ctzll(hm);
+        const int s = wave_count_common(a.off, a.adj, __builtin_amdgcn_readlane(v1, h),
+                                        __builtin_amdgcn_readlane(v2, h));
+        if (lane_id() == h) c = s;
+    }
+    if (r < a.R) cnt8[r] = (unsigned char)(c < 255 ? c : 255);
     if (a.nslot) wave_add_keyed(a.slot_total, r < a.R ? a.nslot[a.rv1[r]] : 0, (i64)c, r < a.R);
     const i64 tot = block_reduce((i64)c, OpSum(), ws);
     if (threadIdx.x == 0) tile_tot[blk] = tot;
@@ -374,27 +460,113 @@ __global__ __launch_bounds__(kBlock) void k_cycle_emit(CycArgs a, const unsigned
     const i64 x = (i64)blk * kCycTile + threadIdx.x;
     int c = 0;
     int v1 = 0, v2 = 0, n1 = 0, n2 = 0, qa = 0, qb = 0;
+    bool recount = false;
     if (x < a.X) {
         c = cnt8[x];
         if (c > 0) {
             resolve<S>(a, x, a.tile_rep[blk], v1, v2, n1, n2, qa, qb);
-            if (c == 255) c = count_common(a.off, a.adj, qa, qb);
+            if (c == 255) {
+                // the count saturated its byte: count again (by the wave
+                // when both lists are long)
+                int sb, se, lb, le;
+                short_long(a.off, qa, qb, sb, se, lb, le);
+                if (se - sb <= kProbe) c = probe_count(a.adj, sb, se, lb, le);
+                else recount = true;
+            }
         }
+    }
+    for (unsigned long long hm = __ballot(recount); hm; hm &= hm - 1) {
+        const int h = __builtin_ctzll(hm);
+        const int s = wave_count_common(a.off, a.adj, __builtin_amdgcn_readlane(qa, h),
+                                        __builtin_amdgcn_readlane(qb, h));
+        if (lane_id() == h) c = s;
     }
     i64 tot;
     i64 mypos = tstart + block_excl_scan((i64)c * (S - 2), ws, &tot);
-    if (c == 0) return;
-    if (packed) {
+    i64 tofs = 0;  // start of the instance's triangle region (packed)
+    if (packed && c > 0) {
         // instance q's own position, budget and region
         const int q = a.nslot[v1];
         mypos = a.slot_base[q] + (mypos - a.slot_first[q]);
         budget = a.slot_budget[q];
-        t1 += a.slot_out[q];
-        t2 += a.slot_out[q];
-        t3 += a.slot_out[q];
+        tofs = a.slot_out[q];
     }
-    if (mypos >= budget) return;
-    int i = a.off[qa], ie = a.off[qa + 1], j = a.off[qb], je = a.off[qb + 1];
+    const bool writes = c > 0 && mypos < budget;
+    const int i0 = writes ? a.off[qa] : 0, ie = writes ? a.off[qa + 1] : 0;
+    const int j0 = writes ? a.off[qb] : 0, je = writes ? a.off[qb + 1] : 0;
+    const bool big = writes && (ie - i0) + (je - j0) > kHeavy;
+    const bool heavy = big && min(ie - i0, je - j0) > kProbe;
+    if (big && !heavy) {
+        // one short list: probe its entries into the other, in order
+        int sb, se, lb, le;
+        short_long(a.off, qa, qb, sb, se, lb, le);
+        int* u1 = t1 + tofs;
+        int* u2 = t2 + tofs;
+        int* u3 = t3 + tofs;
+        i64 g = mypos;
+        for (int i = sb; i < se && g < budget; ++i) {
+            const int mid = a.adj[i];
+            if (!has_partner(a.adj, sb, i, mid, lb, le)) continue;
+            if (S == 3) {
+                put_tri(u1, u2, u3, g++, budget, v1, v2, mid);
+            } else if (S == 4) {
+                put_tri(u1, u2, u3, g++, budget, v1, v2, n1);
+                put_tri(u1, u2, u3, g++, budget, v2, n1, mid);
+            } else {
+                put_tri(u1, u2, u3, g++, budget, v1, v2, n1);
+                put_tri(u1, u2, u3, g++, budget, v2, n1, mid);
+                put_tri(u1, u2, u3, g++, budget, v2, mid, n2);
+            }
+        }
+    }
+    // heavy items: the whole wave walks the shorter list in order, finds each
+    // entry's partner by binary search, and places the k-th common neighbour
+    // at mypos + k * (S - 2) as the merge walk would
+    for (unsigned long long hm = __ballot(heavy); hm; hm &= hm - 1) {
+        const int h = __builtin_ctzll(hm);
+        const int hqa = __builtin_amdgcn_readlane(qa, h), hqb = __builtin_amdgcn_readlane(qb, h);
+        const int hv1 = __builtin_amdgcn_readlane(v1, h), hv2 = __builtin_amdgcn_readlane(v2, h);
+        const int hn1 = __builtin_amdgcn_readlane(n1, h), hn2 = __builtin_amdgcn_readlane(n2, h);
+        const i64 hpos = readlane64(mypos, h);
+        const i64 hbud = readlane64(budget, h);
+        const i64 hofs = readlane64(tofs, h);
+        int sb, se, lb, le;
+        short_long(a.off, hqa, hqb, sb, se, lb, le);
+        const unsigned long long below = (1ull << lane_id()) - 1ull;
+        i64 done = 0;
+        for (int base = sb; base < se && hpos + done * (S - 2) < hbud; base += 64) {
+            const int i = base + lane_id();
+            int mid = 0;
+            bool m = false;
+            if (i < se) {
+                mid = a.adj[i];
+                m = has_partner(a.adj, sb, i, mid, lb, le);
+            }
+            const unsigned long long bm = __ballot(m);
+            if (m) {
+                i64 g = hpos + (done + __popcll(bm & below)) * (S - 2);
+                int* u1 = t1 + hofs;
+                int* u2 = t2 + hofs;
+                int* u3 = t3 + hofs;
+                if (S == 3) {
+                    put_tri(u1, u2, u3, g, hbud, hv1, hv2, mid);
+                } else if (S == 4) {
+                    put_tri(u1, u2, u3, g, hbud, hv1, hv2, hn1);
+                    put_tri(u1, u2, u3, g + 1, hbud, hv2, hn1, mid);
+                } else {
+                    put_tri(u1, u2, u3, g, hbud, hv1, hv2, hn1);
+                    put_tri(u1, u2, u3, g + 1, hbud, hv2, hn1, mid);
+                    put_tri(u1, u2, u3, g + 2, hbud, hv2, mid, hn2);
+                }
+            }
+            done += __popcll(bm);
+        }
+    }
+    if (!writes || big) return;
+    t1 += tofs;
+    t2 += tofs;
+    t3 += tofs;
+    int i = i0, j = j0;
     i64 g = mypos;
     while (i < ie && j < je && g < budget) {
         const int p = a.adj[i], q = a.adj[j];

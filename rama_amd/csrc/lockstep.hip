@@ -135,7 +135,7 @@ std::vector<double> packed_dcoo_lb(Ctx& ctx, Pack& P, unsigned* tickets) {
     const float* W = P.A.val.get();
     seg_reduce<double>(
         ctx, [=] __device__(i64 e) { const float x = W[e]; return x < 0.0f ? (double)x : 0.0; }, eb.data(),
-        deb.get(), Q, 0.0, OpSum(), d.get(), 1, tickets);
+        deb.get(), Q, 0.0, OpSum(), d.get(), 1);
     ctx.d2h(out.data(), d.get(), Q);
     return out;
 }

@@ -324,9 +324,9 @@ void matching_packed(Ctx& ctx, const Csr& U, const int* nslot, int Q, const i64*
     constexpr int kStride = (int)(sizeof(MatchState) / sizeof(double));
     static_assert(sizeof(MatchState) % sizeof(double) == 0, "MatchState layout");
     seg_reduce<double>(ctx, [=] __device__(i64 e) { const float x = W[e]; return x >= 0.0f ? (double)x : 0.0; },
-                       hbeg, dbeg, Q, 0.0, OpSum(), &st->pos_sum, kStride, tickets);
+                       hbeg, dbeg, Q, 0.0, OpSum(), &st->pos_sum, kStride);
     seg_reduce<double>(ctx, [=] __device__(i64 e) { return W[e] >= 0.0f ? 1.0 : 0.0; }, hbeg, dbeg, Q, 0.0,
-                       OpSum(), &st->pos_cnt, kStride, tickets);
+                       OpSum(), &st->pos_cnt, kStride);
     std::vector<MatchState> h((size_t)Q);
     auto replay = [&](const MatchState& x, int rounds, int* count) {
         // the stop rule of filter_edges_by_matching_vertex_based (:121-127)

@@ -546,9 +546,9 @@ void mp_run_packed(Ctx& ctx, MPState& S, const MPPack& pk, int total_tblocks, in
     const float* B = S.t13.get();
     const float* D = S.t23.get();
     seg_reduce<double>(ctx, [=] __device__(i64 e) { const float c = EC[e]; return c < 0.0f ? (double)c : 0.0; },
-                       seg_ebeg_host, pk.ebeg, Q, 0.0, OpSum(), pe.get(), 1, tickets);
+                       seg_ebeg_host, pk.ebeg, Q, 0.0, OpSum(), pe.get(), 1);
     seg_reduce<double>(ctx, [=] __device__(i64 t) { return (double)tri_lb(A[t], B[t], D[t]); }, seg_tbeg_host,
-                       pk.tbeg, Q, 0.0, OpSum(), pe.get() + Q, 1, tickets);
+                       pk.tbeg, Q, 0.0, OpSum(), pe.get() + Q, 1);
     DBuf<MPControl> ctl = ctx.buf<MPControl>(Q);
     k_mp_init_packed<<<grid_for(Q), kBlock, 0, ctx.stream>>>(Q, pe.get(), pe.get() + Q, d_on, ctl.get());
     RAMA_LAUNCHED();

@@ -341,13 +341,15 @@ T reduce(Ctx& ctx, F f, i64 n, T ident, Op op) {
 
 // ------------------------------------------------ segmented reduction
 // Segment q of [beg[q], beg[q+1]) reduced exactly as reduce_to reduces an
-// array of that length on its own: the same number of blocks
-// (seg_blocks(n)), the same grid-stride element order, the same block tree
-// and the same combination of partials in block order by the segment's last
-// block.  So a result is bit-identical whether its instance is reduced alone
-// or packed with others.  bpre (Q+1) is the prefix of the segments' block
-// counts; tickets (Q) start at zero and are reset by each segment's last
-// block.
+// array of that length on its own: the same number of logical blocks
+// (seg_blocks(n)), the same grid-stride element order, the same block tree,
+// and the same combination of the partials in block order.  So a result is
+// bit-identical whether its instance is reduced alone or packed with
+// others.  Two launches and no atomics: a bounded grid of workgroups walks
+// the logical blocks of all segments (each finds its segment by a binary
+// search over the block-count prefix held in LDS) and stores their
+// partials; then one workgroup per segment combines that segment's
+// partials.  bpre (Q+1) is the prefix of the segments' block counts.
 inline unsigned seg_blocks(i64 n) {
     return (unsigned)std::min<i64>(std::max<i64>((n + kBlock - 1) / kBlock, 1), kReduceMaxBlocks);
 }
@@ -363,41 +365,48 @@ __device__ __forceinline__ int block_owner(const int* pre, int Q, int b) {
     return lo;
 }
 
+// Segments whose block-count prefix fits in LDS for the owner search.
+constexpr int kSegLdsMax = 4096;
+
 template <class T, class Op, class F>
-__global__ __launch_bounds__(kBlock) void k_seg_reduce(F f, const i64* __restrict__ beg, const int* __restrict__ bpre,
-                                                       int Q, T ident, Op op, T* partials, unsigned* tickets, T* dst,
-                                                       int stride) {
+__global__ __launch_bounds__(kBlock) void k_seg_partials(F f, const i64* __restrict__ beg,
+                                                         const int* __restrict__ bpre, int Q, int total, T ident,
+                                                         Op op, T* __restrict__ partials) {
     __shared__ T ws[kWaves];
-    __shared__ bool last;
-    const int q = block_owner(bpre, Q, (int)blockIdx.x);
-    const int nb = bpre[q + 1] - bpre[q];
-    const int lb = (int)blockIdx.x - bpre[q];
-    const i64 b0 = beg[q], n = beg[q + 1] - b0;
-    T acc = ident;
-    for (i64 i = (i64)lb * kBlock + threadIdx.x; i < n; i += (i64)nb * kBlock) acc = op(acc, (T)f(b0 + i));
-    const T r = block_reduce(acc, op, ws);
-    if (threadIdx.x == 0) {
-        __hip_atomic_store(&partials[bpre[q] + lb], r, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        const unsigned t = __hip_atomic_fetch_add(&tickets[q], 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
-        last = t == (unsigned)nb - 1;
-    }
+    __shared__ int spre[kSegLdsMax + 1];
+    const bool lds = Q <= kSegLdsMax;
+    if (lds)
+        for (int k = threadIdx.x; k <= Q; k += kBlock) spre[k] = bpre[k];
     __syncthreads();
-    if (!last) return;
-    __threadfence();  // acquire: the segment's partials are visible to plain loads
-    T a = ident;
-    for (int k = threadIdx.x; k < nb; k += kBlock) a = op(a, partials[bpre[q] + k]);
-    const T total = block_reduce(a, op, ws);
-    if (threadIdx.x == 0) {
-        dst[(i64)q * stride] = total;
-        tickets[q] = 0;
+    for (int L = blockIdx.x; L < total; L += gridDim.x) {  // uniform per workgroup
+        const int q = block_owner(lds ? spre : bpre, Q, L);
+        const int p0 = lds ? spre[q] : bpre[q];
+        const int nb = (lds ? spre[q + 1] : bpre[q + 1]) - p0;
+        const int lb = L - p0;
+        const i64 b0 = beg[q], n = beg[q + 1] - b0;
+        T acc = ident;
+        for (i64 i = (i64)lb * kBlock + threadIdx.x; i < n; i += (i64)nb * kBlock) acc = op(acc, (T)f(b0 + i));
+        const T r = block_reduce(acc, op, ws);
+        if (threadIdx.x == 0) partials[L] = r;
     }
+}
+
+template <class T, class Op>
+__global__ __launch_bounds__(kBlock) void k_seg_combine(const int* __restrict__ bpre, T ident, Op op,
+                                                        const T* __restrict__ partials, T* dst, int stride) {
+    __shared__ T ws[kWaves];
+    const int q = blockIdx.x;
+    const int p0 = bpre[q], nb = bpre[q + 1] - p0;
+    T a = ident;
+    for (int k = threadIdx.x; k < nb; k += kBlock) a = op(a, partials[p0 + k]);
+    const T total = block_reduce(a, op, ws);
+    if (threadIdx.x == 0) dst[(i64)q * stride] = total;
 }
 
 // Host side of the segmented reduction: hbeg/dbeg are the segment bounds
 // (Q+1) on the host and on the device; dst[q * stride] receives segment q.
 template <class T, class Op, class F>
-void seg_reduce(Ctx& ctx, F f, const i64* hbeg, const i64* dbeg, int Q, T ident, Op op, T* dst, int stride,
-                unsigned* tickets) {
+void seg_reduce(Ctx& ctx, F f, const i64* hbeg, const i64* dbeg, int Q, T ident, Op op, T* dst, int stride) {
     if (Q <= 0) return;
     static_assert(sizeof(T) <= 8, "reduction type");
     std::vector<int> bpre((size_t)Q + 1);
@@ -406,8 +415,11 @@ void seg_reduce(Ctx& ctx, F f, const i64* hbeg, const i64* dbeg, int Q, T ident,
     DBuf<int> dpre = ctx.buf<int>(Q + 1);
     ctx.upload(dpre.get(), bpre.data(), Q + 1);
     DBuf<T> part = ctx.buf<T>(bpre[Q]);
-    k_seg_reduce<T, Op, F><<<(unsigned)bpre[Q], kBlock, 0, ctx.stream>>>(f, dbeg, dpre.get(), Q, ident, op,
-                                                                          part.get(), tickets, dst, stride);
+    constexpr int kMaxGroups = 4096;  // workgroups walking the logical blocks
+    const unsigned g = (unsigned)std::min(bpre[Q], kMaxGroups);
+    k_seg_partials<T, Op, F><<<g, kBlock, 0, ctx.stream>>>(f, dbeg, dpre.get(), Q, bpre[Q], ident, op, part.get());
+    RAMA_LAUNCHED();
+    k_seg_combine<T, Op><<<(unsigned)Q, kBlock, 0, ctx.stream>>>(dpre.get(), ident, op, part.get(), dst, stride);
     RAMA_LAUNCHED();
 }
 

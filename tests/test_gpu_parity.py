@@ -77,7 +77,7 @@ def test_matching(gpu, oracle, name, mm):
     assert np.array_equal(gm, om)
 
 
-@pytest.mark.parametrize("name", NAMES)
+@pytest.mark.parametrize("name", NAMES + ["hubs"])
 @pytest.mark.parametrize("max_len", [3, 4, 5])
 @pytest.mark.parametrize("factor", [2.0, 0.3])
 def test_conflicted_cycles(gpu, oracle, name, max_len, factor):
@@ -180,7 +180,7 @@ def _same_partition(a, b):
     return len(pairs) == len(set(ia.tolist())) == len(set(ib.tolist()))
 
 
-@pytest.mark.parametrize("name", NAMES)
+@pytest.mark.parametrize("name", NAMES + ["hubs"])
 @pytest.mark.parametrize("preset", ["P", "PD", "PD+", "D"])
 def test_solve(gpu, oracle, name, preset):
     i, j, c = named(name)
