@@ -50,6 +50,8 @@ def parse():
                          "serve: Poisson arrivals against the serving loop and the plain loop; "
                          "copy: one 1 GiB device copy (calibrates HBM byte counters)")
     ap.add_argument("--count", type=int, default=512, help="instances in batch mode")
+    ap.add_argument("--batched-only", action="store_true",
+                    help="batch mode: run the batched solve alone (for profiling), no one-at-a-time loop")
     ap.add_argument("--workers", type=int, default=16)
     ap.add_argument("--requests", type=int, default=160, help="requests per offered load in serve mode")
     ap.add_argument("--kernels", type=int, default=8, help="profiled kernels listed on the line")
@@ -114,6 +116,15 @@ def batch_mode(a):
     E = int(sum(len(x[2]) for x in insts))
     opts = _capi.options("PD", verbose=0)
     _capi.solve_batch(insts[:4], opts, workers=1)  # warm up the runtime
+    if a.batched_only:
+        t = time.perf_counter()
+        bat = _capi.solve_batch(insts, opts, workers=a.workers)
+        t_bat = time.perf_counter() - t
+        return {"metric": "batched solves/s", "value": a.count / t_bat, "unit": "solves/s", "n_gpus": 1,
+                "higher_is_better": True, "dtype": "fp32", "data": "synthetic", "vs_baseline": None,
+                "config": {"workload": "%d instances, n log-uniform [1e3, 2e5], grids + Erdos-Renyi, PD" % a.count,
+                           "edges": E, "workers": a.workers},
+                "seconds": {"batched": t_bat}}
     for x in insts[:4]:
         _capi.solve(*x, opts)
     # the plain form, one instance after the other
