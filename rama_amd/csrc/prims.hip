@@ -155,6 +155,9 @@ struct PassPlan {
 // Arrays of at most this many tiles use the onesweep path (one launch per
 // pass); larger ones the three-launch pass.
 constexpr int kOnesweepMaxTiles = 64;
+// Arrays of at most this many tiles (and more than one block sort holds)
+// take the top-digit pass and the per-bucket block sorts.
+constexpr int kBucketMaxTiles = 64;
 
 // Digit histograms of every pass in one read of the keys.  Each block
 // stores its partial histograms (row blockIdx.x of `part`); the last block
@@ -356,99 +359,249 @@ constexpr int kTsTile = kTsThreads * kTsItems;
 constexpr int kTsWaves = kTsThreads / 64;
 
 template <class K>
-__global__ __launch_bounds__(kTsThreads) void k_tile_sort(K* __restrict__ keys, u32* __restrict__ vals, int n,
-                                                          PassPlan plan) {
-    __shared__ K sk[kTsTile];
-    __shared__ u32 sv[kTsTile];
-    __shared__ int wcnt[kTsWaves][256];
-    __shared__ int dstart[256];
+struct TsSmem {
+    K sk[kTsTile];
+    u32 sv[kTsTile];
+    int wcnt[kTsWaves][256];
+    int dstart[256];
+    int base[256];
+    int ph[8][256];  // per-pass digit histograms of a bucket too large for LDS
+    int misc[2];
+};
+
+// Stable ranks of the block's register-held items by the digit (shift,
+// bits): afterwards s.wcnt[w][d] counts the items of digit d in earlier
+// waves, s.dstart[d] the block's items of digit d, and rnk[k] is item k's
+// rank among its wave's items of the same digit.  Block-uniform call.
+template <class K>
+__device__ __forceinline__ void tile_rank(TsSmem<K>& s, const K (&key)[kTsItems], int shift, int bits,
+                                          int (&rnk)[kTsItems]) {
     const int tid = threadIdx.x, lane = lane_id(), w = tid >> 6;
-    const int wbase = w * 64 * kTsItems;
-    K key[kTsItems];
-    u32 val[kTsItems];
-    int rnk[kTsItems];
+    const u32 mask = (1u << bits) - 1u;
+    const unsigned long long lt = (1ull << lane) - 1ull;
+    for (int k = tid; k < kTsWaves * 256; k += kTsThreads) (&s.wcnt[0][0])[k] = 0;
+    __syncthreads();
 #pragma unroll
     for (int k = 0; k < kTsItems; ++k) {
-        const int i = wbase + k * 64 + lane;
-        const bool ok = i < n;
-        key[k] = ok ? keys[i] : ~K(0);  // padding sorts after every real key (it starts after them)
-        val[k] = ok ? vals[i] : 0u;
+        const u32 d = (u32)(key[k] >> shift) & mask;
+        const unsigned long long peers = digit_peers(d, bits);
+        const int r = __popcll(peers & lt);
+        const int c = __popcll(peers);
+        const int before = s.wcnt[w][d];
+        __builtin_amdgcn_wave_barrier();
+        if (r == 0) s.wcnt[w][d] = before + c;
+        __builtin_amdgcn_wave_barrier();
+        rnk[k] = before + r;
     }
-    const unsigned long long lt = (1ull << lane) - 1ull;
+    __syncthreads();
+    if (tid < 256) {
+        int tot = 0;
+#pragma unroll
+        for (int q = 0; q < kTsWaves; ++q) {
+            const int c = s.wcnt[q][tid];
+            s.wcnt[q][tid] = tot;
+            tot += c;
+        }
+        s.dstart[tid] = tot;
+    }
+    __syncthreads();
+}
+
+// In-place exclusive scan of 256 LDS counts by wave 0.  Block-uniform call.
+__device__ __forceinline__ void scan256(int* a) {
+    if (threadIdx.x < 64) {
+        const int lane = lane_id();
+        int v[4], s = 0;
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            v[q] = a[lane * 4 + q];
+            s += v[q];
+        }
+        int inc = s;
+#pragma unroll
+        for (int o = 1; o < 64; o <<= 1) {
+            const int t = __shfl_up(inc, o);
+            if (lane >= o) inc += t;
+        }
+        int run = inc - s;
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            a[lane * 4 + q] = run;
+            run += v[q];
+        }
+    }
+    __syncthreads();
+}
+
+// Every pass of `plan` over the block's register-held items, through LDS.
+template <class K>
+__device__ __forceinline__ void tile_sort_regs(TsSmem<K>& s, K (&key)[kTsItems], u32 (&val)[kTsItems],
+                                               const PassPlan& plan) {
+    const int lane = lane_id(), w = threadIdx.x >> 6;
+    const int wbase = w * 64 * kTsItems;
+    int rnk[kTsItems];
     for (int p = 0; p < plan.passes; ++p) {
         const int shift = plan.shift[p], bits = plan.width[p];
         const u32 mask = (1u << bits) - 1u;
-        for (int k = tid; k < kTsWaves * 256; k += kTsThreads) (&wcnt[0][0])[k] = 0;
-        __syncthreads();
+        tile_rank(s, key, shift, bits, rnk);
+        scan256(s.dstart);
 #pragma unroll
         for (int k = 0; k < kTsItems; ++k) {
             const u32 d = (u32)(key[k] >> shift) & mask;
-            const unsigned long long peers = digit_peers(d, bits);
-            const int r = __popcll(peers & lt);
-            const int c = __popcll(peers);
-            const int before = wcnt[w][d];
-            __builtin_amdgcn_wave_barrier();
-            if (r == 0) wcnt[w][d] = before + c;
-            __builtin_amdgcn_wave_barrier();
-            rnk[k] = before + r;
-        }
-        __syncthreads();
-        if (tid < 256) {
-            // per digit: earlier waves' counts, and the digit's total
-            int tot = 0;
-#pragma unroll
-            for (int q = 0; q < kTsWaves; ++q) {
-                const int c = wcnt[q][tid];
-                wcnt[q][tid] = tot;
-                tot += c;
-            }
-            dstart[tid] = tot;
-        }
-        __syncthreads();
-        if (w == 0) {
-            // exclusive scan of the 256 digit totals, four per lane
-            int a[4], s = 0;
-#pragma unroll
-            for (int q = 0; q < 4; ++q) {
-                a[q] = dstart[lane * 4 + q];
-                s += a[q];
-            }
-            int inc = s;
-#pragma unroll
-            for (int o = 1; o < 64; o <<= 1) {
-                const int t = __shfl_up(inc, o);
-                if (lane >= o) inc += t;
-            }
-            int run = inc - s;
-#pragma unroll
-            for (int q = 0; q < 4; ++q) {
-                dstart[lane * 4 + q] = run;
-                run += a[q];
-            }
-        }
-        __syncthreads();
-#pragma unroll
-        for (int k = 0; k < kTsItems; ++k) {
-            const u32 d = (u32)(key[k] >> shift) & mask;
-            const int pos = dstart[d] + wcnt[w][d] + rnk[k];
-            sk[pos] = key[k];
-            sv[pos] = val[k];
+            const int pos = s.dstart[d] + s.wcnt[w][d] + rnk[k];
+            s.sk[pos] = key[k];
+            s.sv[pos] = val[k];
         }
         __syncthreads();
 #pragma unroll
         for (int k = 0; k < kTsItems; ++k) {
             const int i = wbase + k * 64 + lane;
-            key[k] = sk[i];
-            val[k] = sv[i];
+            key[k] = s.sk[i];
+            val[k] = s.sv[i];
         }
         __syncthreads();
     }
+}
+
+// Loads up to kTsTile pairs from (ki, vi) [0, n) into registers in
+// (wave, item, lane) order; the padding sorts after every real key (it
+// starts after them and has the largest digit in every pass).
+template <class K>
+__device__ __forceinline__ void tile_load(const K* __restrict__ ki, const u32* __restrict__ vi, int n,
+                                          K (&key)[kTsItems], u32 (&val)[kTsItems]) {
+    const int lane = lane_id(), wbase = (threadIdx.x >> 6) * 64 * kTsItems;
+#pragma unroll
+    for (int k = 0; k < kTsItems; ++k) {
+        const int i = wbase + k * 64 + lane;
+        const bool ok = i < n;
+        key[k] = ok ? ki[i] : ~K(0);
+        val[k] = ok ? vi[i] : 0u;
+    }
+}
+
+template <class K>
+__device__ __forceinline__ void tile_store(K* __restrict__ ko, u32* __restrict__ vo, int n, const K (&key)[kTsItems],
+                                           const u32 (&val)[kTsItems]) {
+    const int lane = lane_id(), wbase = (threadIdx.x >> 6) * 64 * kTsItems;
 #pragma unroll
     for (int k = 0; k < kTsItems; ++k) {
         const int i = wbase + k * 64 + lane;
         if (i < n) {
-            keys[i] = key[k];
-            vals[i] = val[k];
+            ko[i] = key[k];
+            vo[i] = val[k];
+        }
+    }
+}
+
+template <class K>
+__global__ __launch_bounds__(kTsThreads) void k_tile_sort(K* __restrict__ keys, u32* __restrict__ vals, int n,
+                                                          PassPlan plan) {
+    __shared__ TsSmem<K> s;
+    K key[kTsItems];
+    u32 val[kTsItems];
+    tile_load(keys, vals, n, key, val);
+    tile_sort_regs(s, key, val, plan);
+    tile_store(keys, vals, n, key, val);
+}
+
+// Second stage of the top-digit sort.  An LSD pass on the top digit has
+// grouped the pairs into 256 buckets (kin/vin, stable); block b sorts bucket
+// b stably by the remaining low bits (`plan`) and writes it to the same
+// range of kout/vout.  Bucket bounds: with stride 0, `bnd` holds the 256
+// bucket sizes; otherwise bucket d starts at bnd[d * stride] (a scanned
+// digit-major tile table) and the last one ends at n.  A bucket that fits in
+// LDS is sorted as a tile.  A larger one is sorted by the block alone: the
+// histograms of every pass in one sweep, then per pass its tiles in order,
+// each ranked in LDS and scattered after the earlier tiles' items of each
+// digit, ping-ponging between the bucket's two ranges.
+template <class K>
+__global__ __launch_bounds__(kTsThreads) void k_bucket_sort(K* __restrict__ kin, u32* __restrict__ vin,
+                                                            K* __restrict__ kout, u32* __restrict__ vout,
+                                                            const int* __restrict__ bnd, int stride, int n,
+                                                            PassPlan plan) {
+    __shared__ TsSmem<K> s;
+    const int tid = threadIdx.x, b = blockIdx.x;
+    if (stride == 0) {
+        if (tid < 64) {
+            // start of bucket b: the sizes of the buckets before it
+            int acc = 0;
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+                const int d = tid * 4 + q;
+                acc += d < b ? bnd[d] : 0;
+            }
+#pragma unroll
+            for (int o = 32; o >= 1; o >>= 1) acc += __shfl_xor(acc, o, 64);
+            if (tid == 0) {
+                s.misc[0] = acc;
+                s.misc[1] = bnd[b];
+            }
+        }
+    } else if (tid == 0) {
+        const int b0 = bnd[(i64)b * stride];
+        s.misc[0] = b0;
+        s.misc[1] = (b < 255 ? bnd[(i64)(b + 1) * stride] : n) - b0;
+    }
+    __syncthreads();
+    const int beg = s.misc[0], m = s.misc[1];
+    if (m == 0) return;  // uniform
+    K* ka = kin + beg;
+    u32* va = vin + beg;
+    K* kb = kout + beg;
+    u32* vb = vout + beg;
+    K key[kTsItems];
+    u32 val[kTsItems];
+    if (m <= kTsTile) {
+        tile_load(ka, va, m, key, val);
+        tile_sort_regs(s, key, val, plan);
+        tile_store(kb, vb, m, key, val);
+        return;
+    }
+    const int lane = lane_id(), w = tid >> 6, wbase = w * 64 * kTsItems;
+    int rnk[kTsItems];
+    // digit counts do not depend on the order: every pass's histogram in one sweep
+    for (int k = tid; k < 8 * 256; k += kTsThreads) (&s.ph[0][0])[k] = 0;
+    __syncthreads();
+    for (int i = tid; i < m; i += kTsThreads) {
+        const K x = ka[i];
+        for (int p = 0; p < plan.passes; ++p)
+            atomicAdd(&s.ph[p][(u32)(x >> plan.shift[p]) & ((1u << plan.width[p]) - 1u)], 1);
+    }
+    __syncthreads();
+    for (int p = 0; p < plan.passes; ++p) {
+        const int shift = plan.shift[p], bits = plan.width[p];
+        const u32 mask = (1u << bits) - 1u;
+        K* ks = (p & 1) ? kb : ka;
+        u32* vs = (p & 1) ? vb : va;
+        K* kd = (p & 1) ? ka : kb;
+        u32* vd = (p & 1) ? va : vb;
+        if (tid < 256) s.base[tid] = s.ph[p][tid];
+        __syncthreads();
+        scan256(s.base);
+        for (int t0 = 0; t0 < m; t0 += kTsTile) {  // uniform
+            const int nt = min(kTsTile, m - t0);
+            tile_load(ks + t0, vs + t0, nt, key, val);
+            tile_rank(s, key, shift, bits, rnk);
+#pragma unroll
+            for (int k = 0; k < kTsItems; ++k) {
+                if (wbase + k * 64 + lane < nt) {
+                    const u32 d = (u32)(key[k] >> shift) & mask;
+                    const int o = s.base[d] + s.wcnt[w][d] + rnk[k];
+                    kd[o] = key[k];
+                    vd[o] = val[k];
+                }
+            }
+            __syncthreads();
+            // the padding (kTsTile - nt items) all carry the largest digit
+            if (tid < 256) s.base[tid] += s.dstart[tid] - (tid == (int)mask ? kTsTile - nt : 0);
+            __syncthreads();
+        }
+    }
+    if ((plan.passes & 1) == 0) {
+        // an even pass count ends in the input range: copy to the output
+        for (int i = tid; i < m; i += kTsThreads) {
+            kb[i] = ka[i];
+            vb[i] = va[i];
         }
     }
 }
@@ -478,6 +631,44 @@ static bool radix_sort_core(Ctx& ctx, K* keys, u32* vals, K* tk, u32* tv, i64 n,
         prof::Scope ps(ctx, "radix_tile_sort", 2.0 * pair_bytes * n);
         k_tile_sort<K><<<1, kTsThreads, 0, ctx.stream>>>(keys, vals, (int)n, plan);
         RAMA_LAUNCHED();
+        return false;
+    }
+    if (ntiles <= kBucketMaxTiles && bits > 8) {
+        // mid-size arrays: one pass on the top 8 bits into 256 buckets
+        // (histogram + onesweep), then each bucket's low bits in one block
+        PassPlan top{}, low{};
+        top.passes = 1;
+        top.shift[0] = bits - 8;
+        top.width[0] = 8;
+        const int lb = bits - 8;
+        low.passes = (lb + 7) / 8;
+        const int bw = lb / low.passes, ex = lb % low.passes;
+        for (int p = 0, sh = 0; p < low.passes; ++p) {
+            low.width[p] = bw + (p < ex ? 1 : 0);
+            low.shift[p] = sh;
+            sh += low.width[p];
+        }
+        const unsigned hb = (unsigned)std::min<i64>(ntiles, 1024);
+        DBuf<int> hist = ctx.buf<int>((i64)(hb + 1) * 256);
+        int* hsum = hist.get() + (i64)hb * 256;
+        {
+            prof::Scope ps(ctx, "radix_hist_all", (double)sizeof(K) * n);
+            k_radix_hist_all<K><<<hb, kBlock, 0, ctx.stream>>>(keys, n, top, hist.get(), hsum, ctx.reduce_ticket());
+            RAMA_LAUNCHED();
+        }
+        u32 epoch;
+        u64* st = ctx.scan_begin((i64)ntiles * 256, &epoch);
+        {
+            prof::Scope ps(ctx, "radix_onesweep", 2.0 * pair_bytes * n);
+            k_onesweep<K><<<ntiles, kBlock, 0, ctx.stream>>>(keys, vals, tk, tv, n, top.shift[0], 8, hsum, st,
+                                                             ctx.scan_counter.get(), epoch);
+            RAMA_LAUNCHED();
+        }
+        {
+            prof::Scope ps(ctx, "radix_bucket_sort", 2.0 * pair_bytes * n);
+            k_bucket_sort<K><<<256, kTsThreads, 0, ctx.stream>>>(tk, tv, keys, vals, hsum, 0, (int)n, low);
+            RAMA_LAUNCHED();
+        }
         return false;
     }
     static const int onesweep_max = [] {

@@ -27,7 +27,8 @@ def test_dcoo_init(gpu, oracle, name):
     assert g[3] == o[3]
 
 
-@pytest.mark.parametrize("m", [2, 3, 100, 4095, 4097, 8191, 8192, 8193, 65536, 300000])
+@pytest.mark.parametrize("m", [2, 3, 100, 4095, 4097, 8191, 8192, 8193, 20000, 65536, 150000, 262144, 262145,
+                               300000])
 def test_dcoo_init_sort_sizes(gpu, oracle, m):
     # sizes on both sides of the one-block, onesweep and three-launch sort
     # paths; repeated (i, j) pairs with distinct costs expose any unstable pass
@@ -35,6 +36,25 @@ def test_dcoo_init_sort_sizes(gpu, oracle, m):
     n = max(2, int(np.sqrt(m)) + 3)
     i = rng.integers(0, n, size=m).astype(np.int32)
     j = rng.integers(0, n, size=m).astype(np.int32)
+    j = np.where(i == j, (j + 1) % n, j).astype(np.int32)
+    c = rng.standard_normal(m).astype(np.float32)
+    g = gpu.dcoo_init(i, j, c)
+    o = oracle.dcoo_init(i, j, c)
+    for a, b in zip(g[:3], o[:3]):
+        assert np.array_equal(a, b)
+    assert g[3] == o[3]
+
+
+@pytest.mark.parametrize("m,share,n", [(30000, 0.9, 60000), (120000, 0.6, 480000), (200000, 1.0, 800000),
+                                       (400000, 0.3, 1600000), (400000, 0.5, 1600000)])
+def test_dcoo_init_skewed_keys(gpu, oracle, m, share, n):
+    # many edges leave one node: their keys share the top digit, so one
+    # bucket of the mid-size sort exceeds a block's LDS and is sorted by the
+    # block alone in tiles (odd and even numbers of low passes); the two
+    # largest cases take the full-grid passes
+    rng = np.random.Generator(np.random.PCG64(m))
+    i = np.where(rng.uniform(size=m) < share, 0, rng.integers(0, n, size=m)).astype(np.int32)
+    j = rng.integers(1, n, size=m).astype(np.int32)
     j = np.where(i == j, (j + 1) % n, j).astype(np.int32)
     c = rng.standard_normal(m).astype(np.float32)
     g = gpu.dcoo_init(i, j, c)
