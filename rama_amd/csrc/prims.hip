@@ -13,6 +13,33 @@
 
 namespace rama {
 
+// In-place exclusive scan of 256 LDS counts by wave 0.  Block-uniform call.
+__device__ __forceinline__ void scan256(int* a) {
+    if (threadIdx.x < 64) {
+        const int lane = lane_id();
+        int v[4], s = 0;
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            v[q] = a[lane * 4 + q];
+            s += v[q];
+        }
+        int inc = s;
+#pragma unroll
+        for (int o = 1; o < 64; o <<= 1) {
+            const int t = __shfl_up(inc, o);
+            if (lane >= o) inc += t;
+        }
+        int run = inc - s;
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            a[lane * 4 + q] = run;
+            run += v[q];
+        }
+    }
+    __syncthreads();
+}
+
+
 // Three-launch pass for large arrays: per-tile digit histogram
 // (digit-major table), a scan of the table, and the scatter.
 // Adds one count per active lane to h[d], wave-aggregated: up to two
@@ -34,23 +61,30 @@ __device__ __forceinline__ void wave_count(int* h, u32 d, bool ok) {
     if ((active >> lane) & 1ull) atomicAdd(&h[d], 1);
 }
 
+// The three-launch pass runs 512 threads per 4096-pair tile (8 pairs each):
+// half the registers of 16 per thread, so twice the waves per SIMD within
+// the same LDS.
+constexpr int kScThreads = 512;
+constexpr int kScItems = kRsTile / kScThreads;
+constexpr int kScWaves = kScThreads / 64;
+
 template <class K>
-__global__ __launch_bounds__(kBlock) void k_radix_hist(const K* __restrict__ keys, i64 n, int shift, int bits,
-                                                       int* hist, int ntiles) {
+__global__ __launch_bounds__(kScThreads) void k_radix_hist(const K* __restrict__ keys, i64 n, int shift, int bits,
+                                                           int* hist, int ntiles) {
     __shared__ int h[256];
-    h[threadIdx.x] = 0;
+    if (threadIdx.x < 256) h[threadIdx.x] = 0;
     __syncthreads();
     const u32 mask = (1u << bits) - 1u;
     const i64 base = (i64)blockIdx.x * kRsTile;
-    K key[kRsItems];
+    K key[kScItems];
 #pragma unroll
-    for (int k = 0; k < kRsItems; ++k) {
-        const i64 i = base + (i64)k * kBlock + threadIdx.x;
+    for (int k = 0; k < kScItems; ++k) {
+        const i64 i = base + (i64)k * kScThreads + threadIdx.x;
         key[k] = i < n ? keys[i] : K(0);
     }
 #pragma unroll
-    for (int k = 0; k < kRsItems; ++k) {
-        const i64 i = base + (i64)k * kBlock + threadIdx.x;
+    for (int k = 0; k < kScItems; ++k) {
+        const i64 i = base + (i64)k * kScThreads + threadIdx.x;
         wave_count(h, (u32)(key[k] >> shift) & mask, i < n);
     }
     __syncthreads();
@@ -72,29 +106,28 @@ __device__ __forceinline__ unsigned long long digit_peers(u32 d, int bits) {
 }
 
 template <class K>
-__global__ __launch_bounds__(kBlock) void k_radix_scatter(const K* __restrict__ kin, const u32* __restrict__ vin,
-                                                          K* __restrict__ kout, u32* __restrict__ vout, i64 n,
-                                                          int shift, int bits, const int* __restrict__ offs,
-                                                          int ntiles) {
+__global__ __launch_bounds__(kScThreads) void k_radix_scatter(const K* __restrict__ kin, const u32* __restrict__ vin,
+                                                              K* __restrict__ kout, u32* __restrict__ vout, i64 n,
+                                                              int shift, int bits, const int* __restrict__ offs,
+                                                              int ntiles) {
     __shared__ K sk[kRsTile];
     __shared__ u32 sv[kRsTile];
-    __shared__ int wcnt[kWaves][256];
+    __shared__ int wcnt[kScWaves][256];
     __shared__ int dstart[256];
     __shared__ int gbase[256];
-    __shared__ int ws[kWaves];
 
     const int tid = threadIdx.x, lane = lane_id(), w = tid >> 6;
     const u32 mask = (1u << bits) - 1u;
-    for (int k = tid; k < kWaves * 256; k += kBlock) (&wcnt[0][0])[k] = 0;
+    for (int k = tid; k < kScWaves * 256; k += kScThreads) (&wcnt[0][0])[k] = 0;
     __syncthreads();
 
     const i64 tile0 = (i64)blockIdx.x * kRsTile;
-    const i64 wbase = tile0 + (i64)w * (64 * kRsItems);
-    K key[kRsItems];
-    u32 val[kRsItems];
-    int rnk[kRsItems];
+    const i64 wbase = tile0 + (i64)w * (64 * kScItems);
+    K key[kScItems];
+    u32 val[kScItems];
+    int rnk[kScItems];
 #pragma unroll
-    for (int k = 0; k < kRsItems; ++k) {
+    for (int k = 0; k < kScItems; ++k) {
         const i64 i = wbase + (i64)k * 64 + lane;
         const bool ok = i < n;
         key[k] = ok ? kin[i] : ~K(0);
@@ -102,7 +135,7 @@ __global__ __launch_bounds__(kBlock) void k_radix_scatter(const K* __restrict__ 
     }
     const unsigned long long lt = (1ull << lane) - 1ull;
 #pragma unroll
-    for (int k = 0; k < kRsItems; ++k) {
+    for (int k = 0; k < kScItems; ++k) {
         const u32 d = (u32)(key[k] >> shift) & mask;
         const unsigned long long peers = digit_peers(d, bits);
         const int r = __popcll(peers & lt);
@@ -114,22 +147,22 @@ __global__ __launch_bounds__(kBlock) void k_radix_scatter(const K* __restrict__ 
         rnk[k] = before + r;
     }
     __syncthreads();
-    {
+    if (tid < 256) {
         const int d = tid;
         int tot = 0;
 #pragma unroll
-        for (int q = 0; q < kWaves; ++q) {
+        for (int q = 0; q < kScWaves; ++q) {
             const int c = wcnt[q][d];
             wcnt[q][d] = tot;
             tot += c;
         }
-        int all;
-        dstart[d] = block_excl_scan(tot, ws, &all);
+        dstart[d] = tot;
         if (d < (1 << bits)) gbase[d] = offs[(i64)d * ntiles + blockIdx.x];
     }
     __syncthreads();
+    scan256(dstart);
 #pragma unroll
-    for (int k = 0; k < kRsItems; ++k) {
+    for (int k = 0; k < kScItems; ++k) {
         const u32 d = (u32)(key[k] >> shift) & mask;
         const int pos = dstart[d] + wcnt[w][d] + rnk[k];
         sk[pos] = key[k];
@@ -137,7 +170,7 @@ __global__ __launch_bounds__(kBlock) void k_radix_scatter(const K* __restrict__ 
     }
     __syncthreads();
     const int nvalid = (int)((n - tile0) < kRsTile ? (n - tile0) : kRsTile);
-    for (int p = tid; p < nvalid; p += kBlock) {
+    for (int p = tid; p < nvalid; p += kScThreads) {
         const K k = sk[p];
         const u32 d = (u32)(k >> shift) & mask;
         const i64 o = (i64)gbase[d] + (p - dstart[d]);
@@ -407,32 +440,6 @@ __device__ __forceinline__ void tile_rank(TsSmem<K>& s, const K (&key)[kTsItems]
     __syncthreads();
 }
 
-// In-place exclusive scan of 256 LDS counts by wave 0.  Block-uniform call.
-__device__ __forceinline__ void scan256(int* a) {
-    if (threadIdx.x < 64) {
-        const int lane = lane_id();
-        int v[4], s = 0;
-#pragma unroll
-        for (int q = 0; q < 4; ++q) {
-            v[q] = a[lane * 4 + q];
-            s += v[q];
-        }
-        int inc = s;
-#pragma unroll
-        for (int o = 1; o < 64; o <<= 1) {
-            const int t = __shfl_up(inc, o);
-            if (lane >= o) inc += t;
-        }
-        int run = inc - s;
-#pragma unroll
-        for (int q = 0; q < 4; ++q) {
-            a[lane * 4 + q] = run;
-            run += v[q];
-        }
-    }
-    __syncthreads();
-}
-
 // Every pass of `plan` over the block's register-held items, through LDS.
 template <class K>
 __device__ __forceinline__ void tile_sort_regs(TsSmem<K>& s, K (&key)[kTsItems], u32 (&val)[kTsItems],
@@ -681,15 +688,15 @@ static bool radix_sort_core(Ctx& ctx, K* keys, u32* vals, K* tk, u32* tv, i64 n,
         for (int p = 0; p < plan.passes; ++p) {
             {
                 prof::Scope ps(ctx, "radix_hist", (double)sizeof(K) * n);
-                k_radix_hist<K><<<ntiles, kBlock, 0, ctx.stream>>>(ka, n, plan.shift[p], plan.width[p], th.get(),
-                                                                   ntiles);
+                k_radix_hist<K><<<ntiles, kScThreads, 0, ctx.stream>>>(ka, n, plan.shift[p], plan.width[p],
+                                                                       th.get(), ntiles);
                 RAMA_LAUNCHED();
             }
             scan_exclusive_arr<int>(ctx, th.get(), th.get(), ((i64)1 << plan.width[p]) * ntiles);
             {
                 prof::Scope ps(ctx, "radix_scatter", 2.0 * pair_bytes * n);
-                k_radix_scatter<K><<<ntiles, kBlock, 0, ctx.stream>>>(ka, va, kb, vb, n, plan.shift[p],
-                                                                      plan.width[p], th.get(), ntiles);
+                k_radix_scatter<K><<<ntiles, kScThreads, 0, ctx.stream>>>(ka, va, kb, vb, n, plan.shift[p],
+                                                                           plan.width[p], th.get(), ntiles);
                 RAMA_LAUNCHED();
             }
             std::swap(ka, kb);
