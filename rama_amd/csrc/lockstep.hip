@@ -126,7 +126,7 @@ void apply_stats(Ctx& ctx, Pack& P, const SlotStats* d_st, const std::vector<cha
 
 // Per-slot lower bound of the current graph (get_lb), reduced as dcoo_lb
 // reduces the instance's own edge array.
-std::vector<double> packed_dcoo_lb(Ctx& ctx, Pack& P, unsigned* tickets) {
+std::vector<double> packed_dcoo_lb(Ctx& ctx, Pack& P) {
     const int Q = P.Q();
     std::vector<double> out((size_t)Q, 0.0);
     const std::vector<i64> eb = edge_prefix(P);
@@ -305,7 +305,7 @@ std::vector<i64> rearrange_packed(Ctx& ctx, Pack& P, int* t1, int* t2, int* t3, 
 
 // conflicted_cycles for every slot with part[q] (others get none).
 PackedTriangles packed_cycles(Ctx& ctx, Pack& P, int max_len, float factor, float tol,
-                              const std::vector<char>& part, unsigned* tickets) {
+                              const std::vector<char>& part) {
     const int Q = P.Q();
     PackedTriangles R;
     R.raw.assign((size_t)Q, 0);
@@ -437,9 +437,9 @@ PackedTriangles packed_cycles(Ctx& ctx, Pack& P, int max_len, float factor, floa
 // dual_update_cycle_length for the slots with on[q]: fin[q] and num_tri[q]
 // receive the bound and raw triangle count.
 void packed_dual_update(Ctx& ctx, Pack& P, int c, int steps, float factor, float tol, const std::vector<char>& on,
-                        std::vector<double>& fin, std::vector<i64>& num_tri, unsigned* tickets) {
+                        std::vector<double>& fin, std::vector<i64>& num_tri) {
     const int Q = P.Q();
-    PackedTriangles T = packed_cycles(ctx, P, c, factor, tol, on, tickets);
+    PackedTriangles T = packed_cycles(ctx, P, c, factor, tol, on);
     bool need_lb = false, any_mp = false;
     std::vector<char> mp_on((size_t)Q, 0);
     for (int q = 0; q < Q; ++q) {
@@ -449,7 +449,7 @@ void packed_dual_update(Ctx& ctx, Pack& P, int c, int steps, float factor, float
         else mp_on[q] = 1, any_mp = true;
     }
     if (need_lb) {
-        const std::vector<double> lb = packed_dcoo_lb(ctx, P, tickets);
+        const std::vector<double> lb = packed_dcoo_lb(ctx, P);
         for (int q = 0; q < Q; ++q)
             if (on[q] && T.raw[q] == 0) fin[q] = lb[q];
     }
@@ -480,7 +480,7 @@ void packed_dual_update(Ctx& ctx, Pack& P, int c, int steps, float factor, float
     DBuf<i64> dpbeg = upload_vec(ctx, pbeg);
     MPPack pk{Q, dtb.get(), deb.get(), dtblk.get(), deblk.get(), dpbeg.get()};
     std::vector<double> lb((size_t)Q, 0.0);
-    mp_run_packed(ctx, S, pk, tblk[Q], eblk[Q], steps, tb.data(), eb.data(), don.get(), tickets, lb.data());
+    mp_run_packed(ctx, S, pk, tblk[Q], eblk[Q], steps, tb.data(), eb.data(), don.get(), lb.data());
     for (int q = 0; q < Q; ++q)
         if (mp_on[q]) fin[q] = lb[q];
     // the reparametrised graph; dims of the slots that changed
@@ -497,11 +497,11 @@ void packed_dual_update(Ctx& ctx, Pack& P, int c, int steps, float factor, float
 
 // dual_solver for every slot with part[q]; returns the bounds.
 std::vector<double> packed_dual(Ctx& ctx, Pack& P, int max_len, int num_iter, float factor, int num_outer, float tol,
-                                const std::vector<char>& part, unsigned* tickets) {
+                                const std::vector<char>& part) {
     const int Q = P.Q();
     std::vector<double> fin((size_t)Q, 0.0), prev((size_t)Q, 0.0);
     if (max_len < 3 || num_iter == 0 || num_outer == 0) {
-        const std::vector<double> lb = packed_dcoo_lb(ctx, P, tickets);
+        const std::vector<double> lb = packed_dcoo_lb(ctx, P);
         for (int q = 0; q < Q; ++q)
             if (part[q]) fin[q] = lb[q];
         return fin;
@@ -512,7 +512,7 @@ std::vector<double> packed_dual(Ctx& ctx, Pack& P, int max_len, int num_iter, fl
         bool any = false;
         for (int q = 0; q < Q; ++q) any = any || act[q];
         if (!any) break;
-        for (int c = 3; c <= max_len; ++c) packed_dual_update(ctx, P, c, num_iter, factor, tol, act, fin, num_tri, tickets);
+        for (int c = 3; c <= max_len; ++c) packed_dual_update(ctx, P, c, num_iter, factor, tol, act, fin, num_tri);
         for (int q = 0; q < Q; ++q) {
             if (!act[q]) continue;
             if (num_tri[q] == 0 || (o > 0 && (fin[q] - prev[q]) < 1e-3)) act[q] = 0;
@@ -579,8 +579,6 @@ struct Handoff {
 class Lockstep {
    public:
     Lockstep(Ctx& ctx, const Options& o, int workers, LockstepSink& sink) : ctx_(ctx), o_(o), sink_(sink) {
-        tickets_ = ctx_.buf<unsigned>(kMaxSlots);
-        ctx_.zero(tickets_.get(), kMaxSlots);
         times_ = std::getenv("RAMA_STAGE_TIMES") != nullptr;
         t0_ = tlast_ = std::chrono::steady_clock::now();
         for (int w = 0; w < std::max(1, workers); ++w) pool_.emplace_back([this] { worker(); });
@@ -648,7 +646,6 @@ class Lockstep {
     Options o_;
     LockstepSink& sink_;
     Pack P_;
-    DBuf<unsigned> tickets_;
     bool times_ = false;
     std::chrono::steady_clock::time_point t0_, tlast_;
     std::map<std::string, double> stage_ms_;
@@ -955,8 +952,7 @@ void Lockstep::admit(int k, const int64_t* edge_off, const int32_t* i, const int
     // initial dual of the new instances (the bound)
     {
         const std::vector<double> lb = packed_dual(ctx_, P_, o_.max_cycle_length_lb, o_.num_dual_itr_lb,
-                                                   o_.tri_memory_factor, o_.num_outer_itr_dual, 1e-4f, fresh,
-                                                   tickets_.get());
+                                                   o_.tri_memory_factor, o_.num_outer_itr_dual, 1e-4f, fresh);
         for (int q = Q0; q < Q; ++q) P_.s[q].lb = lb[q];
     }
     mark("initial dual", 0, Q);
@@ -1010,8 +1006,7 @@ void Lockstep::round() {
             any = any || on[q];
         }
         if (any)
-            packed_dual(ctx_, P_, o_.max_cycle_length_primal, o_.num_dual_itr_primal, 1.0f, 1, 1e-4f, on,
-                        tickets_.get());
+            packed_dual(ctx_, P_, o_.max_cycle_length_primal, o_.num_dual_itr_primal, 1.0f, 1, 1e-4f, on);
     }
     mark("s:dual", 0, Q);
     // matching on the joint symmetric CSR
@@ -1027,7 +1022,7 @@ void Lockstep::round() {
     ctx_.d2h(hb.data(), dhb.get(), Q + 1);
     DBuf<int> map;
     std::vector<int> matched((size_t)Q, 0);
-    matching_packed(ctx_, U, P_.nslot.get(), Q, hb.data(), dhb.get(), o_.mean_multiplier_mm, tickets_.get(), map,
+    matching_packed(ctx_, U, P_.nslot.get(), Q, hb.data(), dhb.get(), o_.mean_multiplier_mm, map,
                     matched.data());
     U = Csr();
     mark("s:matching", 0, Q);

@@ -294,7 +294,7 @@ __global__ __launch_bounds__(kBlock) void k_match_packed(int n, float mult, int 
 // Instance q's CSR entries are [hbeg[q], hbeg[q+1]) (dbeg on the device).
 // Fills map (global ids) and matched_out[q] = matched vertices of q.
 void matching_packed(Ctx& ctx, const Csr& U, const int* nslot, int Q, const i64* hbeg, const i64* dbeg,
-                     float mean_multiplier, unsigned* tickets, DBuf<int>& map, int* matched_out) {
+                     float mean_multiplier, DBuf<int>& map, int* matched_out) {
     const int n = U.n;
     map = ctx.buf<int>(n > 0 ? n : 1);
     DBuf<MatchState> stb = ctx.buf<MatchState>(Q);

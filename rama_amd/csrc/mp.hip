@@ -536,7 +536,7 @@ __global__ void k_mp_init_packed(int Q, const double* __restrict__ p_edges, cons
 }
 
 void mp_run_packed(Ctx& ctx, MPState& S, const MPPack& pk, int total_tblocks, int total_eblocks, int steps,
-                   const i64* seg_tbeg_host, const i64* seg_ebeg_host, const int* d_on, unsigned* tickets,
+                   const i64* seg_tbeg_host, const i64* seg_ebeg_host, const int* d_on,
                    double* lb_out) {
     const int Q = pk.Q;
     // initial bound per instance, reduced as reduce_to reduces each part alone

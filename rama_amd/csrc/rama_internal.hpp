@@ -103,7 +103,7 @@ void contract_packed(Ctx& ctx, const DCoo& A, const int* node_map, int max_label
 // size U.n and the number of matched vertices.
 int matching_vertex_based(Ctx& ctx, const Csr& U, float mean_multiplier, DBuf<int>& map);
 void matching_packed(Ctx& ctx, const Csr& U, const int* nslot, int Q, const i64* hbeg, const i64* dbeg,
-                     float mean_multiplier, unsigned* tickets, DBuf<int>& map, int* matched_out);
+                     float mean_multiplier, DBuf<int>& map, int* matched_out);
 
 // ---- cycles.hip
 struct Triangles {
@@ -151,9 +151,9 @@ struct MPPack {
 };
 // Runs `steps` iterations for every instance with d_on[q] != 0; lb_out[q]
 // (host) receives each instance's final bound.  seg_*beg_host are the host
-// copies of tbeg/ebeg; tickets (Q, zeroed) serve the segmented reductions.
+// copies of tbeg/ebeg.
 void mp_run_packed(Ctx& ctx, MPState& S, const MPPack& pk, int total_tblocks, int total_eblocks, int steps,
-                   const i64* seg_tbeg_host, const i64* seg_ebeg_host, const int* d_on, unsigned* tickets,
+                   const i64* seg_tbeg_host, const i64* seg_ebeg_host, const int* d_on,
                    double* lb_out);
 double mp_lower_bound(Ctx& ctx, MPState& S);
 double mp_fused_lower_bound(Ctx& ctx, MPState& S);

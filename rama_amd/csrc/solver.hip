@@ -57,6 +57,7 @@ Options to_options(const rama_options_c& c) {
 SolveResult solve(Ctx& ctx, DCoo& A, const Options& o) {
     SolveResult out;
     out.t0 = std::chrono::steady_clock::now();
+    ctx.last_cycles_empty = false;  // a hint from an earlier solve on this context says nothing here
     StageTimes st;
     st.begin(ctx);
     out.lb = dual_solver(ctx, A, o.max_cycle_length_lb, o.num_dual_itr_lb, o.tri_memory_factor, o.num_outer_itr_dual,
@@ -80,6 +81,7 @@ SolveResult solve(Ctx& ctx, DCoo& A, const Options& o) {
 // use_mm whether matching still picks the contractions.
 void solve_rounds(Ctx& ctx, DCoo& A, SolveResult& out, const Options& o, int start_iter, bool use_mm) {
     StageTimes st;
+    if (start_iter > 0) ctx.last_cycles_empty = false;  // resumed on another context: no hint
     const auto t0 = out.t0;  // the time budget counts from the start of the solve
     const i64 N = out.num_labels;
     for (int iter = start_iter; A.nnz > 0; ++iter) {
