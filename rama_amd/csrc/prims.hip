@@ -245,40 +245,40 @@ __global__ __launch_bounds__(kBlock) void k_radix_hist_all(const K* __restrict__
 // tile is reordered in LDS and written out in digit runs.  Status words are
 // (epoch | flag | count) as in the single-pass scan.
 template <class K>
-__global__ __launch_bounds__(kBlock) void k_onesweep(const K* __restrict__ kin, const u32* __restrict__ vin,
+__global__ __launch_bounds__(kScThreads) void k_onesweep(const K* __restrict__ kin, const u32* __restrict__ vin,
                                                      K* __restrict__ kout, u32* __restrict__ vout, i64 n, int shift,
                                                      int bits, const int* __restrict__ hist_pass,
                                                      u64* __restrict__ status, int* ctl, u32 epoch) {
     __shared__ K sk[kRsTile];
     __shared__ u32 sv[kRsTile];
-    __shared__ int wcnt[kWaves][256];
+    __shared__ int wcnt[kScWaves][256];
     __shared__ int tcnt[256];
     __shared__ int dstart[256];
+    __shared__ int dglob[256];
     __shared__ i64 gbase[256];
-    __shared__ int ws[kWaves];
     __shared__ int tile_s;
 
     const int tid = threadIdx.x, lane = lane_id(), w = tid >> 6;
     const u32 mask = (1u << bits) - 1u;
     if (tid == 0) tile_s = atomicAdd(&ctl[0], 1);
-    for (int k = tid; k < kWaves * 256; k += kBlock) (&wcnt[0][0])[k] = 0;
-    tcnt[tid] = 0;
+    for (int k = tid; k < kScWaves * 256; k += kScThreads) (&wcnt[0][0])[k] = 0;
+    if (tid < 256) tcnt[tid] = 0;
     __syncthreads();
     const int tile = tile_s;
     const i64 tile0 = (i64)tile * kRsTile;
-    const i64 wbase = tile0 + (i64)w * (64 * kRsItems);
-    K key[kRsItems];
-    u32 val[kRsItems];
-    int rnk[kRsItems];
+    const i64 wbase = tile0 + (i64)w * (64 * kScItems);
+    K key[kScItems];
+    u32 val[kScItems];
+    int rnk[kScItems];
 #pragma unroll
-    for (int k = 0; k < kRsItems; ++k) {
+    for (int k = 0; k < kScItems; ++k) {
         const i64 i = wbase + (i64)k * 64 + lane;
         const bool ok = i < n;
         key[k] = ok ? kin[i] : ~K(0);  // padding sorts into the last bucket, after all real keys
         val[k] = ok ? vin[i] : 0u;
     }
 #pragma unroll
-    for (int k = 0; k < kRsItems; ++k) wave_count(tcnt, (u32)(key[k] >> shift) & mask, wbase + (i64)k * 64 + lane < n);
+    for (int k = 0; k < kScItems; ++k) wave_count(tcnt, (u32)(key[k] >> shift) & mask, wbase + (i64)k * 64 + lane < n);
     __syncthreads();
     // publish this tile's digit counts early so successors can look back
     u64* my = status + (i64)tile * 256;
@@ -289,7 +289,7 @@ __global__ __launch_bounds__(kBlock) void k_onesweep(const K* __restrict__ kin, 
     // stable ranks within the tile
     const unsigned long long lt = (1ull << lane) - 1ull;
 #pragma unroll
-    for (int k = 0; k < kRsItems; ++k) {
+    for (int k = 0; k < kScItems; ++k) {
         const u32 d = (u32)(key[k] >> shift) & mask;
         const unsigned long long peers = digit_peers(d, bits);
         const int r = __popcll(peers & lt);
@@ -301,21 +301,24 @@ __global__ __launch_bounds__(kBlock) void k_onesweep(const K* __restrict__ kin, 
         rnk[k] = before + r;
     }
     __syncthreads();
-    {
+    if (tid < 256) {
         const int d = tid;
         int tot = 0;
 #pragma unroll
-        for (int q = 0; q < kWaves; ++q) {
+        for (int q = 0; q < kScWaves; ++q) {
             const int c = wcnt[q][d];
             wcnt[q][d] = tot;
             tot += c;
         }
-        int all;
-        dstart[d] = block_excl_scan(tot, ws, &all);
-        // global start of digit d in this pass, then look back over tiles
-        const int hd = d <= (int)mask ? hist_pass[d] : 0;
-        int dall;
-        const int dglob = block_excl_scan(hd, ws, &dall);
+        dstart[d] = tot;
+        dglob[d] = d <= (int)mask ? hist_pass[d] : 0;
+    }
+    __syncthreads();
+    scan256(dstart);
+    scan256(dglob);  // global start of each digit in this pass
+    {
+        // then look back over the earlier tiles, one thread per digit
+        const int d = tid;
         if (d <= (int)mask) {
             i64 prefix = 0;
             if (tile > 0) {
@@ -358,13 +361,13 @@ __global__ __launch_bounds__(kBlock) void k_onesweep(const K* __restrict__ kin, 
                 __hip_atomic_store(&my[d], scan_word(epoch, 2, (u64)(prefix + tcnt[d])), __ATOMIC_RELAXED,
                                    __HIP_MEMORY_SCOPE_AGENT);
             }
-            gbase[d] = (i64)dglob + prefix;
+            gbase[d] = (i64)dglob[d] + prefix;
         }
     }
     __syncthreads();
     if (tid == 0 && tile == (int)gridDim.x - 1) ctl[0] = 0;  // every block has drawn its tile
 #pragma unroll
-    for (int k = 0; k < kRsItems; ++k) {
+    for (int k = 0; k < kScItems; ++k) {
         const u32 d = (u32)(key[k] >> shift) & mask;
         const int pos = dstart[d] + wcnt[w][d] + rnk[k];
         sk[pos] = key[k];
@@ -372,7 +375,7 @@ __global__ __launch_bounds__(kBlock) void k_onesweep(const K* __restrict__ kin, 
     }
     __syncthreads();
     const int nvalid = (int)((n - tile0) < kRsTile ? (n - tile0) : kRsTile);
-    for (int p = tid; p < nvalid; p += kBlock) {
+    for (int p = tid; p < nvalid; p += kScThreads) {
         const K k = sk[p];
         const u32 d = (u32)(k >> shift) & mask;
         const i64 o = gbase[d] + (p - dstart[d]);
@@ -667,7 +670,7 @@ static bool radix_sort_core(Ctx& ctx, K* keys, u32* vals, K* tk, u32* tv, i64 n,
         u64* st = ctx.scan_begin((i64)ntiles * 256, &epoch);
         {
             prof::Scope ps(ctx, "radix_onesweep", 2.0 * pair_bytes * n);
-            k_onesweep<K><<<ntiles, kBlock, 0, ctx.stream>>>(keys, vals, tk, tv, n, top.shift[0], 8, hsum, st,
+            k_onesweep<K><<<ntiles, kScThreads, 0, ctx.stream>>>(keys, vals, tk, tv, n, top.shift[0], 8, hsum, st,
                                                              ctx.scan_counter.get(), epoch);
             RAMA_LAUNCHED();
         }
@@ -718,7 +721,7 @@ static bool radix_sort_core(Ctx& ctx, K* keys, u32* vals, K* tk, u32* tv, i64 n,
         {
             // reads and writes one (key, u32 value) pair per element
             prof::Scope ps(ctx, "radix_onesweep", 2.0 * pair_bytes * n);
-            k_onesweep<K><<<ntiles, kBlock, 0, ctx.stream>>>(ka, va, kb, vb, n, plan.shift[p], plan.width[p],
+            k_onesweep<K><<<ntiles, kScThreads, 0, ctx.stream>>>(ka, va, kb, vb, n, plan.shift[p], plan.width[p],
                                                              hsum + p * 256, st, ctx.scan_counter.get(), epoch);
             RAMA_LAUNCHED();
         }
