@@ -146,11 +146,15 @@ int matching_vertex_based(Ctx& ctx, const Csr& U, float mean_multiplier, DBuf<in
         });
     }
     const float* W = U.w.get();
-    reduce_to<double>(
-        ctx, [=] __device__(i64 e) { const float x = W[e]; return x >= 0.0f ? (double)x : 0.0; }, U.nnz, 0.0, OpSum(),
-        &st->pos_sum);
-    reduce_to<double>(ctx, [=] __device__(i64 e) { return W[e] >= 0.0f ? 1.0 : 0.0; }, U.nnz, 0.0, OpSum(),
-                      &st->pos_cnt);
+    // sum and count of the non-negative costs, in one pass
+    reduce_pair_to(
+        ctx,
+        [=] __device__(i64 e, double& a, double& b) {
+            const float x = W[e];
+            a = x >= 0.0f ? (double)x : 0.0;
+            b = x >= 0.0f ? 1.0 : 0.0;
+        },
+        U.nnz, &st->pos_sum, &st->pos_cnt);
     const i64 threads = (i64)n * kGroup;
     // the stop rule runs on the device; the host only skips launches once it
     // knows the loop has ended: the first kFirstRounds rounds are queued,

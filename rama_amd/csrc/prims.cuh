@@ -332,6 +332,55 @@ void reduce_to(Ctx& ctx, F f, i64 n, T ident, Op op, T* dst) {
     RAMA_LAUNCHED();
 }
 
+// Two double sums in one pass, each reduced exactly as reduce_to would
+// reduce it alone (same grid, grid-stride order, block tree and partial
+// combination), so the results are bit-identical to two separate calls.
+// f(i, a, b) gives element i's two terms.
+template <class F>
+__global__ __launch_bounds__(kBlock) void k_reduce_pair(F f, i64 n, double* partials, unsigned* ticket,
+                                                        double* dst_a, double* dst_b) {
+    __shared__ double ws[kWaves];
+    __shared__ bool last;
+    double a = 0.0, b = 0.0;
+    for (i64 i = (i64)blockIdx.x * kBlock + threadIdx.x; i < n; i += (i64)gridDim.x * kBlock) {
+        double x, y;
+        f(i, x, y);
+        a += x;
+        b += y;
+    }
+    const double ra = block_reduce(a, OpSum(), ws);
+    const double rb = block_reduce(b, OpSum(), ws);
+    if (threadIdx.x == 0) {
+        __hip_atomic_store(&partials[blockIdx.x], ra, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        __hip_atomic_store(&partials[kReduceMaxBlocks + blockIdx.x], rb, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        const unsigned t = __hip_atomic_fetch_add(ticket, 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
+        last = t == gridDim.x - 1;
+    }
+    __syncthreads();
+    if (!last) return;
+    __threadfence();  // acquire: the other blocks' partials are visible to plain loads
+    double sa = 0.0, sb = 0.0;
+    for (unsigned k = threadIdx.x; k < gridDim.x; k += kBlock) {
+        sa += partials[k];
+        sb += partials[kReduceMaxBlocks + k];
+    }
+    const double ta = block_reduce(sa, OpSum(), ws);
+    const double tb = block_reduce(sb, OpSum(), ws);
+    if (threadIdx.x == 0) {
+        *dst_a = ta;
+        *dst_b = tb;
+        *ticket = 0;
+    }
+}
+
+template <class F>
+void reduce_pair_to(Ctx& ctx, F f, i64 n, double* dst_a, double* dst_b) {
+    const unsigned nb = (unsigned)std::min<i64>(std::max<i64>((n + kBlock - 1) / kBlock, 1), kReduceMaxBlocks);
+    double* scratch = static_cast<double*>(ctx.reduce_scratch(2 * kReduceMaxBlocks * 8));
+    k_reduce_pair<F><<<nb, kBlock, 0, ctx.stream>>>(f, n, scratch, ctx.reduce_ticket(), dst_a, dst_b);
+    RAMA_LAUNCHED();
+}
+
 template <class T, class Op, class F>
 T reduce(Ctx& ctx, F f, i64 n, T ident, Op op) {
     DBuf<T> r = ctx.buf<T>(1);
