@@ -147,18 +147,21 @@ static void mp_build_impl(Ctx& ctx, const DCoo& A, Triangles& tri, MPState& S, i
     int* ECNT = S.ecnt.get();
     int* SOFF = S.soff.get();
     const u64 mask = ((u64)1 << kb) - 1;
-    // graph edges
+    // Places in the merged list come from the scan of the chord flags (CP,
+    // N3 + 1 entries): the chords below a key are the chord heads before its
+    // first triangle item.
+    // graph edges: the chords below KA[e] are the heads before PT[e]
     for_each(ctx, E, [=] __device__(i64 e) {
-        const i64 u = e + (Nc > 0 ? lower_bound_dev(CK, Nc, KA[e]) : 0);
+        const i64 u = e + CP[PT[e]];
         EI[u] = AR[e];
         EJ[u] = AC[e];
         EC[u] = AV[e];
         SOFF[u] = PT[e];
         if (e == 0) SOFF[Em] = (int)N3;
     });
-    // chords (cost 0)
+    // chords (cost 0): the graph keys below a chord's key are PA of its head
     for_each(ctx, Nc, [=] __device__(i64 c) {
-        const i64 u = c + lower_bound_dev(KA, E, CK[c]);
+        const i64 u = c + PA[CH[c]];
         EI[u] = (int)((u64)CK[c] >> kb);
         EJ[u] = (int)((u64)CK[c] & mask);
         EC[u] = 0.0f;
@@ -182,7 +185,11 @@ static void mp_build_impl(Ctx& ctx, const DCoo& A, Triangles& tri, MPState& S, i
     u32* POS = S.pos.get();
     for_each(ctx, N3, [=] __device__(i64 q) {
         const K k = KT[q];
-        const int u = (int)(PA[q] + (Nc > 0 ? lower_bound_dev(CK, Nc, k) : 0));
+        // chord heads before q, less the head of q's own run when that run
+        // is a chord and q is not its head
+        const bool dup = q > 0 && KT[q - 1] == k;
+        const bool chord = PA[q] == E || KA[PA[q]] != k;
+        const int u = (int)(PA[q] + CP[q] - (dup && chord ? 1 : 0));
         const u32 v = VT[q], t = v / 3, s = v % 3;
         (s == 0 ? C12 : (s == 1 ? C13 : C23))[t] = u;
         (s == 0 ? TC12 : (s == 1 ? TC13 : TC23))[t] = 0.0f;  // triangle costs start at zero
