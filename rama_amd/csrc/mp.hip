@@ -41,19 +41,37 @@ __device__ __forceinline__ i64 lower_bound_dev(const K* a, i64 n, K x) {
 
 // out[i] = lower_bound(a[0..n), q[i]) for ascending queries q[0..m).  Each
 // block bounds its 256 answers by the answers of its first and last query
-// (two full searches), so the per-thread searches run in a narrow, cached
-// window instead of walking the whole array.
+// (two full searches); a window of at most kLbWin keys is staged in LDS with
+// coalesced loads and searched there, a wider one in place.
+constexpr int kLbWin = 2048;
 template <class K>
 __global__ __launch_bounds__(kBlock) void k_sorted_lb(const K* __restrict__ a, i64 n, const K* __restrict__ q, i64 m,
                                                       int* __restrict__ out) {
     __shared__ i64 range[2];
+    __shared__ K sa[kLbWin];
     const i64 b0 = (i64)blockIdx.x * kBlock;
     const i64 b1 = (b0 + kBlock < m ? b0 + kBlock : m) - 1;
     if (threadIdx.x == 0) range[0] = lower_bound_dev(a, n, q[b0]);
     if (threadIdx.x == 64) range[1] = lower_bound_dev(a, n, q[b1]);
     __syncthreads();
+    const i64 r0 = range[0], r1 = range[1];
     const i64 i = b0 + threadIdx.x;
-    if (i <= b1) out[i] = (int)lower_bound_dev(a, range[0], range[1], q[i]);
+    if (r1 - r0 <= kLbWin) {  // uniform
+        const int w = (int)(r1 - r0);
+        for (int k = threadIdx.x; k < w; k += kBlock) sa[k] = a[r0 + k];
+        __syncthreads();
+        if (i <= b1) {
+            const K x = q[i];
+            int lo = 0, hi = w;
+            while (lo < hi) {
+                const int mid = (lo + hi) >> 1;
+                if (sa[mid] < x) lo = mid + 1; else hi = mid;
+            }
+            out[i] = (int)(r0 + lo);
+        }
+    } else if (i <= b1) {
+        out[i] = (int)lower_bound_dev(a, r0, r1, q[i]);
+    }
 }
 
 template <class K>
