@@ -130,6 +130,18 @@ def test_packed_modes_equal_alone(gpu, preset):
     _assert_same(_alone(gpu, insts, opts), gpu.solve_batch(insts, opts, workers=4))
 
 
+@pytest.mark.parametrize("preset", ["PD", "PD+"])
+def test_packed_hubs_equal_alone(gpu, preset):
+    # supernode-like hubs among ordinary instances: the heavy-pair paths of
+    # the packed cycle search (per-lane probing, whole-wave intersections)
+    # write into each instance's own triangle region
+    from tests.instances_small import hub_graph
+    insts = [hub_graph(1200 + 100 * k, 8 + k, 150 + 40 * k, seed=30 + k) for k in range(4)]
+    insts += mixed_instances(6, seed=23)
+    opts = gpu.options(preset, verbose=0)
+    _assert_same(_alone(gpu, insts, opts), gpu.solve_batch(insts, opts, workers=4))
+
+
 def test_packed_duplicates_gaps_and_tiny(gpu):
     from tests.instances_small import named, random_graph
     rng = np.random.default_rng(5)
