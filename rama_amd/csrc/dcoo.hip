@@ -330,19 +330,50 @@ DCoo contract(Ctx& ctx, const DCoo& A, const int* node_map, int max_label, bool*
 }
 
 // compute_offsets (include/rama_utils.h:403-413): out[v] = #{ids < v} for
-// v in [0, max_value+1]; one thread per boundary of the sorted ids.
-template <class RowOf>
-__global__ void k_offsets(RowOf rowof, i64 n, int end_value, int* out) {
-    const i64 i = (i64)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i > n) return;
-    const int prev = i > 0 ? rowof(i - 1) : -1;
-    const int cur = i < n ? rowof(i) : end_value;
-    for (int r = prev + 1; r <= cur && r <= end_value; ++r) out[r] = (int)i;
+// v in [0, max_value+1], one thread per row: out[r] = lower_bound(ids, r).
+// Each block bounds its 256 answers by two searches and stages that window
+// of ids in LDS when it is short.  (One thread per boundary of the ids would
+// leave a long run of empty rows, as a sparse forest over many nodes has,
+// to a single thread.)
+constexpr int kOffWin = 2048;
+__device__ __forceinline__ i64 lower_bound_ids(const int* __restrict__ a, i64 lo, i64 hi, int x) {
+    while (lo < hi) {
+        const i64 mid = (lo + hi) >> 1;
+        if (a[mid] < x) lo = mid + 1; else hi = mid;
+    }
+    return lo;
+}
+__global__ __launch_bounds__(kBlock) void k_offsets_rows(const int* __restrict__ ids, i64 n, int end_value,
+                                                         int* __restrict__ out) {
+    __shared__ i64 range[2];
+    __shared__ int sa[kOffWin];
+    const i64 r0 = (i64)blockIdx.x * kBlock;
+    const i64 r1 = (r0 + kBlock - 1 < end_value ? r0 + kBlock - 1 : end_value);
+    if (threadIdx.x == 0) range[0] = lower_bound_ids(ids, 0, n, (int)r0);
+    if (threadIdx.x == 64) range[1] = lower_bound_ids(ids, 0, n, (int)r1);
+    __syncthreads();
+    const i64 lo = range[0], hi = range[1];
+    const i64 r = r0 + threadIdx.x;
+    if (hi - lo <= kOffWin) {  // uniform
+        const int w = (int)(hi - lo);
+        for (int k = threadIdx.x; k < w; k += kBlock) sa[k] = ids[lo + k];
+        __syncthreads();
+        if (r <= r1) {
+            int a = 0, b = w;
+            while (a < b) {
+                const int mid = (a + b) >> 1;
+                if (sa[mid] < (int)r) a = mid + 1; else b = mid;
+            }
+            out[r] = (int)(lo + a);
+        }
+    } else if (r <= r1) {
+        out[r] = (int)lower_bound_ids(ids, lo, hi, (int)r);
+    }
 }
 
 void compute_offsets(Ctx& ctx, const int* sorted_ids, i64 n, int max_value, int* out) {
-    auto f = [=] __device__(i64 i) { return sorted_ids[i]; };
-    k_offsets<<<grid_for(n + 1), kBlock, 0, ctx.stream>>>(f, n, max_value + 1, out);
+    const int end_value = max_value + 1;
+    k_offsets_rows<<<grid_for((i64)end_value + 1), kBlock, 0, ctx.stream>>>(sorted_ids, n, end_value, out);
     RAMA_LAUNCHED();
 }
 
