@@ -104,7 +104,10 @@ __global__ __launch_bounds__(kBlock) void k_match(int n, float mult, int round, 
     int add = 0;
     if (v < n && !matched[v]) {
         const int vb = best[v];
-        if (vb > v && !matched[vb] && best[vb] == v) {
+        // vb's flag need not be read: a vertex matched in an earlier round
+        // keeps its partner in best[] (pick_best skips matched vertices), so
+        // best[vb] == v with v unmatched means vb is unmatched too
+        if (vb > v && best[vb] == v) {
             map[vb] = v;
             matched[v] = 1;
             matched[vb] = 1;
@@ -272,7 +275,9 @@ __global__ __launch_bounds__(kBlock) void k_match_packed(int n, float mult, int 
     int add = 0;
     if (v < n && match_round_active(sts + q, mult, round, &thr) && !matched[v]) {
         const int vb = best[v];
-        if (vb > v && !matched[vb] && best[vb] == v) {
+        // as in k_match: best[vb] == v already implies vb is unmatched
+        // (vb is in v's instance, whose rounds run or stop together)
+        if (vb > v && best[vb] == v) {
             map[vb] = v;
             matched[v] = 1;
             matched[vb] = 1;
