@@ -299,9 +299,10 @@ struct Ctx {
     size_t up_cap = 0, up_off = 0;
     int64_t up_sync = -1;
     template <class T>
-    void upload(T* d, const T* h, i64 n) {
+    void upload(T* d, const T* h, i64 n, const char* fn = __builtin_FUNCTION(), int line = __builtin_LINE()) {
         const size_t bytes = sizeof(T) * (size_t)n;
         if (bytes == 0) return;
+        if (sync_sites_on()) note_sync_site(fn, -line);  // uploads are counted at negative line numbers
         const size_t need = (bytes + 255) & ~(size_t)255;
         if (up_sync != syncs) {
             up_off = 0;

@@ -72,9 +72,10 @@ __device__ __forceinline__ int seg_of(const i64* pre, int Q, i64 x) {
 }
 
 template <class T>
-DBuf<T> upload_vec(Ctx& ctx, const std::vector<T>& h) {
+DBuf<T> upload_vec(Ctx& ctx, const std::vector<T>& h, const char* fn = __builtin_FUNCTION(),
+                   int line = __builtin_LINE()) {
     DBuf<T> d = ctx.buf<T>(std::max<i64>((i64)h.size(), 1));
-    ctx.upload(d.get(), h.data(), (i64)h.size());
+    ctx.upload(d.get(), h.data(), (i64)h.size(), fn, line);
     return d;
 }
 
