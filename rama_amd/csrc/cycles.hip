@@ -102,10 +102,11 @@ __device__ __forceinline__ i64 find_rep(const CycArgs& a, i64 x, i64 r0) {
 template <int S>
 __device__ __forceinline__ bool resolve(const CycArgs& a, i64 x, i64 r0, int& v1, int& v2, int& n1, int& n2,
                                         int& qa, int& qb) {
-    const i64 r = find_rep(a, x, r0);
+    // 3-cycles: expansion x is rep edge x (no offsets, no tile reps)
+    const i64 r = S == 3 ? x : find_rep(a, x, r0);
     v1 = a.rv1[r];
     v2 = a.rv2[r];
-    const i64 loc = x - a.xoff[r];
+    const i64 loc = S == 3 ? 0 : x - a.xoff[r];
     if (S == 3) {
         qa = v1;
         qb = v2;
@@ -464,7 +465,7 @@ __global__ __launch_bounds__(kBlock) void k_cycle_emit(CycArgs a, const unsigned
     if (x < a.X) {
         c = cnt8[x];
         if (c > 0) {
-            resolve<S>(a, x, a.tile_rep[blk], v1, v2, n1, n2, qa, qb);
+            resolve<S>(a, x, S == 3 ? 0 : a.tile_rep[blk], v1, v2, n1, n2, qa, qb);
             if (c == 255) {
                 // the count saturated its byte: count again (by the wave
                 // when both lists are long)
@@ -602,13 +603,12 @@ template <int S>
 static i64 run_stage(Ctx& ctx, const Csr& G, const int* rv1, const int* rv2, i64 R, i64 base, i64 budget,
                      Triangles& T) {
     const int* OFF = G.off.get();
-    DBuf<i64> xoff = ctx.buf<i64>(R + 1);
-    if (S == 3) {
-        iota(ctx, xoff.get(), R + 1);
-    } else if (S == 4) {
+    DBuf<i64> xoff;  // 3-cycles: expansion = rep edge, no offsets
+    if (S > 3) xoff = ctx.buf<i64>(R + 1);
+    if (S == 4) {
         scan_exclusive<i64>(ctx, [=] __device__(i64 r) { const int v = rv1[r]; return (i64)(OFF[v + 1] - OFF[v]); },
                             xoff.get(), R);
-    } else {
+    } else if (S == 5) {
         scan_exclusive<i64>(
             ctx,
             [=] __device__(i64 r) {
@@ -621,10 +621,13 @@ static i64 run_stage(Ctx& ctx, const Csr& G, const int* rv1, const int* rv2, i64
     if (ctx.verbose && S > 3) std::printf("%d-cycles: number of expansions: %lld\n", S, (long long)X);
     if (X == 0) return 0;
     const i64 ntiles = (X + kCycTile - 1) / kCycTile;
-    DBuf<int> tile_rep = ctx.buf<int>(ntiles);
-    k_tile_rep<<<grid_for(R), kBlock, 0, ctx.stream>>>(xoff.get(), R, tile_rep.get());
-    RAMA_LAUNCHED();
-    CycArgs a{OFF, G.adj.get(), rv1, rv2, xoff.get(), tile_rep.get(), R, X};
+    DBuf<int> tile_rep;
+    if (S > 3) {
+        tile_rep = ctx.buf<int>(ntiles);
+        k_tile_rep<<<grid_for(R), kBlock, 0, ctx.stream>>>(xoff.get(), R, tile_rep.get());
+        RAMA_LAUNCHED();
+    }
+    CycArgs a{OFF, G.adj.get(), rv1, rv2, S > 3 ? xoff.get() : nullptr, S > 3 ? tile_rep.get() : nullptr, R, X};
     DBuf<unsigned char> cnt8 = ctx.buf<unsigned char>(X);
     DBuf<i64> tile = ctx.buf<i64>(ntiles + 1);
     RAMA_REQUIRE(ntiles < ((i64)1 << 31), "too many cycle expansions");
@@ -726,13 +729,12 @@ void run_stage_packed(Ctx& ctx, const Csr& G, const int* rv1, const int* rv2, i6
     for (int q = 0; q < Q; ++q) h_total[q] = 0;
     if (R == 0) return;
     const int* OFF = G.off.get();
-    DBuf<i64> xoff = ctx.buf<i64>(R + 1);
-    if (S == 3) {
-        iota(ctx, xoff.get(), R + 1);
-    } else if (S == 4) {
+    DBuf<i64> xoff;  // 3-cycles: expansion = rep edge, no offsets
+    if (S > 3) xoff = ctx.buf<i64>(R + 1);
+    if (S == 4) {
         scan_exclusive<i64>(ctx, [=] __device__(i64 r) { const int v = rv1[r]; return (i64)(OFF[v + 1] - OFF[v]); },
                             xoff.get(), R);
-    } else {
+    } else if (S == 5) {
         scan_exclusive<i64>(
             ctx,
             [=] __device__(i64 r) {
@@ -745,13 +747,16 @@ void run_stage_packed(Ctx& ctx, const Csr& G, const int* rv1, const int* rv2, i6
     if (X == 0) return;
     const i64 ntiles = (X + kCycTile - 1) / kCycTile;
     RAMA_REQUIRE(ntiles < ((i64)1 << 31), "too many cycle expansions");
-    DBuf<int> tile_rep = ctx.buf<int>(ntiles);
-    k_tile_rep<<<grid_for(R), kBlock, 0, ctx.stream>>>(xoff.get(), R, tile_rep.get());
-    RAMA_LAUNCHED();
+    DBuf<int> tile_rep;
+    if (S > 3) {
+        tile_rep = ctx.buf<int>(ntiles);
+        k_tile_rep<<<grid_for(R), kBlock, 0, ctx.stream>>>(xoff.get(), R, tile_rep.get());
+        RAMA_LAUNCHED();
+    }
     DBuf<i64> slot = ctx.buf<i64>(2 * (i64)Q + 1);  // [0, Q) totals, [Q, 2Q] their exclusive scan
     i64* TOT = slot.get();
     ctx.zero(TOT, Q);
-    CycArgs a{OFF, G.adj.get(), rv1, rv2, xoff.get(), tile_rep.get(), R, X};
+    CycArgs a{OFF, G.adj.get(), rv1, rv2, S > 3 ? xoff.get() : nullptr, S > 3 ? tile_rep.get() : nullptr, R, X};
     a.nslot = nslot;
     a.slot_total = TOT;
     a.slot_first = TOT + Q;
