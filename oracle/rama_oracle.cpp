@@ -414,23 +414,42 @@ struct MP {
         normalize_triangles(a1, a2, a3);
         t1 = a1; t2 = a2; t3 = a3;
         const int64_t T = (int64_t)t1.size();
-        std::vector<uint64_t> keys;
-        keys.reserve(A.nnz() + 3 * T);
-        for (size_t e = 0; e < A.nnz(); ++e) keys.push_back(key2(A.row[e], A.col[e]));
+        // triangle edges, sorted and unique (:81-99)
+        std::vector<uint64_t> tk;
+        tk.reserve(3 * T);
         for (int64_t t = 0; t < T; ++t) {
-            keys.push_back(key2(t1[t], t2[t]));
-            keys.push_back(key2(t1[t], t3[t]));
-            keys.push_back(key2(t2[t], t3[t]));
+            tk.push_back(key2(t1[t], t2[t]));
+            tk.push_back(key2(t1[t], t3[t]));
+            tk.push_back(key2(t2[t], t3[t]));
         }
-        std::sort(keys.begin(), keys.end());
-        keys.erase(std::unique(keys.begin(), keys.end()), keys.end());
+        std::sort(tk.begin(), tk.end());
+        tk.erase(std::unique(tk.begin(), tk.end()), tk.end());
+        // set_union_by_key(A, triangle edges) (:131-156): every entry of A is
+        // kept with its own cost, repeated (row, col) pairs included, and the
+        // triangle edges absent from A are appended with cost 0; A is sorted
+        // by (row, col) and the union's stable re-sort keeps repeats in A's order.
+        std::vector<uint64_t> keys;
+        keys.reserve(A.nnz() + tk.size());
+        size_t a = 0, b = 0;
+        const size_t na = A.nnz(), nt = tk.size();
+        while (a < na || b < nt) {
+            const uint64_t ka = a < na ? key2(A.row[a], A.col[a]) : ~0ull;
+            if (a < na && (b == nt || ka <= tk[b])) {
+                if (b < nt && ka == tk[b]) ++b;
+                keys.push_back(ka);
+                c.push_back(A.val[a]);
+                ++a;
+            } else {
+                keys.push_back(tk[b++]);
+                c.push_back(0.0f);
+            }
+        }
         const size_t E = keys.size();
         i.resize(E); j.resize(E);
-        c.assign(E, 0.0f);
         for (size_t e = 0; e < E; ++e) { i[e] = (int)(keys[e] >> 32); j[e] = (int)(keys[e] & 0xffffffffu); }
+        // a triangle edge corresponds to the first entry carrying its key
+        // (set_intersection_by_key in compute_triangle_edge_correspondence, :46-50)
         auto find = [&](uint64_t k) { return (int)(std::lower_bound(keys.begin(), keys.end(), k) - keys.begin()); };
-        // original edge costs: set_union_by_key takes A's value for A's keys
-        for (size_t e = 0; e < A.nnz(); ++e) c[find(key2(A.row[e], A.col[e]))] = A.val[e];
         c12.resize(T); c13.resize(T); c23.resize(T);
         cnt.assign(E, 0);
         for (int64_t t = 0; t < T; ++t) {

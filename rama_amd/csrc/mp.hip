@@ -26,33 +26,34 @@ namespace rama {
 __device__ __forceinline__ float edge_share(float c, int cnt) { return c / float(cnt); }
 
 
+// out[i] = lower_bound(a[0..n), q[i]) for ascending queries q[0..m).  With
+// kRepUpper a query equal to its predecessor takes upper_bound instead (the
+// answers stay ascending).  Each block bounds its 256 answers by the answers
+// of its first and last query (two full searches); a window of at most
+// kLbWin keys is staged in LDS with coalesced loads and searched there, a
+// wider one in place.
+constexpr int kLbWin = 2048;
+template <class K, bool kRepUpper>
+__device__ __forceinline__ bool lb_upper(const K* __restrict__ q, i64 i) {
+    return kRepUpper && i > 0 && q[i - 1] == q[i];
+}
 template <class K>
-__device__ __forceinline__ i64 lower_bound_dev(const K* a, i64 lo, i64 hi, K x) {
+__device__ __forceinline__ i64 bound_dev(const K* a, i64 lo, i64 hi, K x, bool upper) {
     while (lo < hi) {
         const i64 mid = (lo + hi) >> 1;
-        if (a[mid] < x) lo = mid + 1; else hi = mid;
+        if (a[mid] < x || (upper && a[mid] == x)) lo = mid + 1; else hi = mid;
     }
     return lo;
 }
-template <class K>
-__device__ __forceinline__ i64 lower_bound_dev(const K* a, i64 n, K x) {
-    return lower_bound_dev(a, (i64)0, n, x);
-}
-
-// out[i] = lower_bound(a[0..n), q[i]) for ascending queries q[0..m).  Each
-// block bounds its 256 answers by the answers of its first and last query
-// (two full searches); a window of at most kLbWin keys is staged in LDS with
-// coalesced loads and searched there, a wider one in place.
-constexpr int kLbWin = 2048;
-template <class K>
+template <class K, bool kRepUpper>
 __global__ __launch_bounds__(kBlock) void k_sorted_lb(const K* __restrict__ a, i64 n, const K* __restrict__ q, i64 m,
                                                       int* __restrict__ out) {
     __shared__ i64 range[2];
     __shared__ K sa[kLbWin];
     const i64 b0 = (i64)blockIdx.x * kBlock;
     const i64 b1 = (b0 + kBlock < m ? b0 + kBlock : m) - 1;
-    if (threadIdx.x == 0) range[0] = lower_bound_dev(a, n, q[b0]);
-    if (threadIdx.x == 64) range[1] = lower_bound_dev(a, n, q[b1]);
+    if (threadIdx.x == 0) range[0] = bound_dev(a, 0, n, q[b0], lb_upper<K, kRepUpper>(q, b0));
+    if (threadIdx.x == 64) range[1] = bound_dev(a, 0, n, q[b1], lb_upper<K, kRepUpper>(q, b1));
     __syncthreads();
     const i64 r0 = range[0], r1 = range[1];
     const i64 i = b0 + threadIdx.x;
@@ -62,31 +63,36 @@ __global__ __launch_bounds__(kBlock) void k_sorted_lb(const K* __restrict__ a, i
         __syncthreads();
         if (i <= b1) {
             const K x = q[i];
+            const bool up = lb_upper<K, kRepUpper>(q, i);
             int lo = 0, hi = w;
             while (lo < hi) {
                 const int mid = (lo + hi) >> 1;
-                if (sa[mid] < x) lo = mid + 1; else hi = mid;
+                if (sa[mid] < x || (up && sa[mid] == x)) lo = mid + 1; else hi = mid;
             }
             out[i] = (int)(r0 + lo);
         }
     } else if (i <= b1) {
-        out[i] = (int)lower_bound_dev(a, r0, r1, q[i]);
+        out[i] = (int)bound_dev(a, r0, r1, q[i], lb_upper<K, kRepUpper>(q, i));
     }
 }
 
-template <class K>
+template <class K, bool kRepUpper = false>
 static void sorted_lb(Ctx& ctx, const K* a, i64 n, const K* q, i64 m, int* out) {
     if (m <= 0) return;
-    k_sorted_lb<K><<<grid_for(m), kBlock, 0, ctx.stream>>>(a, n, q, m, out);
+    k_sorted_lb<K, kRepUpper><<<grid_for(m), kBlock, 0, ctx.stream>>>(a, n, q, m, out);
     RAMA_LAUNCHED();
 }
 
 // Builds the message-passing problem.  The graph's edges are already sorted
-// and unique; only the 3T triangle-edge items (key, slot 3t+k) are sorted.
-// Triangle keys absent from the graph are the chords.  In the merged edge
-// list a key's position is (#graph keys below it) + (#chord keys below it);
-// an edge's slots are the sorted triangle items carrying its key, which are
-// contiguous and already in (triangle, slot) order (the sort is stable).
+// (a (row, col) pair may repeat); only the 3T triangle-edge items (key, slot
+// 3t+k) are sorted.  Triangle keys absent from the graph are the chords.  In
+// the merged edge list a graph entry's position is (#graph entries before it)
+// + (#chord keys below it), a chord's (#graph entries below it) + (#chords
+// before it); an edge's slots are the sorted triangle items carrying its key,
+// which are contiguous and already in (triangle, slot) order (the sort is
+// stable).  Every graph entry is kept with its own cost and a repeated pair's
+// triangles all go to its first entry (set_union_by_key and
+// set_intersection_by_key in src/multicut_message_passing.cu:46-50,131-156).
 template <class K>
 static void mp_build_impl(Ctx& ctx, const DCoo& A, Triangles& tri, MPState& S, int kb) {
     const i64 E = A.nnz, T = tri.n, N3 = 3 * T;
@@ -123,7 +129,9 @@ static void mp_build_impl(Ctx& ctx, const DCoo& A, Triangles& tri, MPState& S, i
     int* PA = pa.get();
     int* PT = pt.get();
     sorted_lb<K>(ctx, KA, E, KT, N3, PA);
-    sorted_lb<K>(ctx, KT, N3, KA, E, PT);
+    // a repeat of a graph pair searches past the pair's items: its run of
+    // slots is empty and the first entry owns them
+    sorted_lb<K, true>(ctx, KT, N3, KA, E, PT);
     // chords: first item of a key run whose key is not a graph edge
     auto is_chord = [=] __device__(i64 q) {
         if (q > 0 && KT[q] == KT[q - 1]) return 0;
@@ -168,7 +176,8 @@ static void mp_build_impl(Ctx& ctx, const DCoo& A, Triangles& tri, MPState& S, i
     // Places in the merged list come from the scan of the chord flags (CP,
     // N3 + 1 entries): the chords below a key are the chord heads before its
     // first triangle item.
-    // graph edges: the chords below KA[e] are the heads before PT[e]
+    // graph edges: the chords below KA[e] are the heads before PT[e] (no
+    // chord head lies inside the items of a graph key)
     for_each(ctx, E, [=] __device__(i64 e) {
         const i64 u = e + CP[PT[e]];
         EI[u] = AR[e];

@@ -193,6 +193,26 @@ def test_message_passing_bound_is_monotone_and_valid(oracle):
         assert lbs[-1] <= brute_optimum(n, i, j, c) + 1e-5
 
 
+@pytest.mark.parametrize("rep_cost", [5.0, -1.5])
+def test_repeated_pair_stays_outside_the_triangles(oracle, rep_cost):
+    # thrust's set_union_by_key keeps all m copies of a key repeated in the
+    # first range and set_intersection_by_key matches the first of them, so a
+    # repeated (row, col) pair survives message passing as two edges: the
+    # first one carries the pair's triangles, the repeat keeps its own cost
+    # and only adds min(0, cost) to the bound.
+    base = (np.array([0, 0, 1], np.int32), np.array([1, 2, 2], np.int32), np.array([2.0, 3.0, -4.0], np.float32))
+    rep = (np.array([0, 0, 0, 1], np.int32), np.array([1, 1, 2, 2], np.int32),
+           np.array([2.0, rep_cost, 3.0, -4.0], np.float32))
+    t = [np.array([0], np.int32), np.array([1], np.int32), np.array([2], np.int32)]
+    bi, bj, bc, blbs = oracle.message_passing(*base, *t, 0, 3)
+    ri, rj, rc, rlbs = oracle.message_passing(*rep, *t, 0, 3)
+    assert list(ri) == [0, 0, 0, 1] and list(rj) == [1, 1, 2, 2]
+    assert rc[1] == np.float32(rep_cost)
+    np.testing.assert_array_equal(rc[[0, 2, 3]], bc)
+    assert bc[0] != np.float32(2.0)  # the first entry did take part
+    np.testing.assert_allclose(rlbs, blbs + min(0.0, rep_cost), rtol=0, atol=1e-9)
+
+
 def test_partition_hash_is_relabel_invariant():
     import importlib.util
     import os
