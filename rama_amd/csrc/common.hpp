@@ -246,12 +246,18 @@ struct Ctx {
 
     int64_t syncs = 0;  // host round trips (diagnostic)
 
-    // Single-pass scan state: per-tile status words tagged with an epoch so
-    // no launch needs the buffer cleared; a tile counter that the last tile
-    // resets; a device error flag raised by a look-back that gave up.
+    // Onesweep look-back state: per-tile status words tagged with an epoch
+    // so no launch needs the buffer cleared; a tile counter that the last
+    // tile resets; a device error flag raised by a look-back that gave up.
     DBuf<uint64_t> scan_status;
     DBuf<int> scan_counter;  // [0] tile counter, [1] error flag
     uint32_t scan_epoch = 0;
+    // Partials of the two-launch scan.
+    DBuf<uint64_t> scan_partials;
+    void* scan_scratch(size_t bytes) {
+        if ((size_t)scan_partials.size() * 8 < bytes) scan_partials = DBuf<uint64_t>(arena, (int64_t)(bytes / 8 + 1));
+        return scan_partials.get();
+    }
     // Reduction partials and the ticket counter (reset by the last block).
     DBuf<uint64_t> red_partials;
     DBuf<unsigned> red_ticket;

@@ -111,16 +111,10 @@ struct OpMax {
 };
 
 // ------------------------------------------------------------------ scan
-// Single-pass exclusive scan of non-negative integer sums with decoupled
-// look-back.  Each block takes the next tile index from an atomic counter,
-// so every smaller tile belongs to a block that is already running; it
-// publishes its aggregate, walks back over predecessors until one carries an
-// inclusive prefix, and publishes its own inclusive prefix.  Status words
-// pack (epoch:16 | flag:2 | value:46); a word from another epoch reads as
-// "not ready".  A look-back that spins too long raises ctx's error flag
-// (checked by scan_check) instead of hanging.
-constexpr int kScanItems = 8;
-constexpr int kScanTile = kBlock * kScanItems;
+// Status words of the onesweep sort's decoupled look-back pack
+// (epoch:16 | flag:2 | value:46); a word from another epoch reads as "not
+// ready".  A look-back that spins too long raises ctx's error flag (checked
+// by scan_check) instead of hanging.
 constexpr u64 kScanValMask = (1ull << 46) - 1;
 constexpr long kScanSpinLimit = 1l << 24;
 
@@ -128,8 +122,16 @@ __device__ __forceinline__ u64 scan_word(u32 epoch, u32 flag, u64 v) {
     return ((u64)epoch << 48) | ((u64)flag << 46) | (v & kScanValMask);
 }
 
-// out(i, exclusive_prefix, value) is called for every i < n; total (if not
-// null) receives the sum of all values.
+// Exclusive scan of non-negative integer sums, reduce-then-scan over a
+// bounded grid: block b owns a contiguous chunk of whole tiles; the first
+// launch stores each chunk's sum, the second adds the sums of the chunks
+// before its own (a block reduction over at most kScanMaxBlocks partials)
+// and scans its chunk tile by tile carrying the running prefix.  No atomics
+// and no waiting on other blocks.  One chunk: the second launch only.
+constexpr int kScanItems = 8;
+constexpr int kScanTile = kBlock * kScanItems;
+constexpr unsigned kScanMaxBlocks = 2048;
+
 // Items are evaluated and emitted in striped order (item k*kBlock + t of the
 // tile by thread t), so the loads inside f and the stores inside out are
 // coalesced across each wave; the scan itself runs on the blocked order
@@ -138,104 +140,74 @@ __device__ __forceinline__ u64 scan_word(u32 epoch, u32 flag, u64 v) {
 constexpr int kScanPad = kScanItems + 1;
 __device__ __forceinline__ int scan_slot(int j) { return (j / kScanItems) * kScanPad + j % kScanItems; }
 
-template <class T, class F, class Out>
-__global__ __launch_bounds__(kBlock) void k_scan_single(F f, i64 n, Out out, T* total, u64* status, int* ctl,
-                                                        u32 epoch) {
+template <class T, class F>
+__global__ __launch_bounds__(kBlock) void k_scan_up(F f, i64 n, i64 chunk, T* __restrict__ partials) {
     __shared__ T ws[kWaves];
-    __shared__ int tile_s;
-    __shared__ T prefix_s;
-    __shared__ T sv[kBlock * kScanPad];
-    if (threadIdx.x == 0) tile_s = atomicAdd(&ctl[0], 1);
-    __syncthreads();
-    const int tile = tile_s;
-    const i64 tile0 = (i64)tile * kScanTile;
-    T v[kScanItems];
-#pragma unroll
-    for (int k = 0; k < kScanItems; ++k) {
-        const int j = k * kBlock + threadIdx.x;
-        const i64 i = tile0 + j;
-        v[k] = i < n ? (T)f(i) : T(0);
-        sv[scan_slot(j)] = v[k];
-    }
-    __syncthreads();
+    const i64 c0 = (i64)blockIdx.x * chunk;
+    const i64 c1 = c0 + chunk < n ? c0 + chunk : n;
     T s = T(0);
-    T bl[kScanItems];
+    i64 i = c0 + threadIdx.x;
+    for (; i + (kScanItems - 1) * kBlock < c1; i += kScanItems * kBlock) {
+        T x[kScanItems];
 #pragma unroll
-    for (int k = 0; k < kScanItems; ++k) {
-        bl[k] = sv[threadIdx.x * kScanPad + k];
-        s += bl[k];
+        for (int k = 0; k < kScanItems; ++k) x[k] = (T)f(i + k * kBlock);
+#pragma unroll
+        for (int k = 0; k < kScanItems; ++k) s += x[k];
     }
-    T agg;
-    const T excl = block_excl_scan(s, ws, &agg);
-    if (threadIdx.x < 64) {
-        // wave 0: publish the aggregate, then look back 64 predecessors per
-        // step; lane l reads tile p - l.  The nearest inclusive prefix ends
-        // the walk once every tile before it in the window is ready.
-        const int lane = threadIdx.x;
-        T prefix = T(0);
-        if (tile == 0) {
-            if (lane == 0)
-                __hip_atomic_store(&status[0], scan_word(epoch, 2, (u64)agg), __ATOMIC_RELAXED,
-                                   __HIP_MEMORY_SCOPE_AGENT);
-        } else {
-            if (lane == 0)
-                __hip_atomic_store(&status[tile], scan_word(epoch, 1, (u64)agg), __ATOMIC_RELAXED,
-                                   __HIP_MEMORY_SCOPE_AGENT);
-            int p = tile - 1;
-            long spins = 0;
-            while (true) {
-                const int q = p - lane;
-                u64 w = q >= 0 ? __hip_atomic_load(&status[q], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)
-                               : scan_word(epoch, 2, 0);
-                const u32 fl = (u32)(w >> 46) & 3u;
-                const bool ready = (u32)(w >> 48) == epoch && fl != 0;
-                const unsigned long long incl = __ballot(ready && fl == 2);
-                const unsigned long long notready = __ballot(!ready);
-                const int k = incl ? __ffsll((long long)incl) - 1 : 63;
-                const unsigned long long need = k == 63 ? ~0ull : ((2ull << k) - 1ull);
-                if (notready & need) {
-                    if (++spins > kScanSpinLimit) {
-                        if (lane == 0) ctl[1] = 1;
-                        break;
-                    }
-                    __builtin_amdgcn_s_sleep(1);
-                    continue;
-                }
-                T x = lane <= k ? (T)(w & kScanValMask) : T(0);
+    for (; i < c1; i += kBlock) s += (T)f(i);
+    const T r = block_reduce(s, OpSum(), ws);
+    if (threadIdx.x == 0) partials[blockIdx.x] = r;
+}
+
+// out(i, exclusive_prefix, value) is called for every i < n; total (if not
+// null) receives the sum of all values.
+template <class T, class F, class Out>
+__global__ __launch_bounds__(kBlock) void k_scan_down(F f, i64 n, i64 chunk, Out out,
+                                                      const T* __restrict__ partials, T* total) {
+    __shared__ T ws[kWaves];
+    __shared__ T sv[kBlock * kScanPad];
+    T p = T(0);
+    for (unsigned b = threadIdx.x; b < blockIdx.x; b += kBlock) p += partials[b];
+    T run = block_reduce(p, OpSum(), ws);
+    const i64 c0 = (i64)blockIdx.x * chunk;
+    const i64 c1 = c0 + chunk < n ? c0 + chunk : n;
+    for (i64 tile0 = c0; tile0 < c1; tile0 += kScanTile) {
+        T v[kScanItems];
 #pragma unroll
-                for (int d = 32; d >= 1; d >>= 1) x += shfl_down64(x, d);
-                x = __shfl(x, 0, 64);
-                prefix += x;
-                if (incl) break;
-                p -= 64;
-            }
-            if (lane == 0)
-                __hip_atomic_store(&status[tile], scan_word(epoch, 2, (u64)(prefix + agg)), __ATOMIC_RELAXED,
-                                   __HIP_MEMORY_SCOPE_AGENT);
+        for (int k = 0; k < kScanItems; ++k) {
+            const int j = k * kBlock + threadIdx.x;
+            const i64 i = tile0 + j;
+            v[k] = i < c1 ? (T)f(i) : T(0);
+            sv[scan_slot(j)] = v[k];
         }
-        if (lane == 0) {
-            prefix_s = prefix;
-            if (tile == (int)gridDim.x - 1) {
-                if (total) *total = prefix + agg;
-                ctl[0] = 0;  // every block has drawn its tile: reset for the next launch
-            }
+        __syncthreads();
+        T s = T(0);
+        T bl[kScanItems];
+#pragma unroll
+        for (int k = 0; k < kScanItems; ++k) {
+            bl[k] = sv[threadIdx.x * kScanPad + k];
+            s += bl[k];
         }
-    }
-    __syncthreads();
-    // blocked exclusive prefixes back into LDS, then emit in striped order
-    T run = prefix_s + excl;
+        T agg;
+        const T excl = block_excl_scan(s, ws, &agg);  // its barriers order the reads above before the writes below
+        // blocked exclusive prefixes back into LDS, then emit in striped order
+        T r = run + excl;
 #pragma unroll
-    for (int k = 0; k < kScanItems; ++k) {
-        sv[threadIdx.x * kScanPad + k] = run;
-        run += bl[k];
-    }
-    __syncthreads();
+        for (int k = 0; k < kScanItems; ++k) {
+            sv[threadIdx.x * kScanPad + k] = r;
+            r += bl[k];
+        }
+        __syncthreads();
 #pragma unroll
-    for (int k = 0; k < kScanItems; ++k) {
-        const int j = k * kBlock + threadIdx.x;
-        const i64 i = tile0 + j;
-        if (i < n) out(i, sv[scan_slot(j)], v[k]);
+        for (int k = 0; k < kScanItems; ++k) {
+            const int j = k * kBlock + threadIdx.x;
+            const i64 i = tile0 + j;
+            if (i < c1) out(i, sv[scan_slot(j)], v[k]);
+        }
+        run += agg;
+        __syncthreads();  // the next tile overwrites sv
     }
+    if (total && threadIdx.x == 0 && blockIdx.x == gridDim.x - 1) *total = run;
 }
 
 template <class T>
@@ -250,7 +222,7 @@ struct StoreExcl {
     __device__ __forceinline__ void operator()(i64 i, T run, T) const { p[i] = run; }
 };
 
-// Generic single-pass scan: out(i, prefix, value) for all i < n; *total on device.
+// Generic scan: out(i, prefix, value) for all i < n; *total on device.
 template <class T, class F, class Out>
 void scan_apply(Ctx& ctx, F f, i64 n, Out out, T* total) {
     if (n <= 0) {
@@ -258,16 +230,23 @@ void scan_apply(Ctx& ctx, F f, i64 n, Out out, T* total) {
         return;
     }
     const i64 nt = (n + kScanTile - 1) / kScanTile;
-    RAMA_REQUIRE(nt < ((i64)1 << 31), "scan too large");
-    u32 epoch;
-    u64* st = ctx.scan_begin(nt, &epoch);
-    k_scan_single<T, F, Out><<<(unsigned)nt, kBlock, 0, ctx.stream>>>(f, n, out, total, st, ctx.scan_counter.get(),
-                                                                      epoch);
+    const i64 per = (nt + kScanMaxBlocks - 1) / kScanMaxBlocks;  // tiles per block
+    const i64 chunk = per * kScanTile;
+    const unsigned g = (unsigned)((n + chunk - 1) / chunk);
+    T* partials = nullptr;
+    if (g > 1) {
+        static_assert(sizeof(T) <= 8, "scan type");
+        partials = static_cast<T*>(ctx.scan_scratch(kScanMaxBlocks * 8));
+        k_scan_up<T, F><<<g, kBlock, 0, ctx.stream>>>(f, n, chunk, partials);
+        RAMA_LAUNCHED();
+    }
+    k_scan_down<T, F, Out><<<g, kBlock, 0, ctx.stream>>>(f, n, chunk, out, partials, total);
     RAMA_LAUNCHED();
 }
 
 // Exclusive scan of f(0..n-1) into out[0..n]; out[n] = total.  `out` may
-// alias the storage f reads (each tile reads all its items before writing).
+// alias the storage f reads (item i is written only by the block that owns
+// it, after the first launch and after its tile has read all its items).
 template <class T, class F>
 void scan_exclusive(Ctx& ctx, F f, T* out, i64 n) {
     if (n <= 0) {
@@ -291,34 +270,45 @@ inline void scan_check(Ctx& ctx) {
 }
 
 // ------------------------------------------------------------- reduction
-// One launch: every block reduces its grid-stride share and stores the
-// partial; the last block to finish (by an atomic ticket) combines the
-// partials in block order, so the result is deterministic for a given n.
+// Every block reduces its grid-stride share and stores the partial; a
+// second one-block launch combines the partials in block order, so the
+// result is deterministic for a given n.  A single block combines its own
+// partial the same way, in the same launch.  No tickets: same-address
+// device atomics from every block serialise.
 constexpr unsigned kReduceMaxBlocks = 1024;
 
+// a = op(ident, partials[b]) over b ≡ thread (mod kBlock), then the block
+// tree: the combination of the partials in block order.
+template <class T, class Op>
+__device__ __forceinline__ T combine_partials(const T* partials, unsigned nb, T ident, Op op, T* ws) {
+    T a = ident;
+    for (unsigned b = threadIdx.x; b < nb; b += kBlock) a = op(a, partials[b]);
+    return block_reduce(a, op, ws);
+}
+
 template <class T, class Op, class F>
-__global__ __launch_bounds__(kBlock) void k_reduce1(F f, i64 n, T ident, Op op, T* partials, unsigned* ticket,
-                                                    T* dst) {
+__global__ __launch_bounds__(kBlock) void k_reduce1(F f, i64 n, T ident, Op op, T* partials, T* dst) {
     __shared__ T ws[kWaves];
-    __shared__ bool last;
+    __shared__ T own;
     T acc = ident;
     for (i64 i = (i64)blockIdx.x * kBlock + threadIdx.x; i < n; i += (i64)gridDim.x * kBlock) acc = op(acc, (T)f(i));
     const T r = block_reduce(acc, op, ws);
-    if (threadIdx.x == 0) {
-        __hip_atomic_store(&partials[blockIdx.x], r, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        const unsigned t = __hip_atomic_fetch_add(ticket, 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
-        last = t == gridDim.x - 1;
+    if (gridDim.x > 1) {
+        if (threadIdx.x == 0) partials[blockIdx.x] = r;
+        return;
     }
+    if (threadIdx.x == 0) own = r;
     __syncthreads();
-    if (!last) return;
-    __threadfence();  // acquire: the other blocks' partials are visible to plain loads
-    T a = ident;
-    for (unsigned b = threadIdx.x; b < gridDim.x; b += kBlock) a = op(a, partials[b]);
-    const T total = block_reduce(a, op, ws);
-    if (threadIdx.x == 0) {
-        *dst = total;
-        *ticket = 0;  // every block has taken its ticket: reset for the next launch
-    }
+    const T total = combine_partials(&own, 1u, ident, op, ws);
+    if (threadIdx.x == 0) *dst = total;
+}
+
+template <class T, class Op>
+__global__ __launch_bounds__(kBlock) void k_reduce_combine(const T* __restrict__ partials, unsigned nb, T ident,
+                                                           Op op, T* dst) {
+    __shared__ T ws[kWaves];
+    const T total = combine_partials(partials, nb, ident, op, ws);
+    if (threadIdx.x == 0) *dst = total;
 }
 
 // Deterministic reduction (fixed grid, fixed tree); result stays on device in *dst.
@@ -326,10 +316,13 @@ template <class T, class Op, class F>
 void reduce_to(Ctx& ctx, F f, i64 n, T ident, Op op, T* dst) {
     const unsigned nb = (unsigned)std::min<i64>(std::max<i64>((n + kBlock - 1) / kBlock, 1), kReduceMaxBlocks);
     static_assert(sizeof(T) <= 8, "reduction type");
-    void* scratch = ctx.reduce_scratch(kReduceMaxBlocks * 8);
-    k_reduce1<T, Op, F><<<nb, kBlock, 0, ctx.stream>>>(f, n, ident, op, static_cast<T*>(scratch),
-                                                       ctx.reduce_ticket(), dst);
+    T* scratch = static_cast<T*>(ctx.reduce_scratch(kReduceMaxBlocks * 8));
+    k_reduce1<T, Op, F><<<nb, kBlock, 0, ctx.stream>>>(f, n, ident, op, scratch, dst);
     RAMA_LAUNCHED();
+    if (nb > 1) {
+        k_reduce_combine<T, Op><<<1, kBlock, 0, ctx.stream>>>(scratch, nb, ident, op, dst);
+        RAMA_LAUNCHED();
+    }
 }
 
 // Two double sums in one pass, each reduced exactly as reduce_to would
@@ -337,10 +330,9 @@ void reduce_to(Ctx& ctx, F f, i64 n, T ident, Op op, T* dst) {
 // combination), so the results are bit-identical to two separate calls.
 // f(i, a, b) gives element i's two terms.
 template <class F>
-__global__ __launch_bounds__(kBlock) void k_reduce_pair(F f, i64 n, double* partials, unsigned* ticket,
-                                                        double* dst_a, double* dst_b) {
+__global__ __launch_bounds__(kBlock) void k_reduce_pair(F f, i64 n, double* partials, double* dst_a, double* dst_b) {
     __shared__ double ws[kWaves];
-    __shared__ bool last;
+    __shared__ double own[2];
     double a = 0.0, b = 0.0;
     for (i64 i = (i64)blockIdx.x * kBlock + threadIdx.x; i < n; i += (i64)gridDim.x * kBlock) {
         double x, y;
@@ -350,26 +342,34 @@ __global__ __launch_bounds__(kBlock) void k_reduce_pair(F f, i64 n, double* part
     }
     const double ra = block_reduce(a, OpSum(), ws);
     const double rb = block_reduce(b, OpSum(), ws);
+    if (gridDim.x > 1) {
+        if (threadIdx.x == 0) {
+            partials[blockIdx.x] = ra;
+            partials[kReduceMaxBlocks + blockIdx.x] = rb;
+        }
+        return;
+    }
     if (threadIdx.x == 0) {
-        __hip_atomic_store(&partials[blockIdx.x], ra, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        __hip_atomic_store(&partials[kReduceMaxBlocks + blockIdx.x], rb, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        const unsigned t = __hip_atomic_fetch_add(ticket, 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
-        last = t == gridDim.x - 1;
+        own[0] = ra;
+        own[1] = rb;
     }
     __syncthreads();
-    if (!last) return;
-    __threadfence();  // acquire: the other blocks' partials are visible to plain loads
-    double sa = 0.0, sb = 0.0;
-    for (unsigned k = threadIdx.x; k < gridDim.x; k += kBlock) {
-        sa += partials[k];
-        sb += partials[kReduceMaxBlocks + k];
-    }
-    const double ta = block_reduce(sa, OpSum(), ws);
-    const double tb = block_reduce(sb, OpSum(), ws);
+    const double ta = combine_partials(own, 1u, 0.0, OpSum(), ws);
+    const double tb = combine_partials(own + 1, 1u, 0.0, OpSum(), ws);
     if (threadIdx.x == 0) {
         *dst_a = ta;
         *dst_b = tb;
-        *ticket = 0;
+    }
+}
+
+__global__ __launch_bounds__(kBlock) inline void k_reduce_pair_combine(const double* __restrict__ partials, unsigned nb,
+                                                                       double* dst_a, double* dst_b) {
+    __shared__ double ws[kWaves];
+    const double ta = combine_partials(partials, nb, 0.0, OpSum(), ws);
+    const double tb = combine_partials(partials + kReduceMaxBlocks, nb, 0.0, OpSum(), ws);
+    if (threadIdx.x == 0) {
+        *dst_a = ta;
+        *dst_b = tb;
     }
 }
 
@@ -377,8 +377,12 @@ template <class F>
 void reduce_pair_to(Ctx& ctx, F f, i64 n, double* dst_a, double* dst_b) {
     const unsigned nb = (unsigned)std::min<i64>(std::max<i64>((n + kBlock - 1) / kBlock, 1), kReduceMaxBlocks);
     double* scratch = static_cast<double*>(ctx.reduce_scratch(2 * kReduceMaxBlocks * 8));
-    k_reduce_pair<F><<<nb, kBlock, 0, ctx.stream>>>(f, n, scratch, ctx.reduce_ticket(), dst_a, dst_b);
+    k_reduce_pair<F><<<nb, kBlock, 0, ctx.stream>>>(f, n, scratch, dst_a, dst_b);
     RAMA_LAUNCHED();
+    if (nb > 1) {
+        k_reduce_pair_combine<<<1, kBlock, 0, ctx.stream>>>(scratch, nb, dst_a, dst_b);
+        RAMA_LAUNCHED();
+    }
 }
 
 template <class T, class Op, class F>
