@@ -233,6 +233,7 @@ struct Ctx {
     }
     ~Ctx() {
         if (stream) (void)hipStreamSynchronize(stream);
+        if (sync_ev) (void)hipEventDestroy(sync_ev);
         arena.trim();
         pool::give_pinned(pinned);
         if (up_pinned) (void)hipHostFree(up_pinned);
@@ -292,10 +293,24 @@ struct Ctx {
         *epoch = scan_epoch;
         return scan_status.get();
     }
+    // Host waits go through an event recorded on the stream and created
+    // without hipEventBlockingSync, which the runtime waits on actively;
+    // RAMA_SYNC_MODE=stream waits on the stream instead.
+    hipEvent_t sync_ev = nullptr;
     void sync(const char* fn = __builtin_FUNCTION(), int line = __builtin_LINE()) {
         ++syncs;
         if (sync_sites_on()) note_sync_site(fn, line);
-        RAMA_HIP(hipStreamSynchronize(stream));
+        static const bool by_stream = [] {
+            const char* e = std::getenv("RAMA_SYNC_MODE");
+            return e && std::strcmp(e, "stream") == 0;
+        }();
+        if (by_stream) {
+            RAMA_HIP(hipStreamSynchronize(stream));
+            return;
+        }
+        if (!sync_ev) RAMA_HIP(hipEventCreateWithFlags(&sync_ev, hipEventDisableTiming));
+        RAMA_HIP(hipEventRecord(sync_ev, stream));
+        RAMA_HIP(hipEventSynchronize(sync_ev));
     }
 
     // Host-to-device upload through a pinned staging ring, asynchronous on

@@ -248,14 +248,26 @@ __device__ __forceinline__ int count_vs_regs_fast(const int* __restrict__ adj, i
     if (rdup) return count_vs_regs(adj, ia, ie, rb);
     const int lane = lane_id();
     int c = 0, prev = -1;
-    for (int k = ia; k < ie; ++k) {
-        const int x = adj[k];
-        if (x > rmax) break;
-        unsigned long long hit = 0;
+    // four streamed entries per round of loads (past the end reads as
+    // above rmax), walked in order up to the first entry above rmax
+    for (int k = ia; k < ie; k += 4) {
+        int x[4];
 #pragma unroll
-        for (int q = 0; q < kRegList; ++q) hit |= __ballot(rb[q] == x);
-        c += (int)((hit >> lane) & 1ull) & (int)(x != prev);
-        prev = x;
+        for (int j = 0; j < 4; ++j) x[j] = k + j < ie ? adj[k + j] : 0x7fffffff;
+        bool stop = false;
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            if (x[j] > rmax) {
+                stop = true;
+                break;
+            }
+            unsigned long long hit = 0;
+#pragma unroll
+            for (int q = 0; q < kRegList; ++q) hit |= __ballot(rb[q] == x[j]);
+            c += (int)((hit >> lane) & 1ull) & (int)(x[j] != prev);
+            prev = x[j];
+        }
+        if (stop) break;
     }
     return c;
 }
@@ -417,15 +429,21 @@ __global__ __launch_bounds__(kBlock) void k_cycle_count5(CycArgs a, const i64* _
         reg_list_props(rb, nq, rmax, rdup);
     }
     TileAcc acc;
+    // the next n1 and its list bounds are loaded while this one is walked
+    int n1 = d1 > 0 ? a.adj[b1] : 0;
+    int ia = d1 > 0 ? a.off[n1] : 0, ie = d1 > 0 ? a.off[n1 + 1] : 0;
     for (int l = 0; l < d1; ++l) {
-        const int n1 = a.adj[b1 + l];
+        const int nn = l + 1 < d1 ? a.adj[b1 + l + 1] : 0;
+        const int na = l + 1 < d1 ? a.off[nn] : 0, ne = l + 1 < d1 ? a.off[nn + 1] : 0;
         int c = 0;
         if (n1 != n2)
-            c = regs ? count_vs_regs_fast(a.adj, a.off[n1], a.off[n1 + 1], rb, rmax, rdup)
-                     : count_common(a.off, a.adj, n1, n2);
+            c = regs ? count_vs_regs_fast(a.adj, ia, ie, rb, rmax, rdup) : count_common(a.off, a.adj, n1, n2);
         cnt8[x0 + l] = (unsigned char)(c < 255 ? c : 255);
         acc.add(tile_tot, x0 + l, 3 * (i64)c);
         mine += 3 * (i64)c;
+        n1 = nn;
+        ia = na;
+        ie = ne;
     }
     acc.flush(tile_tot);
     xend = x0 + d1;

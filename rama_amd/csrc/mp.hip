@@ -374,8 +374,17 @@ __device__ __forceinline__ double mp_edge(i64 e, const int* __restrict__ soff, c
     const int s0 = soff[e], s1 = soff[e + 1];
     if (s1 > s0) {
         if (kSweep) {
+            // in slot order; four loads in flight ahead of the additions
             double s = 0.0;
-            for (int k = s0; k < s1; ++k) s += (double)delta[k];
+            int k = s0;
+            for (; k + 4 <= s1; k += 4) {
+                const float x0 = delta[k], x1 = delta[k + 1], x2 = delta[k + 2], x3 = delta[k + 3];
+                s += (double)x0;
+                s += (double)x1;
+                s += (double)x2;
+                s += (double)x3;
+            }
+            for (; k < s1; ++k) s += (double)delta[k];
             c = (float)s;
         } else {
             c = 0.0f;
