@@ -162,18 +162,19 @@ __global__ __launch_bounds__(kBlock) void k_contract_reduce(const u64* __restric
     }
 }
 
-// Statistics of the sorted contraction keys in one launch (grid-stride
-// over the n sorted edges): out[1] = largest col, out[0] = largest col of an
-// off-diagonal edge, out[2] = row of the last key; each self-loop run
-// (the diagonal) is summed in sorted order and rounded to float, a negative
-// one raises out[3] (bad contraction), and they are added to *diag.  Block results go to
-// `part`; the last block (atomic ticket, reset afterwards) combines them.
+// Statistics of the sorted contraction keys (grid-stride over the n sorted
+// edges): out[1] = largest col, out[0] = largest col of an off-diagonal
+// edge, out[2] = row of the last key; each self-loop run (the diagonal) is
+// summed in sorted order and rounded to float, a negative one raises out[3]
+// (bad contraction), and their total goes to *diag.  Each block stores its
+// results in `part` (4 ints) and `dpart`; a one-block launch combines them
+// in block order (no atomics: same-address atomics from every block
+// serialise).
 __global__ __launch_bounds__(kBlock) void k_contract_stats(const u64* __restrict__ K, const u32* __restrict__ V,
-                                                          i64 n, int kb, int* part, unsigned* ticket, int* out,
-                                                          double* diag) {
+                                                          i64 n, int kb, int* __restrict__ part,
+                                                          double* __restrict__ dpart) {
     __shared__ int ws[kWaves];
     __shared__ double wd[kWaves];
-    __shared__ bool last;
     const u64 mask = ((u64)1 << kb) - 1;
     int all = -1, kept = -1, bad = 0;
     double dsum = 0.0;
@@ -210,34 +211,38 @@ __global__ __launch_bounds__(kBlock) void k_contract_stats(const u64* __restrict
     bad = block_reduce(bad, OpMax(), ws);
     dsum = block_reduce(dsum, OpSum(), wd);
     if (threadIdx.x == 0) {
-        if (dsum != 0.0) atomicAdd(diag, dsum);
-        __hip_atomic_store(&part[3 * blockIdx.x], all, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        __hip_atomic_store(&part[3 * blockIdx.x + 1], kept, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        __hip_atomic_store(&part[3 * blockIdx.x + 2], bad, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        const unsigned t = __hip_atomic_fetch_add(ticket, 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
-        last = t == gridDim.x - 1;
+        part[4 * blockIdx.x] = all;
+        part[4 * blockIdx.x + 1] = kept;
+        part[4 * blockIdx.x + 2] = bad;
+        dpart[blockIdx.x] = dsum;
     }
-    __syncthreads();
-    if (!last) return;
-    __threadfence();  // acquire: the other blocks' partials are visible to plain loads
-    all = -1;
-    kept = -1;
-    bad = 0;
-    for (unsigned b = threadIdx.x; b < gridDim.x; b += kBlock) {
-        const int x = part[3 * b], y = part[3 * b + 1], z = part[3 * b + 2];
+}
+
+__global__ __launch_bounds__(kBlock) void k_contract_stats_combine(const u64* __restrict__ K, i64 n, int kb,
+                                                                  const int* __restrict__ part,
+                                                                  const double* __restrict__ dpart, unsigned nb,
+                                                                  int* out, double* diag) {
+    __shared__ int ws[kWaves];
+    __shared__ double wd[kWaves];
+    int all = -1, kept = -1, bad = 0;
+    double d = 0.0;
+    for (unsigned b = threadIdx.x; b < nb; b += kBlock) {
+        const int x = part[4 * b], y = part[4 * b + 1], z = part[4 * b + 2];
         all = x > all ? x : all;
         kept = y > kept ? y : kept;
         bad = z > bad ? z : bad;
+        d += dpart[b];
     }
     all = block_reduce(all, OpMax(), ws);
     kept = block_reduce(kept, OpMax(), ws);
     bad = block_reduce(bad, OpMax(), ws);
+    d = block_reduce(d, OpSum(), wd);
     if (threadIdx.x == 0) {
         out[1] = all;
         out[0] = kept;
         out[2] = (int)(K[n - 1] >> kb);
         out[3] = bad;
-        *ticket = 0;
+        *diag = d;
     }
 }
 
@@ -313,9 +318,13 @@ DCoo contract(Ctx& ctx, const DCoo& A, const int* node_map, int max_label, bool*
     // the self-loop check
     {
         const unsigned nb = (unsigned)std::min<i64>(std::max<i64>((n + kBlock - 1) / kBlock, 1), kReduceMaxBlocks);
-        int* part = static_cast<int*>(ctx.reduce_scratch((size_t)kReduceMaxBlocks * 12));
+        char* scratch = static_cast<char*>(ctx.reduce_scratch((size_t)kReduceMaxBlocks * 24));
+        int* part = reinterpret_cast<int*>(scratch);
+        double* dpart = reinterpret_cast<double*>(scratch + (size_t)kReduceMaxBlocks * 16);
         prof::Scope ps(ctx, "contract_stats", 12.0 * n);
-        k_contract_stats<<<nb, kBlock, 0, ctx.stream>>>(K, V, n, kb, part, ctx.reduce_ticket(), SC + 2, DG);
+        k_contract_stats<<<nb, kBlock, 0, ctx.stream>>>(K, V, n, kb, part, dpart);
+        RAMA_LAUNCHED();
+        k_contract_stats_combine<<<1, kBlock, 0, ctx.stream>>>(K, n, kb, part, dpart, nb, SC + 2, DG);
         RAMA_LAUNCHED();
     }
     int h[8];
