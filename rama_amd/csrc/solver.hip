@@ -115,7 +115,9 @@ void solve_rounds(Ctx& ctx, DCoo& A, SolveResult& out, const Options& o, int sta
             n_contract = matching_vertex_based(ctx, U, o.mean_multiplier_mm, m);
             cur = ctx.buf<int>(Un);
             compress_matching(ctx, m.get(), Un, cur.get());
-            max_label = Un - 1;  // bound on the compressed labels (sizes the sort keys)
+            // largest compressed label (sizes the sort keys): every matched
+            // pair shares one label and n_contract counts both its vertices
+            max_label = Un - 1 - n_contract / 2;
             cur_size = Un;
             if ((float)n_contract < (float)A.rows * o.matching_thresh_crossover_ratio) {
                 if (o.verbose)
