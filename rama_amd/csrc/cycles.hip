@@ -198,6 +198,23 @@ __device__ __forceinline__ int probe_count(const int* __restrict__ adj, int sb, 
     return c;
 }
 
+// count_common of the lists [ia, ie) and [ib, ib + nb) by one lane: from
+// the shorter list by binary searches when the pair is heavy and that list
+// is short, else the merge walk.
+__device__ __forceinline__ int lane_count_common(const int* __restrict__ adj, int ia, int ie, int ib, int nb) {
+    const int na = ie - ia;
+    if (na + nb > kHeavy && min(na, nb) <= kProbe)
+        return na <= nb ? probe_count(adj, ia, ie, ib, ib + nb) : probe_count(adj, ib, ib + nb, ia, ie);
+    int i = ia, j = ib, je = ib + nb, c = 0;
+    while (i < ie && j < je) {
+        const int x = adj[i], y = adj[j];
+        c += (x == y);
+        i += (x <= y);
+        j += (y <= x);
+    }
+    return c;
+}
+
 // count_common by the whole wave (qa, qb wave-uniform): each lane takes
 // entries of the shorter list and binary-searches the longer one.
 __device__ __forceinline__ int wave_count_common(const int* __restrict__ off, const int* __restrict__ adj, int qa,
@@ -371,13 +388,27 @@ __global__ __launch_bounds__(kBlock) void k_cycle_count4(CycArgs a, unsigned cha
             reg_list_props(rb, n2, rmax, rdup);
         }
         TileAcc acc;
+        // the next n1's list bounds are loaded while this one is walked
+        int ia = 0, ie = 0;
+        if (d1 > 0) {
+            const int n1 = a.adj[b1];
+            ia = a.off[n1];
+            ie = a.off[n1 + 1];
+        }
         for (int l = 0; l < d1; ++l) {
-            const int n1 = a.adj[b1 + l];
-            const int c = regs ? count_vs_regs_fast(a.adj, a.off[n1], a.off[n1 + 1], rb, rmax, rdup)
-                               : count_common(a.off, a.adj, n1, v2);
+            int na = 0, ne = 0;
+            if (l + 1 < d1) {
+                const int nn = a.adj[b1 + l + 1];
+                na = a.off[nn];
+                ne = a.off[nn + 1];
+            }
+            const int c = regs ? count_vs_regs_fast(a.adj, ia, ie, rb, rmax, rdup)
+                               : lane_count_common(a.adj, ia, ie, b2, n2);
             cnt8[x0 + l] = (unsigned char)(c < 255 ? c : 255);
             acc.add(tile_tot, x0 + l, 2 * (i64)c);
             mine += 2 * (i64)c;
+            ia = na;
+            ie = ne;
         }
         acc.flush(tile_tot);
         xend = x0 + d1;
@@ -437,7 +468,7 @@ __global__ __launch_bounds__(kBlock) void k_cycle_count5(CycArgs a, const i64* _
         const int na = l + 1 < d1 ? a.off[nn] : 0, ne = l + 1 < d1 ? a.off[nn + 1] : 0;
         int c = 0;
         if (n1 != n2)
-            c = regs ? count_vs_regs_fast(a.adj, ia, ie, rb, rmax, rdup) : count_common(a.off, a.adj, n1, n2);
+            c = regs ? count_vs_regs_fast(a.adj, ia, ie, rb, rmax, rdup) : lane_count_common(a.adj, ia, ie, bq, nq);
         cnt8[x0 + l] = (unsigned char)(c < 255 ? c : 255);
         acc.add(tile_tot, x0 + l, 3 * (i64)c);
         mine += 3 * (i64)c;
