@@ -210,6 +210,12 @@ static void mp_build_impl(Ctx& ctx, const DCoo& A, Triangles& tri, MPState& S, i
     float* TC13 = S.t13.get();
     float* TC23 = S.t23.get();
     u32* POS = S.pos.get();
+    // triangle costs start at zero (coalesced, not per sorted item)
+    for_each(ctx, T, [=] __device__(i64 t) {
+        TC12[t] = 0.0f;
+        TC13[t] = 0.0f;
+        TC23[t] = 0.0f;
+    });
     for_each(ctx, N3, [=] __device__(i64 q) {
         const K k = KT[q];
         // chord heads before q, less the head of q's own run when that run
@@ -219,7 +225,6 @@ static void mp_build_impl(Ctx& ctx, const DCoo& A, Triangles& tri, MPState& S, i
         const int u = (int)(PA[q] + CP[q] - (dup && chord ? 1 : 0));
         const u32 v = VT[q], t = v / 3, s = v % 3;
         (s == 0 ? C12 : (s == 1 ? C13 : C23))[t] = u;
-        (s == 0 ? TC12 : (s == 1 ? TC13 : TC23))[t] = 0.0f;  // triangle costs start at zero
         POS[v] = (u32)q;  // slot 3t+s sends to place q, inside its edge's run [SOFF[u], SOFF[u+1])
     });
 }

@@ -87,6 +87,19 @@ def test_compute_offsets_and_compress(gpu, oracle):
     assert np.array_equal(gpu.compress_labels(data, 3500), oracle.compress_labels(data, 3500))
 
 
+@pytest.mark.parametrize("labels", [1, 2047, 2048, 2049, 4194304, 4194305, 9000001])
+def test_compress_labels_scan_sizes(gpu, labels):
+    # the label-range scan on both sides of one tile (2048 items) and of the
+    # point where a scan chunk takes a second tile (2048 tiles); the
+    # compressed label is the rank among the distinct values present
+    rng = np.random.Generator(np.random.PCG64(labels))
+    data = rng.integers(0, labels, size=min(200000, 3 * labels)).astype(np.int32)
+    data[0] = 0
+    data[-1] = labels - 1
+    expect = np.searchsorted(np.unique(data), data).astype(np.int32)
+    assert np.array_equal(gpu.compress_labels(data, labels - 1), expect)
+
+
 @pytest.mark.parametrize("name", NAMES)
 @pytest.mark.parametrize("mm", [0.0, 0.05, 0.5])
 def test_matching(gpu, oracle, name, mm):
