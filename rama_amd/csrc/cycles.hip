@@ -256,13 +256,14 @@ __device__ __forceinline__ int count_vs_regs(const int* __restrict__ adj, int ia
 
 // count_vs_regs when the register list has no repeated value (rdup false):
 // then a streamed value counts on its first copy only, when some register
-// entry equals it.  The 16 comparisons per streamed value become lane
-// masks combined on the scalar unit, and the walk stops at the first value
-// above the register list's largest entry (rmax; both lists are sorted).
-// With repeated register entries it falls back to count_vs_regs.
-__device__ __forceinline__ int count_vs_regs_fast(const int* __restrict__ adj, int ia, int ie,
-                                                  const int (&rb)[kRegList], int rmax, bool rdup) {
-    if (rdup) return count_vs_regs(adj, ia, ie, rb);
+// entry equals it.  The comparisons per streamed value become lane masks
+// combined on the scalar unit, and the walk stops at the first value above
+// the register list's largest entry (rmax; both lists are sorted).  Only
+// the first KL register entries are compared (the entries past a list's
+// length are -1 and match nothing).
+template <int KL>
+__device__ __forceinline__ int count_regs_masks(const int* __restrict__ adj, int ia, int ie,
+                                                const int (&rb)[kRegList], int rmax) {
     const int lane = lane_id();
     int c = 0, prev = -1;
     // four streamed entries per round of loads (past the end reads as
@@ -280,13 +281,31 @@ __device__ __forceinline__ int count_vs_regs_fast(const int* __restrict__ adj, i
             }
             unsigned long long hit = 0;
 #pragma unroll
-            for (int q = 0; q < kRegList; ++q) hit |= __ballot(rb[q] == x[j]);
+            for (int q = 0; q < KL; ++q) hit |= __ballot(rb[q] == x[j]);
             c += (int)((hit >> lane) & 1ull) & (int)(x[j] != prev);
             prev = x[j];
         }
         if (stop) break;
     }
     return c;
+}
+
+// Register entries the lanes about to intersect must compare (the longest
+// of their lists: 8, 12 or 16), agreed by the active lanes of the wave.
+__device__ __forceinline__ int reg_width(int nb) {
+    if (__ballot(nb > 8) == 0ull) return 8;
+    if (__ballot(nb > 12) == 0ull) return 12;
+    return kRegList;
+}
+
+// kw: reg_width of the lanes that call it together.  With repeated
+// register entries it falls back to count_vs_regs.
+__device__ __forceinline__ int count_vs_regs_fast(const int* __restrict__ adj, int ia, int ie,
+                                                  const int (&rb)[kRegList], int rmax, bool rdup, int kw) {
+    if (rdup) return count_vs_regs(adj, ia, ie, rb);
+    if (kw == 8) return count_regs_masks<8>(adj, ia, ie, rb, rmax);
+    if (kw == 12) return count_regs_masks<12>(adj, ia, ie, rb, rmax);
+    return count_regs_masks<kRegList>(adj, ia, ie, rb, rmax);
 }
 
 // Largest entry and whether a value repeats, for a register list of nb
@@ -349,7 +368,7 @@ __global__ __launch_bounds__(kBlock) void k_cycle_count3(CycArgs a, unsigned cha
             int rmax;
             bool rdup;
             reg_list_props(rb, n2, rmax, rdup);
-            c = count_vs_regs_fast(a.adj, a.off[v1], a.off[v1 + 1], rb, rmax, rdup);
+            c = count_vs_regs_fast(a.adj, a.off[v1], a.off[v1 + 1], rb, rmax, rdup, reg_width(n2));
         } else {
             c = count_common(a.off, a.adj, v1, v2);
         }
@@ -381,11 +400,12 @@ __global__ __launch_bounds__(kBlock) void k_cycle_count4(CycArgs a, unsigned cha
         const i64 x0 = a.xoff[r];
         int rb[kRegList];
         const bool regs = n2 <= kRegList;
-        int rmax = -1;
+        int rmax = -1, kw = kRegList;
         bool rdup = false;
         if (regs) {
             load_reg_list(a.adj, b2, n2, rb);
             reg_list_props(rb, n2, rmax, rdup);
+            kw = reg_width(n2);
         }
         TileAcc acc;
         // the next n1's list bounds are loaded while this one is walked
@@ -402,7 +422,7 @@ __global__ __launch_bounds__(kBlock) void k_cycle_count4(CycArgs a, unsigned cha
                 na = a.off[nn];
                 ne = a.off[nn + 1];
             }
-            const int c = regs ? count_vs_regs_fast(a.adj, ia, ie, rb, rmax, rdup)
+            const int c = regs ? count_vs_regs_fast(a.adj, ia, ie, rb, rmax, rdup, kw)
                                : lane_count_common(a.adj, ia, ie, b2, n2);
             cnt8[x0 + l] = (unsigned char)(c < 255 ? c : 255);
             acc.add(tile_tot, x0 + l, 2 * (i64)c);
@@ -453,11 +473,12 @@ __global__ __launch_bounds__(kBlock) void k_cycle_count5(CycArgs a, const i64* _
     const i64 x0 = a.xoff[r] + (i64)l2 * d1;
     int rb[kRegList];
     const bool regs = nq <= kRegList;
-    int rmax = -1;
+    int rmax = -1, kw = kRegList;
     bool rdup = false;
     if (regs) {
         load_reg_list(a.adj, bq, nq, rb);
         reg_list_props(rb, nq, rmax, rdup);
+        kw = reg_width(nq);
     }
     TileAcc acc;
     // the next n1 and its list bounds are loaded while this one is walked
@@ -468,7 +489,8 @@ __global__ __launch_bounds__(kBlock) void k_cycle_count5(CycArgs a, const i64* _
         const int na = l + 1 < d1 ? a.off[nn] : 0, ne = l + 1 < d1 ? a.off[nn + 1] : 0;
         int c = 0;
         if (n1 != n2)
-            c = regs ? count_vs_regs_fast(a.adj, ia, ie, rb, rmax, rdup) : lane_count_common(a.adj, ia, ie, bq, nq);
+            c = regs ? count_vs_regs_fast(a.adj, ia, ie, rb, rmax, rdup, kw)
+                     : lane_count_common(a.adj, ia, ie, bq, nq);
         cnt8[x0 + l] = (unsigned char)(c < 255 ? c : 255);
         acc.add(tile_tot, x0 + l, 3 * (i64)c);
         mine += 3 * (i64)c;
