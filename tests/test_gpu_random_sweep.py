@@ -47,3 +47,29 @@ def test_random_solve_equals_oracle(gpu, oracle, seed):
     pairs = set(zip(ig.tolist(), io.tolist()))
     assert len(pairs) == len(set(ig.tolist())) == len(set(io.tolist())), "partition differs"
     assert abs(glb - olb) <= 1e-9 * max(1.0, abs(olb))
+
+
+@pytest.mark.parametrize("seed", list(range(24, 72)))
+def test_random_solve_equals_oracle_all_presets(gpu, oracle, seed):
+    # the same structures plus small dense graphs, across P/PD/PD+/D
+    rng = np.random.Generator(np.random.PCG64(5000 + seed))
+    if seed % 5 < 4:
+        i, j, c = _instance(seed)
+    else:
+        n = int(rng.integers(50, 600))
+        i, j, c = random_graph(n, int(n * rng.uniform(1.0, 8.0)), seed=seed, neg_frac=float(rng.uniform(0.1, 0.7)))
+    preset = ["P", "PD", "PD+", "D"][(seed // 5) % 4]
+    try:
+        o, olb, _ = oracle.solve(i, j, c, oracle.options(preset))
+    except Exception as e:
+        with pytest.raises(Exception) as ge:
+            gpu.solve(i, j, c, gpu.options(preset, verbose=0))
+        assert str(e) in str(ge.value)
+        return
+    g, glb, _, _ = gpu.solve(i, j, c, gpu.options(preset, verbose=0))
+    assert len(g) == len(o)
+    _, ig = np.unique(g, return_inverse=True)
+    _, io = np.unique(o, return_inverse=True)
+    pairs = set(zip(ig.tolist(), io.tolist()))
+    assert len(pairs) == len(set(ig.tolist())) == len(set(io.tolist())), "partition differs"
+    assert abs(glb - olb) <= 1e-9 * max(1.0, abs(olb))
