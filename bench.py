@@ -220,8 +220,8 @@ def main():
         print(json.dumps(batch_mode(a) if a.mode == "batch" else serve_mode(a)))
         return
     if a.gpus != 1:
-        print("bench.py measures one GPU: this build runs the solver on a single MI355X; "
-              "multi-GPU runs are not part of it (see DESIGN.md for how the path would shard).", file=sys.stderr)
+        print("bench.py measures one GPU: the solver runs on a single MI355X; "
+              "multi-GPU runs are not supported.", file=sys.stderr)
         sys.exit(2)
     import torch
 
