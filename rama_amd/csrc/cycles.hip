@@ -48,15 +48,20 @@ struct CycArgs {
     i64* counted = nullptr;
 };
 
-// Adds the block's attempted writes to counted[0] and raises counted[1] to
-// the end of its expansions.  Block-uniform call.
+// Adds the block's attempted writes to its counter line's [0] and raises
+// its [1] to the end of its expansions.  The counters are spread over
+// kCountedLines lines of 16 i64 (128 bytes): device atomics from every block
+// to one address serialise; the host sums the [0]s and takes the largest
+// [1].  Block-uniform call.
+constexpr int kCountedLines = 16;
 __device__ __forceinline__ void block_counted(i64* counted, i64 mine, i64 xend) {
     __shared__ i64 ws[kWaves];
     const i64 s = block_reduce(mine, OpSum(), ws);
     const i64 m = block_reduce(xend, OpMax(), ws);
     if (threadIdx.x == 0) {
-        if (s) atomicAdd(reinterpret_cast<unsigned long long*>(counted), (unsigned long long)s);
-        atomicMax(reinterpret_cast<unsigned long long*>(counted + 1), (unsigned long long)m);
+        i64* line = counted + (blockIdx.x % kCountedLines) * 16;
+        if (s) atomicAdd(reinterpret_cast<unsigned long long*>(line), (unsigned long long)s);
+        atomicMax(reinterpret_cast<unsigned long long*>(line + 1), (unsigned long long)m);
     }
 }
 
@@ -717,8 +722,8 @@ static i64 run_stage(Ctx& ctx, const Csr& G, const int* rv1, const int* rv2, i64
     i64 tiles_used = ntiles;
     if (S > 3) {
         ctx.zero(tile.get(), ntiles);
-        DBuf<i64> cnt = ctx.buf<i64>(2);
-        ctx.zero(cnt.get(), 2);
+        DBuf<i64> cnt = ctx.buf<i64>(kCountedLines * 16);
+        ctx.zero(cnt.get(), kCountedLines * 16);
         a.counted = cnt.get();
         DBuf<i64> yoff;
         DBuf<int> ytile_rep;
@@ -761,7 +766,16 @@ static i64 run_stage(Ctx& ctx, const Csr& G, const int* rv1, const int* rv2, i64
             }
             done += n;
             if (!early || done >= nblk) break;
-            ctx.read(cnt.get(), hc, 2);
+            {
+                i64 lines[kCountedLines * 16];
+                ctx.read(cnt.get(), lines, kCountedLines * 16);
+                hc[0] = 0;
+                hc[1] = 0;
+                for (int k = 0; k < kCountedLines; ++k) {
+                    hc[0] += lines[k * 16];
+                    hc[1] = std::max(hc[1], lines[k * 16 + 1]);
+                }
+            }
             if (hc[0] >= need) break;
             // next chunk: 1.25x the blocks that the rate so far says reach
             // the budget (all the rest when it says they will not)
