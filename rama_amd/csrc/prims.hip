@@ -61,10 +61,10 @@ __device__ __forceinline__ void wave_count(int* h, u32 d, bool ok) {
     if ((active >> lane) & 1ull) atomicAdd(&h[d], 1);
 }
 
-// The three-launch pass runs 512 threads per 4096-pair tile (8 pairs each):
-// half the registers of 16 per thread, so twice the waves per SIMD within
-// the same LDS.
-constexpr int kScThreads = 512;
+// The radix passes run 1024 threads per 4096-pair tile (4 pairs each): few
+// registers per thread, so many waves per SIMD within the same LDS (faster
+// than 512 threads with 8 pairs, and those than 256 with 16).
+constexpr int kScThreads = 1024;
 constexpr int kScItems = kRsTile / kScThreads;
 constexpr int kScWaves = kScThreads / 64;
 
