@@ -15,8 +15,18 @@ Also reported on the same JSON line:
                 solver's stream over the timed steps (the warm-up steps time
                 every kernel to pick the dominant one and fill `kernels`;
                 the timed steps record events around that kernel only);
+  batched       the 512-instance batched workload (n log-uniform in [1e3,
+                2e5], Voronoi grids and Erdos-Renyi graphs, PD): solves/s of
+                one packed batch vs the same instances solved one at a time
+                by the plain host entry, and whether every result is identical;
+  serving       Poisson arrivals of PD requests (lognormal edge counts,
+                median 5e4, sigma 1, cap 2e7) at 0.5, 0.9 and 1.5 x the plain
+                loop's measured capacity: solves/s and p50/p99 latency of the
+                serving loop (16 lanes, 200 GB HBM budget) against the plain
+                one-request-at-a-time loop, and whether results are identical;
   cpu_baseline  the CPU restatement (oracle, 1 thread) timed on a bounded
-                256x1024 crop of the same generator.
+                512x1024 crop of the same generator, plus the 64x64 grid
+                workload in full.
 """
 import argparse
 import json
@@ -55,21 +65,33 @@ def parse():
     ap.add_argument("--workers", type=int, default=16)
     ap.add_argument("--requests", type=int, default=160, help="requests per offered load in serve mode")
     ap.add_argument("--kernels", type=int, default=8, help="profiled kernels listed on the line")
+    ap.add_argument("--no-batch", action="store_true", help="plain mode: leave out the batched block")
+    ap.add_argument("--no-serve", action="store_true", help="plain mode: leave out the serving block")
     return ap.parse_args()
 
 
 def cpu_baseline():
-    """Oracle (serial C++ restatement) on a bounded crop of the same workload."""
+    """Oracle (serial C++ restatement, one thread) on a bounded crop of the
+    same workload (a quarter of the grid at the same cluster density), and
+    the 64x64 grid workload in full."""
     from oracle import oracle as O
     from rama_amd import instances
 
-    i, j, c, _ = instances.grid2d_voronoi(H=256, W=1024, n_seeds=250, seed=0)
+    i, j, c, _ = instances.grid2d_voronoi(H=512, W=1024, n_seeds=500, seed=0)
     t = time.perf_counter()
     O.solve(i, j, c, O.options("PD"))
     dt = time.perf_counter() - t
+    i0, j0, c0, _ = instances.grid2d_rectangles(seed=0)
+    reps = 20
+    t = time.perf_counter()
+    for _ in range(reps):
+        O.solve(i0, j0, c0, O.options("PD"))
+    dt0 = (time.perf_counter() - t) / reps
     return {"value": len(c) / dt, "unit": "edges/s", "cores": 1, "kind": "port",
-            "sample": "PD solve of a 256x1024 crop (%d edges, 250 Voronoi seeds) of the same generator, %.1f s"
-                      % (len(c), dt)}
+            "sample": "PD solve of a 512x1024 crop (%d edges, 500 Voronoi seeds) of the same generator, %.1f s"
+                      % (len(c), dt),
+            "host_cpus": os.cpu_count(),
+            "grid64": {"edges": len(c0), "ms_per_solve": dt0 * 1e3, "edges_per_s": len(c0) / dt0}}
 
 
 def copy_peak(torch, nbytes=2 << 30, reps=5):
@@ -103,65 +125,60 @@ def copy_mode():
             "kib_each": src.numel() * 4 / 1024}
 
 
-def batch_mode(a):
+def _batch_instances(count):
+    from rama_amd import instances
+
+    sizes = instances.batch_sizes(count, 1e3, 2e5, seed=12345)
+    return [instances.batch_instance(k, int(n))[:3] for k, n in enumerate(sizes)]
+
+
+def batched_block(count=512, workers=16, batched_only=False):
     """512 instances (n log-uniform in [1e3, 2e5]; even: Voronoi grids, odd:
     Erdos-Renyi), batched on one GPU (lockstep packed solve; `workers`
     threads continue the instances that switch to the spanning-forest
     contraction) vs the same instances solved one at a time by the plain
     host entry.  Both include the host-to-device copies."""
-    from rama_amd import _capi, instances
+    from rama_amd import _capi
 
-    sizes = instances.batch_sizes(a.count, 1e3, 2e5, seed=12345)
-    insts = [instances.batch_instance(k, int(n))[:3] for k, n in enumerate(sizes)]
+    insts = _batch_instances(count)
     E = int(sum(len(x[2]) for x in insts))
     opts = _capi.options("PD", verbose=0)
     _capi.solve_batch(insts[:4], opts, workers=1)  # warm up the runtime
-    if a.batched_only:
+    out = {"workload": "%d instances, n log-uniform [1e3, 2e5], Voronoi grids + Erdos-Renyi, PD" % count,
+           "instances": count, "edges": E, "workers": workers}
+    if not batched_only:
+        for x in insts[:4]:
+            _capi.solve(*x, opts)
+        # the plain form, one instance after the other
         t = time.perf_counter()
-        bat = _capi.solve_batch(insts, opts, workers=a.workers)
-        t_bat = time.perf_counter() - t
-        return {"metric": "batched solves/s", "value": a.count / t_bat, "unit": "solves/s", "n_gpus": 1,
-                "higher_is_better": True, "dtype": "fp32", "data": "synthetic", "vs_baseline": None,
-                "config": {"workload": "%d instances, n log-uniform [1e3, 2e5], grids + Erdos-Renyi, PD" % a.count,
-                           "edges": E, "workers": a.workers},
-                "seconds": {"batched": t_bat}}
-    for x in insts[:4]:
-        _capi.solve(*x, opts)
-    # the plain form, one instance after the other
+        one = [_capi.solve(*x, opts)[:2] for x in insts]
+        t_one = time.perf_counter() - t
     t = time.perf_counter()
-    one = [_capi.solve(*x, opts)[:2] for x in insts]
-    t_one = time.perf_counter() - t
-    t = time.perf_counter()
-    bat = _capi.solve_batch(insts, opts, workers=a.workers)
+    bat = _capi.solve_batch(insts, opts, workers=workers)
     t_bat = time.perf_counter() - t
-    same = all(np.array_equal(x[0], y[0]) and x[1] == y[1] for x, y in zip(one, bat))
-    obj = sum(_capi.objective(l, *x) for (l, _), x in zip(bat, insts))
-    return {"metric": "batched solves/s", "value": a.count / t_bat, "unit": "solves/s", "n_gpus": 1,
-            "higher_is_better": True, "dtype": "fp32", "data": "synthetic", "vs_baseline": None,
-            "config": {"workload": "%d instances, n log-uniform [1e3, 2e5], grids + Erdos-Renyi, PD" % a.count,
-                       "edges": E, "workers": a.workers},
-            "one_at_a_time_solves_per_s": a.count / t_one, "speedup": t_one / t_bat,
-            "identical_results": bool(same), "total_objective": obj,
-            "seconds": {"batched": t_bat, "one_at_a_time": t_one}}
+    out["batched_solves_per_s"] = count / t_bat
+    out["batched_s"] = t_bat
+    out["total_objective"] = float(sum(_capi.objective(l, *x) for (l, _), x in zip(bat, insts)))
+    if not batched_only:
+        out["one_at_a_time_solves_per_s"] = count / t_one
+        out["one_at_a_time_s"] = t_one
+        out["speedup"] = t_one / t_bat
+        out["identical_results"] = bool(all(np.array_equal(x[0], y[0]) and x[1] == y[1] for x, y in zip(one, bat)))
+        out["one_at_a_time_total_objective"] = float(sum(_capi.objective(l, *x) for (l, _), x in zip(one, insts)))
+    return out
 
 
-def serve_mode(a):
+def serving_block(loads=(0.5, 0.9, 1.5), requests=160, workers=16):
     """Poisson arrivals (lognormal edge counts, median 5e4, sigma 1, cap 2e7)
-    at 0.5, 0.9, 1.5 and 6 x the plain loop's capacity, against the serving loop
-    (lockstep pack with continuous admission, a.workers threads for the
-    forest phase, 200 GB HBM budget) and the plain one-at-a-time loop."""
-    import math
-
+    at the given multiples of the plain loop's capacity, against the serving
+    loop (lanes with continuous admission into lockstep packs, 200 GB HBM
+    budget) and the plain one-request-at-a-time loop.  Latency runs from
+    submission to labels on the host."""
     from rama_amd import _capi, instances
 
-    rng = np.random.Generator(np.random.PCG64(2024))
-    pool = []
-    for k in range(48):
-        m = int(min(2e7, round(math.exp(rng.normal(math.log(5e4), 1.0)))))
-        n = max(64, m // 2 if k % 2 == 0 else m // 4)
-        pool.append(instances.batch_instance(k, n)[:3])
+    pool = instances.serve_pool()
     opts = _capi.options("PD", verbose=0)
-    # capacity of the plain loop: mean solve time over the pool, one at a time
+    # capacity of the plain loop: solves/s over the pool, one at a time
     with _capi.Server(workers=1, opts=opts, mode="workers") as srv:
         srv.wait(srv.submit(*pool[0]))
         t = time.perf_counter()
@@ -169,16 +186,12 @@ def serve_mode(a):
             srv.wait(srv.submit(*x))
         cap = len(pool) / (time.perf_counter() - t)
 
-    def run(workers, rate, nreq, mode):
-        seq = rng.integers(0, len(pool), size=nreq)
-        gaps = rng.exponential(1.0 / rate, size=nreq)
-        with _capi.Server(workers=workers, hbm_budget_bytes=200 << 30, opts=opts, mode=mode) as srv:
+    def run(n_lanes, stream, mode):
+        with _capi.Server(workers=n_lanes, hbm_budget_bytes=200 << 30, opts=opts, mode=mode) as srv:
             t0 = time.perf_counter()
-            at = t0
             tickets = []
-            for q, g in zip(seq, gaps):
-                at += g
-                d = at - time.perf_counter()
+            for q, at in stream:
+                d = t0 + at - time.perf_counter()
                 if d > 0:
                     time.sleep(d)
                 tickets.append((q, srv.submit(*pool[q])))
@@ -187,42 +200,59 @@ def serve_mode(a):
             for q, tk in tickets:
                 lab, lb, l_ms, _ = srv.wait(tk)
                 lat.append(l_ms)
-                res.setdefault(int(q), (lab, lb))
+                res.setdefault(q, (lab, lb))
             wall = time.perf_counter() - t0
         lat = np.asarray(lat)
-        return {"solves_per_s": nreq / wall, "p50_ms": float(np.percentile(lat, 50)),
+        return {"solves_per_s": len(stream) / wall, "p50_ms": float(np.percentile(lat, 50)),
                 "p99_ms": float(np.percentile(lat, 99))}, res
 
-    loads = {}
-    for f in (0.5, 0.9, 1.5, 6.0):
-        served, r1 = run(a.workers, f * cap, a.requests, "lockstep")
-        plain, r2 = run(1, f * cap, a.requests, "workers")
-        same = all(np.array_equal(r1[q][0], r2[q][0]) and r1[q][1] == r2[q][1] for q in r1 if q in r2)
-        loads[str(f)] = {"offered_per_s": f * cap, "served": served, "plain_loop": plain, "identical_results": same}
-    return {"metric": "served solves/s + p99 latency", "value": loads["6.0"]["served"]["solves_per_s"],
-            "unit": "solves/s", "n_gpus": 1, "higher_is_better": True, "dtype": "fp32", "data": "synthetic",
-            "vs_baseline": None,
-            "config": {"workload": "Poisson arrivals, lognormal edges (median 5e4, sigma 1, cap 2e7), grids + ER, PD",
-                       "workers": a.workers, "hbm_budget_GB": 200, "plain_capacity_per_s": cap},
-            "loads": loads}
+    out = {"workload": "Poisson arrivals, lognormal edges (median 5e4, sigma 1, cap 2e7), "
+                       "Voronoi grids + Erdos-Renyi, PD; %d requests per load from a %d-request pool"
+                       % (requests, len(pool)),
+           "lanes": workers, "hbm_budget_GB": 200, "plain_capacity_per_s": cap, "loads": {}}
+    for k, f in enumerate(loads):
+        stream = instances.poisson_stream(f * cap, requests, len(pool), seed=100 + k)
+        served, r1 = run(workers, stream, "lockstep")
+        plain, r2 = run(1, stream, "workers")
+        same = all(np.array_equal(r1[q][0], r2[q][0]) and r1[q][1] == r2[q][1] for q in r1)
+        obj1 = float(sum(_capi.objective(r1[q][0], *pool[q]) for q, _ in stream))
+        obj2 = float(sum(_capi.objective(r2[q][0], *pool[q]) for q, _ in stream))
+        out["loads"][str(f)] = {"offered_per_s": f * cap, "served": served, "plain_loop": plain,
+                                "identical_results": bool(same), "objective_served": obj1, "objective_plain": obj2}
+    return out
+
+
+def progress(msg):
+    print("[bench] " + msg, file=sys.stderr, flush=True)
 
 
 def main():
     a = parse()
-    if a.mode == "copy":
-        print(json.dumps(copy_mode()))
-        return
-    if a.mode != "plain":
-        # more hardware queues so the worker streams run side by side
-        os.environ.setdefault("GPU_MAX_HW_QUEUES", "16")
-        import torch  # noqa: F401  (one HIP runtime for the process)
-
-        print(json.dumps(batch_mode(a) if a.mode == "batch" else serve_mode(a)))
-        return
     if a.gpus != 1:
         print("bench.py measures one GPU: the solver runs on a single MI355X; "
               "multi-GPU runs are not supported.", file=sys.stderr)
         sys.exit(2)
+    if a.mode == "copy":
+        print(json.dumps(copy_mode()))
+        return
+    # more hardware queues so the batch workers' and serving lanes' streams
+    # run side by side (set before the first HIP call of the process)
+    os.environ.setdefault("GPU_MAX_HW_QUEUES", "16")
+    if a.mode != "plain":
+        import torch  # noqa: F401  (one HIP runtime for the process)
+
+        if a.mode == "batch":
+            b = batched_block(a.count, a.workers, a.batched_only)
+            print(json.dumps({"metric": "batched solves/s", "value": b["batched_solves_per_s"], "unit": "solves/s",
+                              "n_gpus": 1, "higher_is_better": True, "dtype": "fp32", "data": "synthetic",
+                              "vs_baseline": None, "batched": b}))
+        else:
+            loads = (0.5, 0.9, 1.5, 6.0)
+            sv = serving_block(loads, a.requests, a.workers)
+            print(json.dumps({"metric": "served solves/s + p99 latency", "unit": "solves/s", "n_gpus": 1,
+                              "value": sv["loads"]["6.0"]["served"]["solves_per_s"], "higher_is_better": True,
+                              "dtype": "fp32", "data": "synthetic", "vs_baseline": None, "serving": sv}))
+        return
     import torch
 
     from rama_amd import _capi, instances, profiler
@@ -282,7 +312,10 @@ def main():
             roof["traffic"] = table[roof["kernel"]]
         roof["measured_copy_peak"] = copy_peak(torch)
     out = {
-        "metric": "%s edges/s" % a.preset,
+        # the headline metric: PD edges/s is `value`; the served solves/s and
+        # p99 latency against the sequential loop are in the serving block
+        "metric": ("PD edges/s; served solves/s + p99 latency vs sequential at equal objective"
+                   if a.preset == "PD" and not a.no_serve else "%s edges/s" % a.preset),
         "value": E / dt,
         "unit": "edges/s",
         "n_gpus": 1,
@@ -300,6 +333,17 @@ def main():
         "roofline": roof,
         "kernels": profiler.top(all_stats, a.kernels),
     }
+    progress("plain PD: %.2f ms per solve" % (dt * 1e3))
+    del di, dj, dc, labels
+    torch.cuda.empty_cache()
+    _capi.lib().rama_release_cached(0)
+    if not a.no_batch:
+        out["batched"] = batched_block(512, a.workers)
+        progress("batched: %.1f solves/s vs %.1f one at a time" % (out["batched"]["batched_solves_per_s"],
+                                                                   out["batched"]["one_at_a_time_solves_per_s"]))
+    if not a.no_serve:
+        out["serving"] = serving_block((0.5, 0.9, 1.5), a.requests, a.workers)
+        progress("serving done")
     if not a.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline()
     print(json.dumps(out))

@@ -60,10 +60,11 @@ inline std::vector<std::vector<int>> rama_cuda_gpu_pointers(const int* i, const 
     int32_t ms = 0;
     rama_timeline* tl = nullptr;
     const rama_options_c o = opts.to_c();
-    // the reference writes as many labels as it produced; the caller sized
-    // the buffer for num_nodes (+1 tolerated for safety)
+    // the reference writes node_mapping.size() = largest node id + 1 labels
+    // into a buffer the caller sized for num_nodes; more labels than that
+    // fail with the size error instead of writing past the buffer
     rama_detail::check(rama_solve_device(i, j, edge_costs, num_edges, &o, device, stream, node_labels,
-                                         (int64_t)num_nodes + 1, &n, &lb, &ms, &tl));
+                                         (int64_t)num_nodes, &n, &lb, &ms, &tl));
     return rama_detail::take_timeline(tl);
 }
 

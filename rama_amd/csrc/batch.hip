@@ -341,8 +341,14 @@ int rama_server_wait(rama_server* s, int64_t ticket, int32_t* labels_out, int64_
             return RAMA_ERR_INVALID;
         }
         r = it->second;
+        // the wait releases the lock: submits may rehash the map and another
+        // waiter on the same ticket may have removed it, so the iterator is
+        // not reused; only the first waiter to wake takes the result
         s->cv_done.wait(l, [&] { return r->done; });
-        s->all.erase(it);
+        if (s->all.erase(ticket) == 0) {
+            set_last_error("ticket already collected by another wait");
+            return RAMA_ERR_INVALID;
+        }
     }
     if (r->status != 0) {
         set_last_error(r->error);

@@ -3,6 +3,7 @@
 // copies for the host-buffer entry points.
 #include <chrono>
 #include <cmath>
+#include <memory>
 
 #include "rama_internal.hpp"
 
@@ -88,10 +89,8 @@ void solve_common(Ctx& ctx, DBuf<int>& i, DBuf<int>& j, DBuf<float>& c, i64 m, c
         labels = std::move(r.labels);
         n_labels = r.num_labels;
     }
-    if (tl) {
-        *tl = new rama_timeline();
-        (*tl)->snaps = std::move(r.timeline);
-    }
+    // the caller hands the timeline out once every later check has passed
+    if (tl) *tl = new rama_timeline{std::move(r.timeline)};
 }
 
 }  // namespace rama
@@ -141,15 +140,20 @@ int rama_solve_host(const int32_t* i, const int32_t* j, const float* costs, int6
                     rama_timeline** tl) {
     return guarded([&] {
         RAMA_REQUIRE(i && j && costs && n_labels_out && lb_out, "NULL argument");
+        if (tl) *tl = nullptr;
         Ctx ctx(pick_device(), nullptr, opts && opts->verbose);
         DBuf<int> di = upload(ctx, i, m), dj = upload(ctx, j, m);
         DBuf<float> dc = upload(ctx, costs, m);
         DBuf<int> labels;
         i64 n = 0;
-        solve_common(ctx, di, dj, dc, m, opts, labels, n, lb_out, time_ms_out, tl);
+        rama_timeline* t = nullptr;
+        std::unique_ptr<rama_timeline> own;
+        solve_common(ctx, di, dj, dc, m, opts, labels, n, lb_out, time_ms_out, tl ? &t : nullptr);
+        own.reset(t);
         *n_labels_out = n;
         if (n > cap) throw Error(RAMA_ERR_INVALID, "label buffer too small: need " + std::to_string(n));
         if (n > 0) download(ctx, labels_out, labels.get(), n);
+        if (tl) *tl = own.release();
     });
 }
 
@@ -158,6 +162,7 @@ int rama_solve_device(const int32_t* d_i, const int32_t* d_j, const float* d_c, 
                       double* lb_out, int32_t* time_ms_out, rama_timeline** tl) {
     return guarded([&] {
         RAMA_REQUIRE(d_i && d_j && d_c && n_labels_out && lb_out, "NULL argument");
+        if (tl) *tl = nullptr;
         Ctx ctx(device, static_cast<hipStream_t>(stream), opts && opts->verbose);
         DBuf<int> di = ctx.buf<int>(m), dj = ctx.buf<int>(m);
         DBuf<float> dc = ctx.buf<float>(m);
@@ -166,12 +171,16 @@ int rama_solve_device(const int32_t* d_i, const int32_t* d_j, const float* d_c, 
         ctx.d2d(dc.get(), d_c, m);
         DBuf<int> labels;
         i64 n = 0;
+        rama_timeline* t = nullptr;
+        std::unique_ptr<rama_timeline> own;
         // device overload builds dCOO(col=j, row=i) (src/rama_cuda.cu:202); orientation makes both equal
-        solve_common(ctx, di, dj, dc, m, opts, labels, n, lb_out, time_ms_out, tl);
+        solve_common(ctx, di, dj, dc, m, opts, labels, n, lb_out, time_ms_out, tl ? &t : nullptr);
+        own.reset(t);
         *n_labels_out = n;
         if (n > cap) throw Error(RAMA_ERR_INVALID, "label buffer too small: need " + std::to_string(n));
         if (n > 0) ctx.d2d(d_labels_out, labels.get(), n);
         ctx.sync();
+        if (tl) *tl = own.release();
     });
 }
 

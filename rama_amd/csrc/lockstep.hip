@@ -631,6 +631,7 @@ class Lockstep {
     }
     // Publishes the listed slots as finished (labels from the pack).
     void publish(const std::vector<int>& slots) {
+        scan_check(ctx_);  // no result leaves a pack whose sorts may be mis-ordered
         const std::vector<int32_t> hl = gather_labels(ctx_, P_, slots);
         i64 o = 0;
         for (int q : slots) {
@@ -761,6 +762,7 @@ void Lockstep::compact(const std::vector<char>& keep) {
 
 void Lockstep::hand_off_slots(const std::vector<int>& which) {
     if (which.empty()) return;
+    scan_check(ctx_);
     const int Q = P_.Q();
     std::vector<int> isin((size_t)Q, 0);
     std::vector<i64> hoff;
@@ -1220,6 +1222,7 @@ struct BatchSink : LockstepSink {
 // forest contraction are continued by `workers` threads.  Per-instance
 // failures go to status; a failure of the whole pack throws.
 void solve_packed(Ctx& ctx, PackedBatch& B, int workers) {
+    scan_reset(ctx);
     BatchSink sink(B);
     Lockstep L(ctx, B.o, workers, sink);
     RAMA_REQUIRE(B.k <= Lockstep::kMaxSlots, "at most 4096 instances per packed batch");

@@ -262,11 +262,23 @@ void scan_exclusive_arr(Ctx& ctx, const T* in, T* out, i64 n) {
 }
 
 // Raises if any look-back gave up (never expected; guards against a hang).
+// The flag is cleared before raising, so later solves on the same context
+// (batch workers and serving lanes reuse theirs) start clean.
 inline void scan_check(Ctx& ctx) {
     if (ctx.scan_counter.empty()) return;
     int h[2];
     ctx.read(ctx.scan_counter.get(), h, 2);
-    if (h[1]) throw Error(RAMA_ERR_INTERNAL, "single-pass scan look-back timed out");
+    if (h[1]) {
+        ctx.zero(ctx.scan_counter.get(), 2);
+        ctx.sync();
+        throw Error(RAMA_ERR_INTERNAL, "single-pass scan look-back timed out");
+    }
+}
+
+// Clears a look-back error flag left by a solve that ended early on another
+// error, before a new solve on the context relies on it.
+inline void scan_reset(Ctx& ctx) {
+    if (!ctx.scan_counter.empty()) ctx.zero(ctx.scan_counter.get() + 1, 1);
 }
 
 // ------------------------------------------------------------- reduction

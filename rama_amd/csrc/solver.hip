@@ -58,6 +58,7 @@ SolveResult solve(Ctx& ctx, DCoo& A, const Options& o) {
     SolveResult out;
     out.t0 = std::chrono::steady_clock::now();
     ctx.last_cycles_empty = false;  // a hint from an earlier solve on this context says nothing here
+    scan_reset(ctx);
     StageTimes st;
     st.begin(ctx);
     out.lb = dual_solver(ctx, A, o.max_cycle_length_lb, o.num_dual_itr_lb, o.tri_memory_factor, o.num_outer_itr_dual,

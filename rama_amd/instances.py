@@ -10,7 +10,8 @@ emitted once with i < j.
   grid2d_voronoi     2-D grid + long-range offsets, Voronoi clusters
   grid3d_voronoi     3-D 6-grid, Voronoi clusters, mixture costs
   erdos_renyi        G(n, m) without self-loops or duplicates, N(0.3, 1) costs
-  batch_instances    512 mixed instances (grids and ER), log-uniform sizes
+  batch_sizes / batch_instance   512 mixed instances (grids and ER), log-uniform sizes
+  serve_pool / poisson_stream    serving requests: lognormal edge counts, Poisson arrivals
 """
 import math
 
@@ -129,6 +130,30 @@ def batch_instance(k, n):
     if k % 2 == 0:
         return small_grid_voronoi(int(n), seed=k)
     return erdos_renyi(int(n), avg_degree=8, seed=k)
+
+
+def serve_pool(count=48, median=5e4, sigma=1.0, cap=2e7, seed=2024):
+    """Request pool of the serving workload: request k has
+    m = min(cap, round(lognormal(ln median, sigma))) edges; even k is a
+    Voronoi grid (about m / 2 nodes), odd k Erdős–Rényi (m / 4 nodes), both
+    with seed k.  Returns a list of (i, j, c)."""
+    rng = np.random.Generator(np.random.PCG64(seed))
+    pool = []
+    for k in range(count):
+        m = int(min(cap, round(math.exp(rng.normal(math.log(median), sigma)))))
+        n = max(64, m // 2 if k % 2 == 0 else m // 4)
+        pool.append(batch_instance(k, n)[:3])
+    return pool
+
+
+def poisson_stream(rate, count, pool_size, seed):
+    """Arrival schedule of the serving workload: (pool index, arrival time in
+    seconds from the start) for `count` requests with exponential gaps of
+    mean 1 / rate, each request drawn uniformly from the pool."""
+    rng = np.random.Generator(np.random.PCG64(seed))
+    seq = rng.integers(0, pool_size, size=count)
+    at = np.cumsum(rng.exponential(1.0 / rate, size=count))
+    return [(int(q), float(t)) for q, t in zip(seq, at)]
 
 
 def config(name, **kw):
