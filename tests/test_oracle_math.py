@@ -226,3 +226,121 @@ def test_partition_hash_is_relabel_invariant():
     assert mk.partition_hash(a) == mk.partition_hash(perm[a])
     assert mk.partition_hash(a) != mk.partition_hash(np.array([5, 5, 2, 7, 7, 9]))
     assert mk.partition_hash(a) != mk.partition_hash(np.array([5, 5, 2, 2, 2, 9]))
+
+
+# ---- matching rules worked by hand from the reference's kernels
+# (src/maximum_matching_vertex_based.cu:10-137): a vertex keeps the first
+# neighbour whose cost is strictly above the running maximum (initialised to
+# the threshold), so ties go to the lowest neighbour id and a cost equal to
+# the threshold is not eligible; at most 10 rounds, and a round whose matched
+# count grows by less than 10% relative to (previous + 1) is the last.
+
+def _match(oracle, edges, mm=0.05):
+    i = np.array([a for a, _, _ in edges], np.int32)
+    j = np.array([b for _, b, _ in edges], np.int32)
+    c = np.array([x for _, _, x in edges], np.float32)
+    r, co, v, (rows, cols) = oracle.dcoo_init(i, j, c)
+    return oracle.matching(r, co, v, rows, cols, mm)
+
+
+def test_matching_tie_goes_to_lowest_neighbour(oracle):
+    # 0 sees 1 and 2 at the same cost: it picks 1 (first maximum); 1 and 2
+    # both pick 0, so 0-1 is the mutual pair and 2 stays alone
+    m, matched = _match(oracle, [(0, 1, 5.0), (0, 2, 5.0)])
+    assert m.tolist() == [0, 0, 2] and matched == 2
+    # a larger cost later in the row still wins over an earlier one
+    m, matched = _match(oracle, [(0, 1, 3.0), (0, 2, 5.0), (0, 3, 5.0)])
+    assert m.tolist() == [0, 1, 0, 3] and matched == 2
+
+
+def test_matching_threshold_is_strict(oracle):
+    # mm = 1: the threshold is the mean positive cost, 3.0 here; an edge at
+    # exactly the threshold is not eligible, one above it is
+    m, matched = _match(oracle, [(0, 1, 3.0), (2, 3, 3.0)], mm=1.0)
+    assert m.tolist() == [0, 1, 2, 3] and matched == 0
+    m, matched = _match(oracle, [(0, 1, 2.0), (2, 3, 4.0)], mm=1.0)
+    assert m.tolist() == [0, 1, 2, 2] and matched == 2
+
+
+def _increasing_path(first, k, base=1.0):
+    """Path first .. first+k-1 with costs base, base+1, ... toward the end:
+    round r of the matching pairs off the r-th pair from the far end."""
+    return [(first + t, first + t + 1, base + t) for t in range(k - 1)]
+
+
+def test_matching_stop_rule_ends_rounds_early(oracle):
+    # 20 disjoint pairs (cost 10) and a 6-node path of increasing costs:
+    # round 1 matches the pairs and the path's last pair (42 vertices), round
+    # 2 the next path pair (44, relative increase 2/43 < 0.1, the last round),
+    # so the path's first two nodes stay unmatched although a third round
+    # would pair them
+    edges = [(2 * q, 2 * q + 1, 10.0) for q in range(20)] + _increasing_path(40, 6)
+    m, matched = _match(oracle, edges)
+    assert matched == 44
+    assert m[40] == 40 and m[41] == 41
+    assert m[43] == 42 and m[45] == 44
+    assert all(m[2 * q + 1] == 2 * q for q in range(20))
+
+
+def test_matching_stops_after_ten_rounds(oracle):
+    # a 24-node path of increasing costs pairs one pair per round from the
+    # far end; every round's relative increase stays at or above 2/19 > 0.1,
+    # so only the 10-round cap ends it: 20 matched, the first 4 nodes not
+    m, matched = _match(oracle, _increasing_path(0, 24))
+    assert matched == 20
+    assert m[:4].tolist() == [0, 1, 2, 3]
+    assert all(m[2 * q + 1] == 2 * q for q in range(2, 12))
+
+
+# ---- the conflict-free (spanning forest) mapping
+# (src/edge_contractions_woc.cu:341-423, src/edge_contractions_woc_thrust.cu:513-599)
+
+def _cluster_checks(i, j, c, labels):
+    """No non-attractive edge (cost <= 0) inside a cluster, and every cluster
+    connected through its own attractive edges."""
+    lab = np.asarray(labels)
+    inside = lab[i] == lab[j]
+    assert not np.any(inside & (c <= 0)), "a repulsive edge lies inside a cluster"
+    n = len(lab)
+    keep = inside & (c > 0)
+    g = sp.coo_matrix((np.ones(int(keep.sum())), (i[keep], j[keep])), shape=(n, n))
+    k, comp = connected_components(g, directed=False)
+    # the clusters are exactly the components of their attractive edges
+    pairs = set(zip(lab.tolist(), comp.tolist()))
+    assert len(pairs) == len(set(lab.tolist())) == k
+
+
+@pytest.mark.parametrize("seed", range(60))
+def test_woc_mapping_is_conflict_free(oracle, seed):
+    rng = np.random.default_rng(1000 + seed)
+    n = int(rng.integers(6, 120))
+    i, j, c = random_graph(n, int(rng.integers(n, 4 * n)), seed=2000 + seed)
+    c = np.where(rng.uniform(size=len(c)) < 0.1, np.float32(0.0), c).astype(np.float32)
+    r, co, v, (rows, cols) = oracle.dcoo_init(i, j, c)
+    m, count = oracle.woc_mapping(r, co, v, rows, cols)
+    if len(m) == 0:  # no attractive edge at all
+        assert not np.any(v > 0)
+        return
+    # consecutive labels, numbered in order of each cluster's smallest node
+    assert sorted(set(m.tolist())) == list(range(int(m.max()) + 1))
+    first = [int(np.nonzero(m == q)[0][0]) for q in range(int(m.max()) + 1)]
+    assert first == sorted(first)
+    _cluster_checks(r, co, v, m)
+    # count = forest entries (both directions) kept: at most 2 (n - #clusters)
+    assert count <= 2 * (len(m) - (int(m.max()) + 1))
+
+
+def test_woc_thresholding_drops_weak_forest_edges(oracle):
+    # a 6-node path (costs 10, 11, 9.5, 12, 13) closed by a repulsive edge
+    # (0, 5): no triangle or quadrangle, so the frontier mapper runs; its
+    # first iteration drops forest edges below 0.9 x the mean forest cost
+    # (0.9 x 56.5 / 6 = 8.475), which removes the separate weak edge (6, 7);
+    # the conflicted cycle then loses its bottleneck edge (2, 3)
+    edges = [(0, 1, 10.0), (1, 2, 11.0), (2, 3, 9.5), (3, 4, 12.0), (4, 5, 13.0), (0, 5, -1.0), (6, 7, 1.0)]
+    i = np.array([e[0] for e in edges], np.int32)
+    j = np.array([e[1] for e in edges], np.int32)
+    c = np.array([e[2] for e in edges], np.float32)
+    r, co, v, (rows, cols) = oracle.dcoo_init(i, j, c)
+    m, count = oracle.woc_mapping(r, co, v, rows, cols)
+    assert m.tolist() == [0, 0, 0, 1, 1, 1, 2, 3]
+    assert count == 8
