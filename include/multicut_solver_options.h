@@ -1,13 +1,17 @@
 // multicut_solver_options — the solver's option set with the reference's
 // fields, defaults, presets and string form
-// (include/multicut_solver_options.h:4-127 of RAMA).  The command-line
-// parser (from_cl, CLI11) belongs to the CLI front end and is not part of
-// this header; to_c() converts to the C ABI struct of rama_hip.h.
+// (include/multicut_solver_options.h:4-127 of RAMA).  from_cl() parses the
+// reference's command line (its :82-105, which uses CLI11) with a small
+// parser of its own; to_c() converts to the C ABI struct of rama_hip.h.
 #pragma once
 
+#include <sys/stat.h>
+
+#include <cstdlib>
 #include <iostream>
 #include <stdexcept>
 #include <string>
+#include <vector>
 
 #include "rama_hip.h"
 
@@ -91,6 +95,112 @@ struct multicut_solver_options {
     }
 
     void print() const { std::cout << this->get_string(); }
+
+    // Command line of the reference's front end (include/multicut_solver_options.h:82-105):
+    //   -f,--file,file_pos PATH (required, must exist)   -o,--out_sol_file PATH
+    //   positionals after the file: max_cycle_dual [0,5], dual_itr >= 0,
+    //   max_cycle_primal [0,5], dual_itr_primal >= 0, dual_itr_outer >= 0,
+    //   mean_multiplier_mm >= 0, matching_thresh_crossover_ratio >= 0,
+    //   tri_memory_factor > 0
+    //   flags --only_lb, --dump_timeline, --sanitize_graph; -h,--help
+    // Returns -1 when the solver should run, else the process exit code
+    // (0 after --help; CLI11's codes for errors: 104 conversion, 105
+    // validation, 106 missing required option, 109 unexpected arguments,
+    // 114 an option without its value).
+    int from_cl(int argc, char** argv) {
+        enum { kConversion = 104, kValidation = 105, kRequired = 106, kExtras = 109, kMismatch = 114 };
+        auto fail = [](int code, const std::string& msg) {
+            std::cerr << msg << "\nRun with --help for more information.\n";
+            return code;
+        };
+        struct Pos {
+            const char* name;
+            int* i;
+            float* f;
+            double lo, hi;
+            bool lo_open;
+        };
+        const double inf = 1e300;
+        std::vector<Pos> pos = {
+            {"max_cycle_dual", &max_cycle_length_lb, nullptr, 0, 5, false},
+            {"dual_itr", &num_dual_itr_lb, nullptr, 0, inf, false},
+            {"max_cycle_primal", &max_cycle_length_primal, nullptr, 0, 5, false},
+            {"dual_itr_primal", &num_dual_itr_primal, nullptr, 0, inf, false},
+            {"dual_itr_outer", &num_outer_itr_dual, nullptr, 0, inf, false},
+            {"mean_multiplier_mm", nullptr, &mean_multiplier_mm, 0, inf, false},
+            {"matching_thresh_crossover_ratio", nullptr, &matching_thresh_crossover_ratio, 0, inf, false},
+            {"tri_memory_factor", nullptr, &tri_memory_factor, 0, inf, true},
+        };
+        bool have_file = false;
+        size_t next = 0;
+        const std::string prog = argc > 0 ? argv[0] : "rama_text_input";
+        for (int k = 1; k < argc; ++k) {
+            std::string a = argv[k], val;
+            const auto eq = a.find('=');
+            const bool is_long = a.rfind("--", 0) == 0;
+            if (is_long && eq != std::string::npos) {
+                val = a.substr(eq + 1);
+                a = a.substr(0, eq);
+            }
+            auto take = [&](std::string& dst) -> int {
+                if (!val.empty() || (is_long && eq != std::string::npos)) {
+                    dst = val;
+                    return 0;
+                }
+                if (k + 1 >= argc) return fail(kMismatch, a + " requires a value");
+                dst = argv[++k];
+                return 0;
+            };
+            if (a == "-h" || a == "--help") {
+                std::cout << "Solver for multicut problem. \nUsage: " << prog
+                          << " [OPTIONS] file_pos [max_cycle_dual] [dual_itr] [max_cycle_primal] [dual_itr_primal]"
+                             " [dual_itr_outer] [mean_multiplier_mm] [matching_thresh_crossover_ratio]"
+                             " [tri_memory_factor]\n\nOptions:\n"
+                             "  -h,--help                   Print this help message and exit\n"
+                             "  -f,--file,file_pos PATH     Path to multicut instance (.txt) (required)\n"
+                             "  -o,--out_sol_file PATH      Path to save node labeling (.txt)\n"
+                             "  --only_lb                   Only compute the lower bound\n"
+                             "  --dump_timeline             Return the output of each contraction step\n"
+                             "  --sanitize_graph            Input has nodes without edges (their labels are -1)\n";
+                return 0;
+            } else if (a == "-f" || a == "--file") {
+                if (int e = take(input_file)) return e;
+                have_file = true;
+            } else if (a == "-o" || a == "--out_sol_file") {
+                if (int e = take(output_sol_file)) return e;
+            } else if (a == "--only_lb") {
+                only_compute_lb = true;
+            } else if (a == "--dump_timeline") {
+                dump_timeline = true;
+            } else if (a == "--sanitize_graph") {
+                sanitize_graph = true;
+            } else if (a.size() > 1 && a[0] == '-' && !(a[1] >= '0' && a[1] <= '9') && a[1] != '.') {
+                return fail(kExtras, "The following argument was not expected: " + a);
+            } else if (!have_file) {
+                input_file = a;
+                have_file = true;
+            } else if (next < pos.size()) {
+                const Pos& p = pos[next++];
+                char* end = nullptr;
+                const double v = p.i ? (double)std::strtol(a.c_str(), &end, 10) : std::strtod(a.c_str(), &end);
+                if (a.empty() || end == nullptr || *end != '\0')
+                    return fail(kConversion, "Could not convert: " + std::string(p.name) + " = " + a);
+                if (v < p.lo || (p.lo_open && v <= p.lo) || v > p.hi)
+                    return fail(kValidation, std::string(p.name) + ": Value " + a + " not in range [" +
+                                                 std::to_string(p.lo) + " - " +
+                                                 (p.hi >= inf ? std::string("inf") : std::to_string(p.hi)) +
+                                                 (p.lo_open ? "], exclusive of the lower end" : "]"));
+                if (p.i) *p.i = (int)v; else *p.f = (float)v;
+            } else {
+                return fail(kExtras, "The following argument was not expected: " + a);
+            }
+        }
+        if (!have_file) return fail(kRequired, "--file is required");
+        struct stat st;
+        if (stat(input_file.c_str(), &st) != 0)
+            return fail(kValidation, "--file: Path does not exist: " + input_file);
+        return -1;
+    }
 
     rama_options_c to_c() const {
         rama_options_c o;

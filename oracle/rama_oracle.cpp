@@ -28,6 +28,7 @@
 #include <cfloat>
 #include <chrono>
 #include <cmath>
+#include <cstdlib>
 #include <cstdint>
 #include <cstring>
 #include <numeric>
@@ -938,6 +939,9 @@ SolveOut solve_dcoo(Coo& A, const rama_options_c& o) {
         if (o.dump_timeline) out.timeline.push_back(labels);
         if (o.max_time_sec >= 0) {
             auto t = std::chrono::duration_cast<std::chrono::seconds>(std::chrono::steady_clock::now() - begin).count();
+            // RAMA_CLOCK_OFFSET_SEC: whole seconds added to the elapsed time,
+            // the same test hook as the GPU solver's
+            if (const char* off = std::getenv("RAMA_CLOCK_OFFSET_SEC")) t += std::atol(off);
             if (t > o.max_time_sec) break;
         }
     }

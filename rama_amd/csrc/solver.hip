@@ -159,7 +159,11 @@ void solve_rounds(Ctx& ctx, DCoo& A, SolveResult& out, const Options& o, int sta
             out.timeline.push_back(std::move(snap));
         }
         if (o.max_time_sec >= 0) {
-            const auto t = std::chrono::duration_cast<std::chrono::seconds>(std::chrono::steady_clock::now() - t0).count();
+            // whole elapsed seconds, strictly over the budget (src/rama_cuda.cu:151-157);
+            // RAMA_CLOCK_OFFSET_SEC adds whole seconds to the elapsed time, a
+            // test hook that makes the rule observable on sub-second solves
+            auto t = std::chrono::duration_cast<std::chrono::seconds>(std::chrono::steady_clock::now() - t0).count();
+            if (const char* off = std::getenv("RAMA_CLOCK_OFFSET_SEC")) t += std::atol(off);
             if (t > o.max_time_sec) break;
         }
     }

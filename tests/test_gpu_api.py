@@ -195,3 +195,30 @@ def test_full_size_equals_oracle(gpu, name):
     assert mk.partition_hash(lab) == ref["partition_sha256"]
     assert mk.objective(lab, i, j, c) == ref["objective"]
     assert abs(lb - ref["lower_bound"]) <= 1e-9 * abs(ref["lower_bound"])
+
+
+@pytest.mark.parametrize("name", ["grid96x128_lr", "er3000"])
+def test_time_budget_against_oracle(gpu, oracle, monkeypatch, name):
+    # the budget is checked after each contraction round on whole elapsed
+    # seconds, strictly over max_time_sec (src/rama_cuda.cu:151-157);
+    # RAMA_CLOCK_OFFSET_SEC shifts the elapsed time on both sides so the
+    # rule shows on solves shorter than a second
+    i, j, c = named(name)
+    full, full_lb, _, _ = gpu.solve(i, j, c, gpu.options("PD", verbose=0))
+    # a budget the solve never reaches changes nothing
+    lab, lb, _, tl = gpu.solve(i, j, c, gpu.options("PD", verbose=0, max_time_sec=0), want_timeline=True)
+    assert np.array_equal(lab, full) and lb == full_lb
+    monkeypatch.setenv("RAMA_CLOCK_OFFSET_SEC", "1")
+    # 1 s elapsed against a budget of 1 s: not over it, the solve runs on
+    lab, lb, _, _ = gpu.solve(i, j, c, gpu.options("PD", verbose=0, max_time_sec=1))
+    assert np.array_equal(lab, full) and lb == full_lb
+    # against a budget of 0 s: the solve stops after its first round
+    o = gpu.options("PD", verbose=0, max_time_sec=0, dump_timeline=1)
+    lab, lb, _, tl = gpu.solve(i, j, c, o, want_timeline=True)
+    assert len(tl) == 1 and np.array_equal(tl[0], lab)
+    assert lb == full_lb  # the bound is the initial dual pass's
+    assert len(np.unique(lab)) > len(np.unique(full))
+    ref, ref_lb, iters = oracle.solve(i, j, c, oracle.options("PD", max_time_sec=0))
+    assert iters == 1
+    assert _same_partition(lab, ref)
+    assert abs(lb - ref_lb) <= 1e-9 * max(1.0, abs(ref_lb))
