@@ -176,11 +176,13 @@ def _full_size_module():
     return mod
 
 
-@pytest.mark.parametrize("name", ["grid2d_voronoi_1024x2048_PD", "grid3d_voronoi_192_PDplus"])
+@pytest.mark.parametrize("name", ["grid2d_voronoi_1024x2048_PD", "grid3d_voronoi_192_PDplus",
+                                  "grid2d_voronoi_1024x2048_D"])
 def test_full_size_equals_oracle(gpu, name):
     """The whole partition of each full-size workload equals the oracle's
     (hash of the labels renumbered by first occurrence), with the same
-    objective; the bound agrees to 1e-9 relative (reduction order)."""
+    objective; the bound agrees to 1e-9 relative (reduction order).  In D
+    mode (lower bound only) the bound is the whole result."""
     import json
     import os
 
@@ -191,6 +193,10 @@ def test_full_size_equals_oracle(gpu, name):
     i, j, c, _ = gen()
     assert len(c) == ref["edges"]
     lab, lb, _, _ = gpu.solve(i, j, c, gpu.options(preset, verbose=0))
+    assert abs(lb - ref["lower_bound"]) <= 1e-9 * abs(ref["lower_bound"])
+    if preset == "D":  # lower bound only: no labels
+        assert len(lab) == 0
+        return
     assert len(np.unique(lab)) == ref["clusters"]
     assert mk.partition_hash(lab) == ref["partition_sha256"]
     assert mk.objective(lab, i, j, c) == ref["objective"]
