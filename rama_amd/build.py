@@ -5,7 +5,8 @@
 
 hipcc compiles each .hip translation unit for gfx950 in parallel and links
 them into rama_amd/librama_hip.so.  rama_py (pybind11) is host-only C++ that
-calls the C ABI, built with g++ into rama_amd/rama_py<EXT_SUFFIX>.
+calls the C ABI, built with g++ into rama_amd/rama_py<EXT_SUFFIX>; the
+command-line front end rama_amd/rama_text_input likewise.
 """
 import concurrent.futures as cf
 import os
@@ -86,16 +87,32 @@ def build_py():
     return out
 
 
-def build(lib=True, py=True):
+def build_cli():
+    """rama_text_input: the command-line front end over include/rama_cuda.h
+    (host-only C++, g++, linked against librama_hip.so)."""
+    out = os.path.join(HERE, "rama_text_input")
+    tmp = out + ".tmp%d" % os.getpid()
+    cmd = [
+        "g++", "-O2", "-std=c++17", "-I" + os.path.join(ROOT, "include"), os.path.join(CSRC, "rama_text_input.cpp"),
+        "-o", tmp, "-L" + HERE, "-lrama_hip", "-Wl,-rpath,$ORIGIN",
+    ]
+    _run(cmd)
+    os.replace(tmp, out)
+    return out
+
+
+def build(lib=True, py=True, cli=True):
     built = []
     if lib:
         built.append(build_lib())
     if py and os.path.exists(os.path.join(CSRC, "rama_py.cpp")):
         built.append(build_py())
+    if cli:
+        built.append(build_cli())
     return built
 
 
 if __name__ == "__main__":
     only_lib = "--lib" in sys.argv
-    for p in build(lib=True, py=not only_lib):
+    for p in build(lib=True, py=not only_lib, cli=not only_lib):
         print(p)
