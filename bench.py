@@ -55,10 +55,13 @@ def parse():
                     help="grid2d: H x W grid with long-range offsets, 2000 seeds; "
                          "grid3d: D^3 grid, 6-neighbourhood, 20000 seeds (run with --preset PD+)")
     ap.add_argument("--D", type=int, default=192)
-    ap.add_argument("--mode", choices=["plain", "batch", "serve", "copy"], default="plain",
+    ap.add_argument("--mode", choices=["plain", "batch", "serve", "copy", "tune"], default="plain",
                     help="plain: the default line; batch: 512 mixed instances batched vs one at a time; "
                          "serve: Poisson arrivals against the serving loop and the plain loop; "
-                         "copy: one 1 GiB device copy (calibrates HBM byte counters)")
+                         "copy: one 1 GiB device copy (calibrates HBM byte counters); "
+                         "tune: measure the launch-shape knobs and write the tuning table")
+    ap.add_argument("--tune-out", default=os.path.join(ROOT, "rama_amd", "tuning", "gfx950.table"),
+                    help="tune mode: where the table is written")
     ap.add_argument("--count", type=int, default=512, help="instances in batch mode")
     ap.add_argument("--batched-only", action="store_true",
                     help="batch mode: run the batched solve alone (for profiling), no one-at-a-time loop")
@@ -222,6 +225,146 @@ def serving_block(loads=(0.5, 0.9, 1.5), requests=160, workers=16):
     return out
 
 
+# Launch-shape candidates the tuner measures (tuning knobs of include/rama_hip.h).
+TUNE_SORT_SIZES = [1 << k for k in range(14, 25)] + [24 << 20]
+TUNE_SORT_KEYS = ((8, 42, "sort_path_u64"), (4, 21, "sort_path_u32"))
+TUNE_SOLVE_KNOBS = {
+    "pick_blocks": (1024, 2048, 4096, 8192),
+    "match_blocks": (512, 1024, 2048, 4096),
+    "mp_fused_max_blocks": (0, 64, 256, 1024),
+    "scan_chunks": (512, 1024, 2048, 4096),
+    "cycle_chunk_pct": (110, 125, 150, 200),
+}
+
+
+def _buckets(sizes, values):
+    """Rows (max_size, value) from per-size choices: size s_k covers up to the
+    geometric midpoint with the next sampled size, the last one everything
+    larger; neighbouring rows with one value merge."""
+    rows = []
+    for k, (s, v) in enumerate(zip(sizes, values)):
+        hi = int((s * sizes[k + 1]) ** 0.5) if k + 1 < len(sizes) else -1
+        if rows and rows[-1][1] == v:
+            rows[-1] = (hi, v)
+        else:
+            rows.append((hi, v))
+    return rows
+
+
+def tune_mode(a):
+    """Measure the launch-shape knobs on this GPU and write the tuning table.
+
+    Sorts: every path (three launches per pass, onesweep, top-digit +
+    per-bucket blocks) at 512 and 1024 threads per tile, timed by
+    rama_tune_time_sort on random keys of the widths the solver sorts most
+    (42-bit u64 (row, col) keys of a 2M-node graph, 21-bit u32 node keys) at
+    sizes 16K .. 24M pairs; each size bucket gets its fastest path, and
+    threads per tile the shape fastest summed over both key types.  Grid
+    bounds and chunking: coordinate descent on the PD solve time (median of
+    three) of the 1024x2048 grid workload.  None of the knobs changes a
+    result; the solve's partition is checked to stay the same."""
+    import ctypes
+
+    import torch
+
+    from rama_amd import _capi, instances
+
+    L = _capi.lib()
+    L.rama_tuning_load(None)  # measure from the built-in defaults
+    reps = 5
+
+    def tsort(n, kb, bits, path, thr):
+        ms = ctypes.c_double()
+        _capi._check(L.rama_tune_time_sort(n, kb, bits, path, thr, reps, ctypes.byref(ms)))
+        return ms.value
+
+    report = {"sorts": {}, "solve": {}}
+    rows = {}
+    thr_time = {512: [0.0] * len(TUNE_SORT_SIZES), 1024: [0.0] * len(TUNE_SORT_SIZES)}
+    for kb, bits, knob in TUNE_SORT_KEYS:
+        choice = []
+        for si, n in enumerate(TUNE_SORT_SIZES):
+            times = {}
+            for path in (0, 1, 2):
+                if path == 2 and n > 1024 * 4096:
+                    continue
+                for thr in (512, 1024):
+                    times[(path, thr)] = tsort(n, kb, bits, path, thr)
+            best_path = min((0, 1, 2), key=lambda p: min(times.get((p, t), 1e30) for t in (512, 1024)))
+            choice.append(best_path)
+            for t in (512, 1024):
+                thr_time[t][si] += times[(best_path, t)]
+            report["sorts"]["%s n=%d" % (knob, n)] = {"%d/%d" % k: round(v, 4) for k, v in times.items()}
+            progress("tune %s n=%d: path %d (%s)" % (knob, n, best_path,
+                                                    ", ".join("%d/%d %.3f" % (k[0], k[1], v) for k, v in times.items())))
+        rows[knob] = _buckets(TUNE_SORT_SIZES, choice)
+    tiles = [(n + 4095) // 4096 for n in TUNE_SORT_SIZES]
+    rows["sort_threads"] = _buckets(tiles, [512 if thr_time[512][k] < thr_time[1024][k] else 1024
+                                            for k in range(len(tiles))])
+    for knob, rr in rows.items():
+        for ms, v in rr:
+            _capi._check(L.rama_tuning_set(knob.encode(), ms, v))
+
+    i, j, c, _ = instances.grid2d_voronoi(H=1024, W=2048, seed=0)
+    dev = torch.device("cuda", 0)
+    di, dj, dc = (torch.from_numpy(x).to(dev) for x in (i, j, c))
+    labels = torch.empty(int(max(i.max(), j.max())) + 2, dtype=torch.int32, device=dev)
+    stream = torch.cuda.current_stream().cuda_stream
+    opts = _capi.options("PD", verbose=0)
+
+    def solve_ms():
+        ts = []
+        for _ in range(3):
+            torch.cuda.synchronize()
+            t = time.perf_counter()
+            n_lab, _, _, _ = _capi.solve_device(di.data_ptr(), dj.data_ptr(), dc.data_ptr(), len(c), labels.data_ptr(),
+                                                labels.numel(), opts, device=0, stream=stream)
+            torch.cuda.synchronize()
+            ts.append(time.perf_counter() - t)
+        return float(np.median(ts)) * 1e3, labels[:n_lab].cpu().numpy().copy()
+
+    base_ms, ref = solve_ms()
+    base_ms, ref = solve_ms()
+    for knob, cands in TUNE_SOLVE_KNOBS.items():
+        res = {}
+        for v in cands:
+            _capi._check(L.rama_tuning_set(knob.encode(), -1, v))
+            ms, lab = solve_ms()
+            if not np.array_equal(lab, ref):
+                raise RuntimeError("tuning knob %s=%d changed the partition" % (knob, v))
+            res[v] = ms
+        best = min(res, key=res.get)
+        _capi._check(L.rama_tuning_set(knob.encode(), -1, best))
+        rows[knob] = [(-1, best)]
+        report["solve"][knob] = {str(k): round(v, 3) for k, v in res.items()}
+        progress("tune %s: %d (%s)" % (knob, best, ", ".join("%d %.2f ms" % kv for kv in res.items())))
+    final_ms, _ = solve_ms()
+    props = torch.cuda.get_device_properties(0)
+    lines = ["# rama_amd launch-shape table for %s (%s), written by `python bench.py --mode tune`."
+             % (getattr(props, "gcnArchName", "gfx950").split(":")[0], props.name),
+             "# Each knob: rows 'knob max_size value' in increasing max_size; -1 = no upper limit.",
+             "# Sorts: fastest of (0 three-launch, 1 onesweep, 2 top-digit + buckets) x (512, 1024 threads per",
+             "# tile) per size; the rest: coordinate descent on the 1024x2048 grid PD solve",
+             "# (%.2f ms with the defaults, %.2f ms with this table)." % (base_ms, final_ms)]
+    for k in ("sort_path_u64", "sort_path_u32", "sort_threads") + tuple(TUNE_SOLVE_KNOBS):
+        for ms, v in rows[k]:
+            lines.append("%s %d %d" % (k, ms, v))
+    os.makedirs(os.path.dirname(os.path.abspath(a.tune_out)), exist_ok=True)
+    with open(a.tune_out, "w") as f:
+        f.write("\n".join(lines) + "\n")
+    report.update({"table": a.tune_out, "default_ms": base_ms, "tuned_ms": final_ms, "rows": rows})
+    return report
+
+
+def _tuning_source():
+    """The launch-shape table the library runs with (a path relative to the
+    repository, or "defaults")."""
+    from rama_amd import _capi
+
+    src = _capi.lib().rama_tuning_source().decode()
+    return os.path.relpath(src, ROOT) if os.path.isabs(src) else src
+
+
 def progress(msg):
     print("[bench] " + msg, file=sys.stderr, flush=True)
 
@@ -234,6 +377,9 @@ def main():
         sys.exit(2)
     if a.mode == "copy":
         print(json.dumps(copy_mode()))
+        return
+    if a.mode == "tune":
+        print(json.dumps({"mode": "tune", "tuning": tune_mode(a)}))
         return
     # more hardware queues so the batch workers' and serving lanes' streams
     # run side by side (set before the first HIP call of the process)
@@ -332,6 +478,7 @@ def main():
                    "clusters": int(len(np.unique(lab)))},
         "roofline": roof,
         "kernels": profiler.top(all_stats, a.kernels),
+        "tuning": _tuning_source(),
     }
     progress("plain PD: %.2f ms per solve" % (dt * 1e3))
     del di, dj, dc, labels

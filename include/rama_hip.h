@@ -246,6 +246,37 @@ int rama_profile_entry(int64_t k, const char** name, int64_t* launches, double* 
 void rama_profile_only(const char* name);
 
 /* ------------------------------------------------------------------------
+ * Launch-shape table (no counterpart in the reference, whose launches use
+ * fixed shapes, e.g. 256 threads in src/conflicted_cycles_cuda.cu:240-241
+ * and src/maximum_matching_vertex_based.cu:8, and Thrust's own sort
+ * policies).  Knobs, each keyed by a size: sort_path_u64 / sort_path_u32
+ * (pairs -> 0 three launches per pass, 1 onesweep, 2 top-digit + per-bucket
+ * blocks), sort_threads (tiles -> 512 or 1024 threads per 4096-pair tile),
+ * scan_chunks (items -> blocks of the integer scan), pick_blocks (CSR
+ * entries) and match_blocks (vertices) for the matching grids,
+ * mp_fused_max_blocks (message-passing edges -> largest edge grid that sums
+ * its own lower bound), cycle_chunk_pct (expansions -> growth of the chunked
+ * 4/5-cycle counts).  None of them changes a result.  The table is read on
+ * first use from RAMA_TUNING_TABLE, else tuning/gfx950.table beside the
+ * library; "none" or no file keeps the built-in defaults.
+ * ------------------------------------------------------------------------ */
+
+/* Replace the table by the file at `path` ("knob max_size value" lines;
+ * max_size -1 = no limit); NULL or "" = the built-in defaults. */
+int rama_tuning_load(const char* path);
+/* Add or replace the row (knob, max_size) of the table in use. */
+int rama_tuning_set(const char* knob, int64_t max_size, int64_t value);
+/* Value of `knob` for `size` (-1 for an unknown knob). */
+int64_t rama_tuning_get(const char* knob, int64_t size);
+/* Where the table in use came from: a path, "defaults", or "modified ..." */
+const char* rama_tuning_source(void);
+/* Mean device time in ms of `reps` stable sorts of n random (key, u32)
+ * pairs (key_bytes 4 or 8, keys of `bits` bits) by a given path (0, 1, 2 as
+ * sort_path_*) and threads per tile (512 or 1024): what the tuner measures. */
+int rama_tune_time_sort(int64_t n, int32_t key_bytes, int32_t bits, int32_t path, int32_t threads, int32_t reps,
+                        double* ms_out);
+
+/* ------------------------------------------------------------------------
  * Device memory cache (no counterpart in the reference, whose thrust
  * vectors allocate and free per call).  Blocks a finished solve used stay
  * cached for the next solve on the same device, up to RAMA_CACHE_BYTES

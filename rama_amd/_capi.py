@@ -78,6 +78,11 @@ SIGNATURES = {
     "rama_server_submit": ([VP, I32P, I32P, F32P, I64], I64),
     "rama_server_poll": ([VP, I64], I32),
     "rama_server_wait": ([VP, I64, I32P, I64, I64P, F64P, F64P, F64P], I32),
+    "rama_tuning_load": ([ctypes.c_char_p], I32),
+    "rama_tuning_set": ([ctypes.c_char_p, I64, I64], I32),
+    "rama_tuning_get": ([ctypes.c_char_p, I64], I64),
+    "rama_tuning_source": ([], ctypes.c_char_p),
+    "rama_tune_time_sort": ([I64, I32, I32, I32, I32, I32, F64P], I32),
     "rama_server_destroy": ([VP], None),
     "rama_profile_enable": ([I32], None),
     "rama_profile_only": ([ctypes.c_char_p], None),

@@ -21,7 +21,7 @@ BUILD = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "librama_hip.so")
 ARCH = os.environ.get("RAMA_OFFLOAD_ARCH", "gfx950")
 
-HIP_SOURCES = ["profiler", "pool", "prims", "dcoo", "matching", "cycles", "mp", "dual", "forest", "solver", "capi", "batch", "lockstep"]
+HIP_SOURCES = ["profiler", "pool", "prims", "dcoo", "matching", "cycles", "mp", "dual", "forest", "solver", "capi", "batch", "lockstep", "tuning"]
 
 HIPCC_FLAGS = [
     "--offload-arch=" + ARCH,

@@ -16,6 +16,7 @@
 // mids and writes at enumeration position (rep edge, expansion, mid).  The
 // triangle budget therefore keeps exactly the first `budget` writes.
 #include "rama_internal.hpp"
+#include "tuning.hpp"
 
 namespace rama {
 
@@ -777,10 +778,12 @@ static i64 run_stage(Ctx& ctx, const Csr& G, const int* rv1, const int* rv2, i64
                 }
             }
             if (hc[0] >= need) break;
-            // next chunk: 1.25x the blocks that the rate so far says reach
-            // the budget (all the rest when it says they will not)
+            // next chunk: cycle_chunk_pct percent (default 125) of the
+            // blocks that the rate so far says reach the budget (all the
+            // rest when it says they will not)
             const double per_block = (double)hc[0] / (double)done;
-            const double want = per_block > 0.0 ? 1.25 * (double)(need - hc[0]) / per_block + 64.0 : 1e30;
+            const double grow = 0.01 * (double)tune::get(tune::CycleChunkPct, X);
+            const double want = per_block > 0.0 ? grow * (double)(need - hc[0]) / per_block + 64.0 : 1e30;
             chunk = want >= (double)(nblk - done) ? nblk - done : (unsigned)want;
         }
         if (done < nblk) {

@@ -8,6 +8,7 @@
 // are queued at once and the host reads the outcome once.  The result is
 // deterministic: mutual pairs are disjoint, so racing writes agree.
 #include "rama_internal.hpp"
+#include "tuning.hpp"
 
 namespace rama {
 
@@ -86,9 +87,8 @@ __device__ __forceinline__ bool loop_round_active(const MatchLoop* st, float mul
     return true;
 }
 
-// Bounded grids: the rounds after the loop has stopped cost a launch each.
-constexpr unsigned kPickBlocks = 2048;
-constexpr unsigned kMatchBlocks = 1024;
+// Bounded grids (tuning table: pick_blocks, match_blocks): the rounds after
+// the loop has stopped cost a launch each.
 
 // Grid-stride over (vertex, lane) pairs.  A vertex's flag and offsets are
 // loaded together (not the offsets after the flag), and each lane takes two
@@ -233,8 +233,8 @@ int matching_vertex_based(Ctx& ctx, const Csr& U, float mean_multiplier, DBuf<in
         },
         U.nnz, &st->head.pos_sum, &st->head.pos_cnt);
     const i64 threads = (i64)n * kGroup;
-    const unsigned pick_grid = std::min<unsigned>(grid_for(threads), kPickBlocks);
-    const unsigned match_grid = std::min<unsigned>(grid_for(n), kMatchBlocks);
+    const unsigned pick_grid = std::min<unsigned>(grid_for(threads), (unsigned)tune::get(tune::PickBlocks, threads));
+    const unsigned match_grid = std::min<unsigned>(grid_for(n), (unsigned)tune::get(tune::MatchBlocks, n));
     // the stop rule runs on the device; the host only skips launches once it
     // knows the loop has ended: the first kFirstRounds rounds are queued,
     // the state is read, and the remaining rounds are queued only if the

@@ -18,6 +18,7 @@
 // the edge kernel's last block sums them and applies the stall test, so the
 // host reads the outcome once per loop.
 #include "rama_internal.hpp"
+#include "tuning.hpp"
 
 namespace rama {
 
@@ -485,8 +486,7 @@ static void launch_round(Ctx& ctx, MPState& S, bool sweep, MPControl* ctl = null
     }
     // the device-driven loop (ctl set) finalises each iteration in the edge
     // kernel on small grids, in a separate launch on large ones
-    constexpr unsigned kFusedFinalizeMaxBlocks = 256;
-    const bool fused = ctl && ge <= kFusedFinalizeMaxBlocks;
+    const bool fused = ctl && ge <= (unsigned)tune::get(tune::MpFusedMaxBlocks, E);
     const double* parts = fused ? P : nullptr;
     const i64 nparts = (i64)gt + (i64)ge;
     unsigned* ticket = fused ? ctx.reduce_ticket() : nullptr;

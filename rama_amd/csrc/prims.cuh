@@ -5,9 +5,11 @@
 #pragma once
 
 #include <cstring>
+#include <type_traits>
 
 #include "common.hpp"
 #include "profiler.hpp"
+#include "tuning.hpp"
 
 namespace rama {
 
@@ -130,7 +132,7 @@ __device__ __forceinline__ u64 scan_word(u32 epoch, u32 flag, u64 v) {
 // and no waiting on other blocks.  One chunk: the second launch only.
 constexpr int kScanItems = 8;
 constexpr int kScanTile = kBlock * kScanItems;
-constexpr unsigned kScanMaxBlocks = 2048;
+constexpr unsigned kScanMaxBlocks = 4096;  // the tuning table picks up to this many
 
 // Items are evaluated and emitted in striped order (item k*kBlock + t of the
 // tile by thread t), so the loads inside f and the stores inside out are
@@ -229,8 +231,10 @@ void scan_apply(Ctx& ctx, F f, i64 n, Out out, T* total) {
         if (total) ctx.zero(total, 1);
         return;
     }
+    static_assert(std::is_integral<T>::value, "integer scans only: the chunking must not change the sums");
     const i64 nt = (n + kScanTile - 1) / kScanTile;
-    const i64 per = (nt + kScanMaxBlocks - 1) / kScanMaxBlocks;  // tiles per block
+    const i64 maxb = std::min<i64>(tune::get(tune::ScanChunks, n), kScanMaxBlocks);
+    const i64 per = (nt + maxb - 1) / maxb;  // tiles per block
     const i64 chunk = per * kScanTile;
     const unsigned g = (unsigned)((n + chunk - 1) / chunk);
     T* partials = nullptr;
@@ -617,6 +621,11 @@ void radix_sort_pairs(Ctx& ctx, K* keys, u32* vals, K* tk, u32* tv, i64 n, int b
 // Convenience: allocates the scratch itself.
 template <class K>
 void radix_sort_pairs(Ctx& ctx, DBuf<K>& keys, DBuf<u32>& vals, i64 n, int bits);
+// As the pointer form, with the sort path (0 three-launch, 1 onesweep,
+// 2 top-digit + per-bucket blocks) and threads per tile given instead of
+// taken from the tuning table (the tuner times paths with it).
+template <class K>
+void radix_sort_pairs_shaped(Ctx& ctx, K* keys, u32* vals, K* tk, u32* tv, i64 n, int bits, int path, int threads);
 
 inline int bits_for(i64 max_value) {
     int b = 1;

@@ -1,15 +1,19 @@
 // prims.hip — stable LSD radix sort of (u32 or u64 key, u32 value) pairs.
 //
-// Digits of at most 8 bits, balanced over the key width.  One launch builds
-// the digit histograms of every pass; then one "onesweep" launch per pass:
-// wave64 ballot multi-split ranking inside a 4096-pair tile, decoupled
-// look-back across tiles for each digit's global offset, and an LDS reorder
-// so each digit run is written out contiguously.  Items are ranked in
-// (wave, item, lane) order, which is their input order, so each pass is
-// stable and the whole sort is stable.
+// Digits of at most 8 bits, balanced over the key width.  Inside a
+// 4096-pair tile every path ranks by wave64 ballot multi-split and reorders
+// through LDS so each digit run is written out contiguously; items are
+// ranked in (wave, item, lane) order, which is their input order, so each
+// pass is stable and the whole sort is stable.  Across tiles, per array size
+// (tuning table): three launches per pass (per-tile histograms, their scan,
+// the scatter), or one histogram launch for every pass and then one
+// "onesweep" launch per pass with decoupled look-back for each digit's
+// global offset, or a top-digit onesweep pass followed by one block per
+// bucket.  Arrays of at most 8192 pairs are sorted by one block.
 #include <cstdlib>
 
 #include "prims.cuh"
+#include "tuning.hpp"
 
 namespace rama {
 
@@ -61,16 +65,18 @@ __device__ __forceinline__ void wave_count(int* h, u32 d, bool ok) {
     if ((active >> lane) & 1ull) atomicAdd(&h[d], 1);
 }
 
-// The radix passes run 1024 threads per 4096-pair tile (4 pairs each): few
-// registers per thread, so many waves per SIMD within the same LDS (faster
-// than 512 threads with 8 pairs, and those than 256 with 16).
-constexpr int kScThreads = 1024;
-constexpr int kScItems = kRsTile / kScThreads;
-constexpr int kScWaves = kScThreads / 64;
+// The radix passes run NT threads per 4096-pair tile: 1024 (4 pairs each,
+// few registers per thread, so many waves per SIMD within the same LDS) or
+// 512 (8 pairs each), as the tuning table says for the array's tile count.
+#define RAMA_SC_SHAPE(NT)                     \
+    constexpr int kScThreads = (NT);          \
+    constexpr int kScItems = kRsTile / (NT);  \
+    constexpr int kScWaves = (NT) / 64;
 
-template <class K>
-__global__ __launch_bounds__(kScThreads) void k_radix_hist(const K* __restrict__ keys, i64 n, int shift, int bits,
-                                                           int* hist, int ntiles) {
+template <class K, int NT>
+__global__ __launch_bounds__(NT) void k_radix_hist(const K* __restrict__ keys, i64 n, int shift, int bits, int* hist,
+                                                   int ntiles) {
+    RAMA_SC_SHAPE(NT)
     __shared__ int h[256];
     if (threadIdx.x < 256) h[threadIdx.x] = 0;
     __syncthreads();
@@ -105,11 +111,11 @@ __device__ __forceinline__ unsigned long long digit_peers(u32 d, int bits) {
     return peers;
 }
 
-template <class K>
-__global__ __launch_bounds__(kScThreads) void k_radix_scatter(const K* __restrict__ kin, const u32* __restrict__ vin,
-                                                              K* __restrict__ kout, u32* __restrict__ vout, i64 n,
-                                                              int shift, int bits, const int* __restrict__ offs,
-                                                              int ntiles) {
+template <class K, int NT>
+__global__ __launch_bounds__(NT) void k_radix_scatter(const K* __restrict__ kin, const u32* __restrict__ vin,
+                                                      K* __restrict__ kout, u32* __restrict__ vout, i64 n, int shift,
+                                                      int bits, const int* __restrict__ offs, int ntiles) {
+    RAMA_SC_SHAPE(NT)
     __shared__ K sk[kRsTile];
     __shared__ u32 sv[kRsTile];
     __shared__ int wcnt[kScWaves][256];
@@ -186,13 +192,6 @@ struct PassPlan {
     int width[8];
 };
 
-// Arrays of at most this many tiles use the onesweep path (one launch per
-// pass); larger ones the three-launch pass.
-constexpr int kOnesweepMaxTiles = 64;
-// Arrays of at most this many tiles (and more than one block sort holds)
-// take the top-digit pass and the per-bucket block sorts.
-constexpr int kBucketMaxTiles = 64;
-
 // Digit histograms of every pass in one read of the keys.  Each block
 // stores its partial histograms (row blockIdx.x of `part`); the last block
 // to finish (atomic ticket, reset for the next launch) adds the rows in
@@ -245,11 +244,12 @@ __global__ __launch_bounds__(kBlock) void k_radix_hist_all(const K* __restrict__
 // back along its digit for the exclusive prefix over earlier tiles, and the
 // tile is reordered in LDS and written out in digit runs.  Status words are
 // (epoch | flag | count) as in the single-pass scan.
-template <class K>
-__global__ __launch_bounds__(kScThreads) void k_onesweep(const K* __restrict__ kin, const u32* __restrict__ vin,
-                                                     K* __restrict__ kout, u32* __restrict__ vout, i64 n, int shift,
-                                                     int bits, const int* __restrict__ hist_pass,
-                                                     u64* __restrict__ status, int* ctl, u32 epoch) {
+template <class K, int NT>
+__global__ __launch_bounds__(NT) void k_onesweep(const K* __restrict__ kin, const u32* __restrict__ vin,
+                                                 K* __restrict__ kout, u32* __restrict__ vout, i64 n, int shift,
+                                                 int bits, const int* __restrict__ hist_pass, u64* __restrict__ status,
+                                                 int* ctl, u32 epoch) {
+    RAMA_SC_SHAPE(NT)
     __shared__ K sk[kRsTile];
     __shared__ u32 sv[kRsTile];
     __shared__ int wcnt[kScWaves][256];
@@ -617,10 +617,44 @@ __global__ __launch_bounds__(kTsThreads) void k_bucket_sort(K* __restrict__ kin,
     }
 }
 
-// Sorts and returns whether the result ended in (tk, tv) instead of
-// (keys, vals) (odd number of passes).
+// Launch helpers: one instantiation per threads-per-tile shape.
 template <class K>
-static bool radix_sort_core(Ctx& ctx, K* keys, u32* vals, K* tk, u32* tv, i64 n, int bits) {
+static void launch_hist(Ctx& ctx, int nt, int ntiles, const K* keys, i64 n, int shift, int bits, int* th) {
+    if (nt == 512)
+        k_radix_hist<K, 512><<<ntiles, 512, 0, ctx.stream>>>(keys, n, shift, bits, th, ntiles);
+    else
+        k_radix_hist<K, 1024><<<ntiles, 1024, 0, ctx.stream>>>(keys, n, shift, bits, th, ntiles);
+    RAMA_LAUNCHED();
+}
+
+template <class K>
+static void launch_scatter(Ctx& ctx, int nt, int ntiles, const K* ka, const u32* va, K* kb, u32* vb, i64 n,
+                           int shift, int bits, const int* th) {
+    if (nt == 512)
+        k_radix_scatter<K, 512><<<ntiles, 512, 0, ctx.stream>>>(ka, va, kb, vb, n, shift, bits, th, ntiles);
+    else
+        k_radix_scatter<K, 1024><<<ntiles, 1024, 0, ctx.stream>>>(ka, va, kb, vb, n, shift, bits, th, ntiles);
+    RAMA_LAUNCHED();
+}
+
+template <class K>
+static void launch_onesweep(Ctx& ctx, int nt, int ntiles, const K* ka, const u32* va, K* kb, u32* vb, i64 n,
+                            int shift, int bits, const int* hist_pass, u64* st, u32 epoch) {
+    if (nt == 512)
+        k_onesweep<K, 512><<<ntiles, 512, 0, ctx.stream>>>(ka, va, kb, vb, n, shift, bits, hist_pass, st,
+                                                           ctx.scan_counter.get(), epoch);
+    else
+        k_onesweep<K, 1024><<<ntiles, 1024, 0, ctx.stream>>>(ka, va, kb, vb, n, shift, bits, hist_pass, st,
+                                                             ctx.scan_counter.get(), epoch);
+    RAMA_LAUNCHED();
+}
+
+// Sorts and returns whether the result ended in (tk, tv) instead of
+// (keys, vals) (odd number of passes).  path / threads < 0: from the tuning
+// table for this array size.
+template <class K>
+static bool radix_sort_core(Ctx& ctx, K* keys, u32* vals, K* tk, u32* tv, i64 n, int bits, int path = -1,
+                            int threads = -1) {
     if (n <= 1 || bits <= 0) return false;
     RAMA_REQUIRE(n < ((i64)1 << 31), "radix sort supports fewer than 2^31 elements");
     RAMA_REQUIRE(bits <= (int)(8 * sizeof(K)), "radix sort key wider than its type");
@@ -644,9 +678,12 @@ static bool radix_sort_core(Ctx& ctx, K* keys, u32* vals, K* tk, u32* tv, i64 n,
         RAMA_LAUNCHED();
         return false;
     }
-    if (ntiles <= kBucketMaxTiles && bits > 8) {
-        // mid-size arrays: one pass on the top 8 bits into 256 buckets
-        // (histogram + onesweep), then each bucket's low bits in one block
+    if (path < 0) path = (int)tune::get(sizeof(K) == 8 ? tune::SortPathU64 : tune::SortPathU32, n);
+    if (threads < 0) threads = (int)tune::get(tune::SortThreads, ntiles);
+    if (path == 2 && (bits <= 8 || ntiles > tune::kBucketPathMaxTiles)) path = 1;
+    if (path == 2) {
+        // one pass on the top 8 bits into 256 buckets (histogram +
+        // onesweep), then each bucket's low bits in one block
         PassPlan top{}, low{};
         top.passes = 1;
         top.shift[0] = bits - 8;
@@ -671,9 +708,7 @@ static bool radix_sort_core(Ctx& ctx, K* keys, u32* vals, K* tk, u32* tv, i64 n,
         u64* st = ctx.scan_begin((i64)ntiles * 256, &epoch);
         {
             prof::Scope ps(ctx, "radix_onesweep", 2.0 * pair_bytes * n);
-            k_onesweep<K><<<ntiles, kScThreads, 0, ctx.stream>>>(keys, vals, tk, tv, n, top.shift[0], 8, hsum, st,
-                                                             ctx.scan_counter.get(), epoch);
-            RAMA_LAUNCHED();
+            launch_onesweep<K>(ctx, threads, ntiles, keys, vals, tk, tv, n, top.shift[0], 8, hsum, st, epoch);
         }
         {
             prof::Scope ps(ctx, "radix_bucket_sort", 2.0 * pair_bytes * n);
@@ -682,32 +717,25 @@ static bool radix_sort_core(Ctx& ctx, K* keys, u32* vals, K* tk, u32* tv, i64 n,
         }
         return false;
     }
-    static const int onesweep_max = [] {
-        const char* e = std::getenv("RAMA_ONESWEEP_MAX_TILES");
-        return e ? std::atoi(e) : kOnesweepMaxTiles;
-    }();
-    if (ntiles > onesweep_max) {
-        // large arrays: three launches per pass (look-back chains would be long)
+    if (path == 0) {
+        // three launches per pass: per-tile histograms, their scan, scatter
         DBuf<int> th = ctx.buf<int>((i64)256 * ntiles + 1);
         for (int p = 0; p < plan.passes; ++p) {
             {
                 prof::Scope ps(ctx, "radix_hist", (double)sizeof(K) * n);
-                k_radix_hist<K><<<ntiles, kScThreads, 0, ctx.stream>>>(ka, n, plan.shift[p], plan.width[p],
-                                                                       th.get(), ntiles);
-                RAMA_LAUNCHED();
+                launch_hist<K>(ctx, threads, ntiles, ka, n, plan.shift[p], plan.width[p], th.get());
             }
             scan_exclusive_arr<int>(ctx, th.get(), th.get(), ((i64)1 << plan.width[p]) * ntiles);
             {
                 prof::Scope ps(ctx, "radix_scatter", 2.0 * pair_bytes * n);
-                k_radix_scatter<K><<<ntiles, kScThreads, 0, ctx.stream>>>(ka, va, kb, vb, n, plan.shift[p],
-                                                                           plan.width[p], th.get(), ntiles);
-                RAMA_LAUNCHED();
+                launch_scatter<K>(ctx, threads, ntiles, ka, va, kb, vb, n, plan.shift[p], plan.width[p], th.get());
             }
             std::swap(ka, kb);
             std::swap(va, vb);
         }
         return ka != keys;
     }
+    // onesweep: every pass's digit totals in one read, one launch per pass
     const unsigned hb = (unsigned)std::min<i64>(ntiles, 1024);
     DBuf<int> hist = ctx.buf<int>((i64)(hb + 1) * plan.passes * 256);
     int* hsum = hist.get() + (i64)hb * plan.passes * 256;
@@ -722,9 +750,8 @@ static bool radix_sort_core(Ctx& ctx, K* keys, u32* vals, K* tk, u32* tv, i64 n,
         {
             // reads and writes one (key, u32 value) pair per element
             prof::Scope ps(ctx, "radix_onesweep", 2.0 * pair_bytes * n);
-            k_onesweep<K><<<ntiles, kScThreads, 0, ctx.stream>>>(ka, va, kb, vb, n, plan.shift[p], plan.width[p],
-                                                             hsum + p * 256, st, ctx.scan_counter.get(), epoch);
-            RAMA_LAUNCHED();
+            launch_onesweep<K>(ctx, threads, ntiles, ka, va, kb, vb, n, plan.shift[p], plan.width[p], hsum + p * 256,
+                               st, epoch);
         }
         std::swap(ka, kb);
         std::swap(va, vb);
@@ -735,6 +762,14 @@ static bool radix_sort_core(Ctx& ctx, K* keys, u32* vals, K* tk, u32* tv, i64 n,
 template <class K>
 void radix_sort_pairs(Ctx& ctx, K* keys, u32* vals, K* tk, u32* tv, i64 n, int bits) {
     if (radix_sort_core<K>(ctx, keys, vals, tk, tv, n, bits)) {
+        ctx.d2d(keys, tk, n);
+        ctx.d2d(vals, tv, n);
+    }
+}
+
+template <class K>
+void radix_sort_pairs_shaped(Ctx& ctx, K* keys, u32* vals, K* tk, u32* tv, i64 n, int bits, int path, int threads) {
+    if (radix_sort_core<K>(ctx, keys, vals, tk, tv, n, bits, path, threads)) {
         ctx.d2d(keys, tk, n);
         ctx.d2d(vals, tv, n);
     }
@@ -755,6 +790,8 @@ void radix_sort_pairs(Ctx& ctx, DBuf<K>& keys, DBuf<u32>& vals, i64 n, int bits)
 
 template void radix_sort_pairs<u32>(Ctx&, u32*, u32*, u32*, u32*, i64, int);
 template void radix_sort_pairs<u64>(Ctx&, u64*, u32*, u64*, u32*, i64, int);
+template void radix_sort_pairs_shaped<u32>(Ctx&, u32*, u32*, u32*, u32*, i64, int, int, int);
+template void radix_sort_pairs_shaped<u64>(Ctx&, u64*, u32*, u64*, u32*, i64, int, int, int);
 template void radix_sort_pairs<u32>(Ctx&, DBuf<u32>&, DBuf<u32>&, i64, int);
 template void radix_sort_pairs<u64>(Ctx&, DBuf<u64>&, DBuf<u32>&, i64, int);
 
