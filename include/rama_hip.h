@@ -242,6 +242,10 @@ void rama_profile_enable(int on);
 void rama_profile_reset(void);
 int64_t rama_profile_count(void);
 int rama_profile_entry(int64_t k, const char** name, int64_t* launches, double* total_ms, double* total_bytes);
+/* Work of entry k that has no byte roofline: the cycle-search counts report
+ * their expansions (units) and the neighbour-list entries a merge walk of
+ * each expansion's two lists reads (entries). */
+int rama_profile_work(int64_t k, double* units, double* entries);
 /* Restrict timing to launches of one kernel name (NULL or "": all). */
 void rama_profile_only(const char* name);
 

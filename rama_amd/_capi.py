@@ -90,6 +90,7 @@ SIGNATURES = {
     "rama_profile_reset": ([], None),
     "rama_profile_count": ([], I64),
     "rama_profile_entry": ([I64, ctypes.POINTER(ctypes.c_char_p), I64P, F64P, F64P], I32),
+    "rama_profile_work": ([I64, F64P, F64P], I32),
 }
 
 _lib = None

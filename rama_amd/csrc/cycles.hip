@@ -675,6 +675,68 @@ __global__ void k_tile_rep(const i64* __restrict__ xoff, i64 R, int* __restrict_
     for (i64 t = (b + kCycTile - 1) / kCycTile; t * kCycTile < e; ++t) tile_rep[t] = (int)r;
 }
 
+// Work of one count launch for the profiler (only while it is on): the
+// expansions the launch covers and the neighbour-list entries a merge walk
+// of each expansion's two lists reads, deg(a) + deg(b).  S = 3 and 4: the
+// launch covers rep edges [lo, hi); S = 5: threads [lo, hi), thread k of
+// rep edge r being (r, the (k - yoff[r])-th neighbour n2 of v2), whose
+// expansions are (n1, n2) for every n1 of v1 (n1 = n2 included: the count
+// skips it after loading both bounds).  nsum[v] = sum of the degrees of v's
+// neighbours.
+template <int S>
+static void count_work(Ctx& ctx, const Csr& G, const int* rv1, const int* rv2, i64 R, const i64* yoff, const i64* nsum,
+                       i64 lo, i64 hi, double& x, double& entries) {
+    const int* OFF = G.off.get();
+    const int* ADJ = G.adj.get();
+    x = entries = 0.0;
+    if (hi <= lo) return;
+    if (S == 3) {
+        x = (double)(hi - lo);
+        entries = (double)reduce<i64>(
+            ctx,
+            [=] __device__(i64 k) {
+                const int v = rv1[lo + k], u = rv2[lo + k];
+                return (i64)(OFF[v + 1] - OFF[v]) + (i64)(OFF[u + 1] - OFF[u]);
+            },
+            hi - lo, (i64)0, OpSum());
+    } else if (S == 4) {
+        x = (double)reduce<i64>(
+            ctx, [=] __device__(i64 k) { const int v = rv1[lo + k]; return (i64)(OFF[v + 1] - OFF[v]); }, hi - lo,
+            (i64)0, OpSum());
+        entries = (double)reduce<i64>(
+            ctx,
+            [=] __device__(i64 k) {
+                const int v = rv1[lo + k], u = rv2[lo + k];
+                return nsum[v] + (i64)(OFF[v + 1] - OFF[v]) * (i64)(OFF[u + 1] - OFF[u]);
+            },
+            hi - lo, (i64)0, OpSum());
+    } else {
+        x = (double)reduce<i64>(
+            ctx,
+            [=] __device__(i64 r) {
+                const i64 a = yoff[r] > lo ? yoff[r] : lo, b = yoff[r + 1] < hi ? yoff[r + 1] : hi;
+                const int v = rv1[r];
+                return a < b ? (i64)(OFF[v + 1] - OFF[v]) * (b - a) : (i64)0;
+            },
+            R, (i64)0, OpSum());
+        entries = (double)reduce<i64>(
+            ctx,
+            [=] __device__(i64 r) {
+                const i64 a = yoff[r] > lo ? yoff[r] : lo, b = yoff[r + 1] < hi ? yoff[r + 1] : hi;
+                if (a >= b) return (i64)0;
+                const int v = rv1[r], u = rv2[r];
+                const i64 d1 = OFF[v + 1] - OFF[v], base = (i64)OFF[u] - yoff[r];
+                i64 s = 0;
+                for (i64 k = a; k < b; ++k) {
+                    const int n2 = ADJ[base + k];
+                    s += nsum[v] + d1 * (i64)(OFF[n2 + 1] - OFF[n2]);
+                }
+                return s;
+            },
+            R, (i64)0, OpSum());
+    }
+}
+
 // Runs one stage over all expansions; returns the number of attempted writes.
 template <int S>
 static i64 run_stage(Ctx& ctx, const Csr& G, const int* rv1, const int* rv2, i64 R, i64 base, i64 budget,
@@ -708,8 +770,24 @@ static i64 run_stage(Ctx& ctx, const Csr& G, const int* rv1, const int* rv2, i64
     DBuf<unsigned char> cnt8 = ctx.buf<unsigned char>(X);
     DBuf<i64> tile = ctx.buf<i64>(ntiles + 1);
     RAMA_REQUIRE(ntiles < ((i64)1 << 31), "too many cycle expansions");
+    // profiling only: each count launch's expansions and list entries, and
+    // their merge-walk bytes (4 per list entry, 8 of list bounds per expansion)
+    const bool work_on = prof::wants(S == 3 ? "cycles3_count" : (S == 4 ? "cycles4_count" : "cycles5_count"));
+    DBuf<i64> nsum;
+    if (work_on && S > 3) {
+        nsum = ctx.buf<i64>(G.n);
+        i64* NS = nsum.get();
+        const int* ADJ = G.adj.get();
+        for_each(ctx, G.n, [=] __device__(i64 v) {
+            i64 s = 0;
+            for (int k = OFF[v]; k < OFF[v + 1]; ++k) s += OFF[ADJ[k] + 1] - OFF[ADJ[k]];
+            NS[v] = s;
+        });
+    }
+    double wx = 0.0, wy = 0.0;
     if (S == 3) {
-        prof::Scope ps(ctx, "cycles3_count", 0.0);
+        if (work_on) count_work<3>(ctx, G, rv1, rv2, R, nullptr, nullptr, 0, R, wx, wy);
+        prof::Scope ps(ctx, "cycles3_count", 4.0 * wy + 8.0 * wx, wx, wy);
         k_cycle_count3<<<(unsigned)ntiles, kBlock, 0, ctx.stream>>>(a, cnt8.get(), tile.get());
         RAMA_LAUNCHED();
     }
@@ -755,12 +833,16 @@ static i64 run_stage(Ctx& ctx, const Csr& G, const int* rv1, const int* rv2, i64
         while (done < nblk) {
             const unsigned n = std::min(chunk, nblk - done);
             a.blk0 = done;
+            if (work_on) {
+                const i64 lo = (i64)done * kBlock, hi = std::min<i64>((i64)(done + n) * kBlock, S == 4 ? R : Y);
+                count_work<S>(ctx, G, rv1, rv2, R, S == 5 ? yoff.get() : nullptr, nsum.get(), lo, hi, wx, wy);
+            }
             if (S == 4) {
-                prof::Scope ps(ctx, "cycles4_count", 0.0);
+                prof::Scope ps(ctx, "cycles4_count", 4.0 * wy + 8.0 * wx, wx, wy);
                 k_cycle_count4<<<n, kBlock, 0, ctx.stream>>>(a, cnt8.get(), tile.get());
                 RAMA_LAUNCHED();
             } else {
-                prof::Scope ps(ctx, "cycles5_count", 0.0);
+                prof::Scope ps(ctx, "cycles5_count", 4.0 * wy + 8.0 * wx, wx, wy);
                 k_cycle_count5<<<n, kBlock, 0, ctx.stream>>>(a, yoff.get(), ytile_rep.get(), Y, cnt8.get(),
                                                              tile.get());
                 RAMA_LAUNCHED();

@@ -1,6 +1,7 @@
 // profiler.hip — event bookkeeping behind profiler.hpp and its C ABI:
 //   rama_profile_enable(on), rama_profile_reset(),
-//   rama_profile_count(), rama_profile_entry(k, name, launches, ms, bytes).
+//   rama_profile_count(), rama_profile_entry(k, name, launches, ms, bytes),
+//   rama_profile_work(k, units, entries).
 #include <atomic>
 #include <cstdio>
 #include <map>
@@ -45,7 +46,7 @@ const bool g_sites = std::getenv("RAMA_PROFILE_SITES") != nullptr;
 struct Pending {
     std::string name;
     hipEvent_t a, b;
-    double bytes;
+    double bytes, units, entries;
 };
 std::vector<Pending> g_pending;
 std::vector<hipEvent_t> g_free;
@@ -54,6 +55,7 @@ struct Agg {
     int64_t launches = 0;
     double ms = 0.0;
     double bytes = 0.0;
+    double units = 0.0, entries = 0.0;
 };
 std::vector<Agg> g_agg;
 
@@ -71,6 +73,8 @@ void resolve_locked() {
             a->launches += 1;
             a->ms += ms;
             a->bytes += p.bytes;
+            a->units += p.units;
+            a->entries += p.entries;
         }
         (void)hipGetLastError();
         g_free.push_back(p.a);
@@ -108,11 +112,11 @@ hipEvent_t take_event() {
     return e;
 }
 
-void push(const char* name, hipEvent_t a, hipEvent_t b, double bytes) {
+void push(const char* name, hipEvent_t a, hipEvent_t b, double bytes, double units, double entries) {
     std::lock_guard<std::mutex> l(g_mu);
     std::string key(name);
     if (g_sites && tl_site) key = key + "@" + tl_site;
-    g_pending.push_back(Pending{std::move(key), a, b, bytes});
+    g_pending.push_back(Pending{std::move(key), a, b, bytes, units, entries});
     if (g_pending.size() > 4096) resolve_locked();
 }
 
@@ -155,6 +159,15 @@ int rama_profile_entry(int64_t k, const char** name, int64_t* launches, double* 
     *launches = g_agg[k].launches;
     *ms = g_agg[k].ms;
     *bytes = g_agg[k].bytes;
+    return 0;
+}
+
+int rama_profile_work(int64_t k, double* units, double* entries) {
+    std::lock_guard<std::mutex> l(g_mu);
+    resolve_locked();
+    if (k < 0 || k >= (int64_t)g_agg.size() || !units || !entries) return RAMA_ERR_INVALID;
+    *units = g_agg[k].units;
+    *entries = g_agg[k].entries;
     return 0;
 }
 

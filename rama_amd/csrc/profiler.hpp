@@ -11,7 +11,9 @@ namespace prof {
 
 bool enabled();
 bool wants(const char* name);  // enabled and not filtered out
-void push(const char* name, hipEvent_t a, hipEvent_t b, double bytes);
+// `units`: work items of a kernel without a byte roofline (cycle-search
+// expansions); `entries`: neighbour-list entries its merge walks stream.
+void push(const char* name, hipEvent_t a, hipEvent_t b, double bytes, double units = 0.0, double entries = 0.0);
 hipEvent_t take_event();
 
 // Call-site tag for shared primitives (sorts): with RAMA_PROFILE_SITES set,
@@ -31,7 +33,8 @@ class Site {
 // Brackets the launches in its scope with events on ctx.stream.
 class Scope {
    public:
-    Scope(Ctx& ctx, const char* name, double bytes) : ctx_(ctx), name_(name), bytes_(bytes) {
+    Scope(Ctx& ctx, const char* name, double bytes, double units = 0.0, double entries = 0.0)
+        : ctx_(ctx), name_(name), bytes_(bytes), units_(units), entries_(entries) {
         if (wants(name)) {
             a_ = take_event();
             b_ = take_event();
@@ -41,7 +44,7 @@ class Scope {
     ~Scope() {
         if (a_) {
             (void)hipEventRecord(b_, ctx_.stream);
-            push(name_, a_, b_, bytes_);
+            push(name_, a_, b_, bytes_, units_, entries_);
         }
     }
     Scope(const Scope&) = delete;
@@ -50,7 +53,7 @@ class Scope {
    private:
     Ctx& ctx_;
     const char* name_;
-    double bytes_;
+    double bytes_, units_, entries_;
     hipEvent_t a_ = nullptr, b_ = nullptr;
 };
 

@@ -30,26 +30,41 @@ def read():
         ms = ctypes.c_double()
         by = ctypes.c_double()
         if L.rama_profile_entry(k, ctypes.byref(name), ctypes.byref(n), ctypes.byref(ms), ctypes.byref(by)) == 0:
-            out.append({"name": name.value.decode(), "launches": n.value, "ms": ms.value, "bytes": by.value})
+            d = {"name": name.value.decode(), "launches": n.value, "ms": ms.value, "bytes": by.value}
+            units, entries = ctypes.c_double(), ctypes.c_double()
+            if L.rama_profile_work(k, ctypes.byref(units), ctypes.byref(entries)) == 0 and units.value > 0:
+                d["units"], d["entries"] = units.value, entries.value
+            out.append(d)
     out.sort(key=lambda d: -d["ms"])
     return out
 
 
 def top(stats, k=8):
-    return [{"name": d["name"], "launches": d["launches"], "ms": round(d["ms"], 3),
+    """The k slowest kernels: achieved GB/s of their algorithmic bytes; the
+    cycle-search counts add expansions/s and neighbour-list entries/s (their
+    GB/s is the merge-walk model: 4 bytes per list entry and 8 bytes of list
+    bounds per expansion, served mostly from cache)."""
+    out = []
+    for d in stats[:k]:
+        e = {"name": d["name"], "launches": d["launches"], "ms": round(d["ms"], 3),
              "GBps": round(d["bytes"] / (d["ms"] * 1e6), 1) if d["bytes"] > 0 and d["ms"] > 0 else None}
-            for d in stats[:k]]
+        if d.get("units") and d["ms"] > 0:
+            e["expansions_per_s"] = d["units"] / (d["ms"] * 1e-3)
+            e["entries_per_s"] = d["entries"] / (d["ms"] * 1e-3)
+        out.append(e)
+    return out
 
 
 def dominant(stats):
     """Name of the kernel with a byte model that takes the most time."""
-    cands = [d for d in stats if d["bytes"] > 0 and d["ms"] > 0]
+    cands = [d for d in stats if d["bytes"] > 0 and d["ms"] > 0 and not d.get("units")]
     return max(cands, key=lambda x: x["ms"])["name"] if cands else None
 
 
 def roofline(stats, peak_gbs, name=None):
     """Roofline entry of the dominant kernel that has a byte model (or `name`)."""
-    cands = [d for d in stats if d["bytes"] > 0 and d["ms"] > 0 and (name is None or d["name"] == name)]
+    cands = [d for d in stats if d["bytes"] > 0 and d["ms"] > 0 and not d.get("units")
+             and (name is None or d["name"] == name)]
     if not cands:
         return None
     d = max(cands, key=lambda x: x["ms"])
