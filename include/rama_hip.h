@@ -95,6 +95,32 @@ int rama_solve_device(const int32_t* d_i, const int32_t* d_j, const float* d_cos
                       int32_t* d_labels_out, int64_t labels_cap, int64_t* num_labels_out,
                       double* lb_out, int32_t* time_ms_out, rama_timeline** timeline_out);
 
+/*
+ * Allocation-free form of rama_solve_device.  No counterpart in the
+ * reference: it is for callers that cannot allocate device memory (code
+ * under HIP graph capture, tight serving loops, harnesses that forbid
+ * hipMalloc).  Same arguments and results as rama_solve_device, without the
+ * timeline; stream NULL means the null stream itself (no stream is created).
+ * Every device buffer the solve needs is carved out of one static device
+ * array of RAMA_NOALLOC_SCRATCH_BYTES per GPU, which the library holds from
+ * the first such call on a GPU until the process ends; the call makes no
+ * hipMalloc, hipFree, hipHostMalloc, stream or event creation.
+ * num_edges above RAMA_NOALLOC_MAX_EDGES fails with RAMA_ERR_INVALID before
+ * any work; a graph whose solve needs more than the array holds fails with
+ * RAMA_ERR_OOM before the kernel that would use the block is launched.
+ * scratch_peak_out (may be NULL) receives the bytes of the array the solve
+ * used at most.  All calls on one GPU share its array: calls that overlap
+ * on one GPU (from several threads or streams) race and must be serialised
+ * by the caller.  The kernel timing switch (rama_profile_enable) creates
+ * events when on; leave it off here.
+ */
+#define RAMA_NOALLOC_SCRATCH_BYTES ((int64_t)256 << 20)
+#define RAMA_NOALLOC_MAX_EDGES ((int64_t)1000000) /* a PD+ grid solve peaks near 240 bytes per edge */
+int rama_solve_device_noalloc(const int32_t* d_i, const int32_t* d_j, const float* d_costs, int64_t num_edges,
+                              const rama_options_c* opts, int device, void* stream,
+                              int32_t* d_labels_out, int64_t labels_cap, int64_t* num_labels_out,
+                              double* lb_out, int32_t* time_ms_out, int64_t* scratch_peak_out);
+
 int64_t rama_timeline_count(const rama_timeline* t);
 int64_t rama_timeline_length(const rama_timeline* t, int64_t k);
 const int32_t* rama_timeline_labels(const rama_timeline* t, int64_t k);

@@ -54,6 +54,7 @@ SIGNATURES = {
     "rama_options_preset": ([ctypes.c_char_p, OPTP], I32),
     "rama_solve_host": ([I32P, I32P, F32P, I64, OPTP, I32P, I64, I64P, F64P, I32P, ctypes.POINTER(TLP)], I32),
     "rama_solve_device": ([VP, VP, VP, I64, OPTP, I32, VP, VP, I64, I64P, F64P, I32P, ctypes.POINTER(TLP)], I32),
+    "rama_solve_device_noalloc": ([VP, VP, VP, I64, OPTP, I32, VP, VP, I64, I64P, F64P, I32P, I64P], I32),
     "rama_timeline_count": ([TLP], I64),
     "rama_timeline_length": ([TLP, I64], I64),
     "rama_timeline_labels": ([TLP, I64], I32P),
@@ -191,6 +192,23 @@ def solve_device(i_ptr, j_ptr, c_ptr, num_edges, labels_ptr, labels_cap, opts=No
                                    VP(stream or 0), VP(labels_ptr), labels_cap, ctypes.byref(n), ctypes.byref(lb),
                                    ctypes.byref(ms), ctypes.byref(tl) if want_timeline else None))
     return n.value, lb.value, ms.value, _timeline(tl)
+
+
+def solve_device_noalloc(i_ptr, j_ptr, c_ptr, num_edges, labels_ptr, labels_cap, opts=None, device=0, stream=None):
+    """Allocation-free device-pointer solve (rama_solve_device_noalloc).
+    Returns (num_labels, lb, time_ms, scratch_peak_bytes)."""
+    opts = opts or options()
+    n = ctypes.c_int64()
+    lb = ctypes.c_double()
+    ms = ctypes.c_int32()
+    peak = ctypes.c_int64()
+    _check(lib().rama_solve_device_noalloc(VP(i_ptr), VP(j_ptr), VP(c_ptr), num_edges, ctypes.byref(opts), device,
+                                           VP(stream or 0), VP(labels_ptr), labels_cap, ctypes.byref(n),
+                                           ctypes.byref(lb), ctypes.byref(ms), ctypes.byref(peak)))
+    return n.value, lb.value, ms.value, peak.value
+
+
+NOALLOC_MAX_EDGES = 1000000  # RAMA_NOALLOC_MAX_EDGES (include/rama_hip.h)
 
 
 def objective(labels, i, j, c):

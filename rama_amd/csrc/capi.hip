@@ -58,6 +58,35 @@ void download(Ctx& ctx, T* h, const T* d, i64 n) {
 
 }  // namespace
 
+// The device memory of rama_solve_device_noalloc: one copy per GPU, held
+// from the code object's load until the process ends.  In host code the
+// name stands for a host-side shadow, not a device address; only the address
+// from hipGetSymbolAddress (noalloc_scratch) is handed to kernels and copies.
+static __device__ __align__(256) unsigned char scratch[RAMA_NOALLOC_SCRATCH_BYTES];
+
+namespace {
+
+std::mutex g_scratch_mu;
+std::map<int, void*> g_scratch_addr;  // device -> its copy of `scratch`
+
+// Device address of `scratch` on `device`, which must be current: looked up
+// on the first call for the device, then reused.
+void* noalloc_scratch(int device) {
+    std::lock_guard<std::mutex> l(g_scratch_mu);
+    auto it = g_scratch_addr.find(device);
+    if (it != g_scratch_addr.end()) return it->second;
+    void* p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || p == nullptr) {
+        (void)hipGetLastError();
+        throw Error(RAMA_ERR_HIP, std::string("hipGetSymbolAddress(scratch) failed: ") + hipGetErrorString(e));
+    }
+    g_scratch_addr.emplace(device, p);
+    return p;
+}
+
+}  // namespace
+
 namespace rama {
 
 void set_last_error(const std::string& s) { g_last_error = s; }
@@ -181,6 +210,35 @@ int rama_solve_device(const int32_t* d_i, const int32_t* d_j, const float* d_c, 
         if (n > 0) ctx.d2d(d_labels_out, labels.get(), n);
         ctx.sync();
         if (tl) *tl = own.release();
+    });
+}
+
+int rama_solve_device_noalloc(const int32_t* d_i, const int32_t* d_j, const float* d_c, int64_t m,
+                              const rama_options_c* opts, int device, void* stream, int32_t* d_labels_out,
+                              int64_t cap, int64_t* n_labels_out, double* lb_out, int32_t* time_ms_out,
+                              int64_t* scratch_peak_out) {
+    return guarded([&] {
+        RAMA_REQUIRE(d_i && d_j && d_c && n_labels_out && lb_out, "NULL argument");
+        RAMA_REQUIRE(m <= RAMA_NOALLOC_MAX_EDGES, "rama_solve_device_noalloc: " + std::to_string(m) +
+                                                      " edges exceed the limit of " +
+                                                      std::to_string((long long)RAMA_NOALLOC_MAX_EDGES));
+        RAMA_HIP(hipSetDevice(device));
+        void* region = noalloc_scratch(device);
+        Ctx ctx(device, static_cast<hipStream_t>(stream), opts && opts->verbose, region,
+                (size_t)RAMA_NOALLOC_SCRATCH_BYTES);
+        DBuf<int> di = ctx.buf<int>(m), dj = ctx.buf<int>(m);
+        DBuf<float> dc = ctx.buf<float>(m);
+        ctx.d2d(di.get(), d_i, m);
+        ctx.d2d(dj.get(), d_j, m);
+        ctx.d2d(dc.get(), d_c, m);
+        DBuf<int> labels;
+        i64 n = 0;
+        solve_common(ctx, di, dj, dc, m, opts, labels, n, lb_out, time_ms_out, nullptr);
+        *n_labels_out = n;
+        if (n > cap) throw Error(RAMA_ERR_INVALID, "label buffer too small: need " + std::to_string(n));
+        if (n > 0) ctx.d2d(d_labels_out, labels.get(), n);
+        ctx.sync();
+        if (scratch_peak_out) *scratch_peak_out = (int64_t)ctx.arena.peak();
     });
 }
 

@@ -94,8 +94,22 @@ class Arena {
     Arena(const Arena&) = delete;
     Arena& operator=(const Arena&) = delete;
 
+    // Fixed-region mode: every block is carved out of [base, base + bytes)
+    // (first fit over the free extents in address order, 256-byte aligned,
+    // neighbours merged on release); the driver and the process-wide cache
+    // are never asked.  Exhausting the region is an RAMA_ERR_OOM error
+    // raised on the host before any kernel sees the block.
+    void use_region(void* base, size_t bytes) {
+        rbase_ = static_cast<char*>(base);
+        rbytes_ = bytes & ~(size_t)255;
+        rfree_.clear();
+        if (rbytes_) rfree_.emplace(0, rbytes_);
+    }
+    bool fixed() const { return rbase_ != nullptr; }
+
     void* alloc(size_t bytes) {
         if (bytes == 0) bytes = 1;
+        if (rbase_) return region_alloc(bytes);
         const size_t cls = size_class(bytes);
         auto it = free_.find(cls);
         void* p = nullptr;
@@ -128,6 +142,12 @@ class Arena {
         if (!p) return;
         auto it = used_.find(p);
         if (it == used_.end()) return;
+        if (rbase_) {
+            region_free(static_cast<char*>(p) - rbase_, it->second);
+            live_ -= it->second;
+            used_.erase(it);
+            return;
+        }
         free_.emplace(it->second, p);
         live_ -= it->second;
         used_.erase(it);
@@ -166,10 +186,46 @@ class Arena {
     }
 
    private:
+    void* region_alloc(size_t bytes) {
+        const size_t need = (bytes + 255) & ~(size_t)255;
+        for (auto it = rfree_.begin(); it != rfree_.end(); ++it) {
+            if (it->second < need) continue;
+            const size_t off = it->first, len = it->second;
+            rfree_.erase(it);
+            if (len > need) rfree_.emplace(off + need, len - need);
+            void* p = rbase_ + off;
+            used_[p] = need;
+            live_ += need;
+            if (live_ > peak_) peak_ = live_;
+            return p;
+        }
+        throw Error(RAMA_ERR_OOM, "fixed device region of " + std::to_string(rbytes_) + " bytes exhausted: " +
+                                      std::to_string(live_) + " bytes in use, " + std::to_string(need) +
+                                      " more requested");
+    }
+    void region_free(size_t off, size_t len) {
+        auto next = rfree_.lower_bound(off);
+        if (next != rfree_.end() && off + len == next->first) {
+            len += next->second;
+            next = rfree_.erase(next);
+        }
+        if (next != rfree_.begin()) {
+            auto prev = std::prev(next);
+            if (prev->first + prev->second == off) {
+                prev->second += len;
+                return;
+            }
+        }
+        rfree_.emplace(off, len);
+    }
+
     int device_;
     std::multimap<size_t, void*> free_;
     std::unordered_map<void*, size_t> used_;
     size_t live_ = 0, peak_ = 0, reserved_ = 0;
+    char* rbase_ = nullptr;               // fixed region (nullptr: caching mode)
+    size_t rbytes_ = 0;
+    std::map<size_t, size_t> rfree_;      // free extents of the region: offset -> length
 };
 
 // Typed device buffer owned by an Arena (move-only).
@@ -231,14 +287,27 @@ struct Ctx {
         }
         pinned = pool::take_pinned();
     }
+    // Allocation-free context: device memory comes only from the fixed
+    // region [region, region + region_bytes), work runs on the caller's
+    // stream as given (the null stream included), scalar read-backs land in
+    // host memory inside the context and waits are stream synchronisations,
+    // so nothing is allocated, created or pinned.
+    Ctx(int dev, hipStream_t s, bool verb, void* region, size_t region_bytes)
+        : device(dev), stream(s), arena(dev), verbose(verb), no_alloc(true) {
+        RAMA_HIP(hipSetDevice(dev));
+        arena.use_region(region, region_bytes);
+        pinned = host_scalars;
+    }
     ~Ctx() {
-        if (stream) (void)hipStreamSynchronize(stream);
+        if (stream || no_alloc) (void)hipStreamSynchronize(stream);
         if (sync_ev) (void)hipEventDestroy(sync_ev);
         arena.trim();
-        pool::give_pinned(pinned);
+        if (!no_alloc) pool::give_pinned(pinned);
         if (up_pinned) (void)hipHostFree(up_pinned);
         if (own_stream) (void)hipStreamDestroy(stream);
     }
+    const bool no_alloc = false;
+    alignas(64) unsigned char host_scalars[pool::kPinnedBytes];  // read-back area of an allocation-free context
     Ctx(const Ctx&) = delete;
     Ctx& operator=(const Ctx&) = delete;
 
@@ -304,7 +373,7 @@ struct Ctx {
             const char* e = std::getenv("RAMA_SYNC_MODE");
             return e && std::strcmp(e, "stream") == 0;
         }();
-        if (by_stream) {
+        if (by_stream || no_alloc) {
             RAMA_HIP(hipStreamSynchronize(stream));
             return;
         }
@@ -324,6 +393,11 @@ struct Ctx {
         const size_t bytes = sizeof(T) * (size_t)n;
         if (bytes == 0) return;
         if (sync_sites_on()) note_sync_site(fn, -line);  // uploads are counted at negative line numbers
+        if (no_alloc) {  // no pinned staging: copy from the caller's memory and wait
+            RAMA_HIP(hipMemcpyAsync(d, h, bytes, hipMemcpyHostToDevice, stream));
+            sync(fn, line);
+            return;
+        }
         const size_t need = (bytes + 255) & ~(size_t)255;
         if (up_sync != syncs) {
             up_off = 0;
