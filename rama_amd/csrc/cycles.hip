@@ -15,6 +15,8 @@
 // every tile its write offset; an emit pass revisits only expansions with
 // mids and writes at enumeration position (rep edge, expansion, mid).  The
 // triangle budget therefore keeps exactly the first `budget` writes.
+#include <cstdlib>
+
 #include "rama_internal.hpp"
 #include "tuning.hpp"
 
@@ -47,6 +49,11 @@ struct CycArgs {
     // end of the expansions they covered ([1], a maximum).
     unsigned blk0 = 0;
     i64* counted = nullptr;
+    // diagnostic (RAMA_CYCLE_STATS): threads of the 5-cycle count, and those
+    // that counted through a 2-hop table
+    unsigned long long* stats = nullptr;
+    // 5-cycle count: node ids fit the 2-hop tables' keys (below kT5MaxNode)
+    int tables = 0;
 };
 
 // Adds the block's attempted writes to its counter line's [0] and raises
@@ -445,38 +452,38 @@ __global__ __launch_bounds__(kBlock) void k_cycle_count4(CycArgs a, unsigned cha
 
 // 5-cycles: one thread per (repulsive edge r, n2 = l2-th neighbour of v2);
 // its expansions (n1 = l1-th neighbour of v1, n2) are xoff[r] + l2*d1 + l1
-// for l1 = 0 .. d1-1, each intersecting N(n1) with N(n2) (held in
-// registers); n1 == n2 is skipped.  Threads of rep r are yoff[r] ..
-// yoff[r+1]-1; ytile_rep gives each block's first rep.  tile_tot must be
-// zero on entry.
-__global__ __launch_bounds__(kBlock) void k_cycle_count5(CycArgs a, const i64* __restrict__ yoff,
-                                                         const int* __restrict__ ytile_rep, i64 Y,
-                                                         unsigned char* __restrict__ cnt8,
-                                                         i64* __restrict__ tile_tot) {
-    const unsigned blk = a.blk0 + xcd_block();
-    const i64 y = (i64)blk * kCycTile + threadIdx.x;
-    const bool on = y < Y;
-    i64 mine = 0, xend = 0;
-    int slot = 0;
-    if (on) {
-    // rep owning thread y (galloping from the block's first rep)
-    i64 lo = ytile_rep[blk], hi = a.R - 1, step = 1;
-    while (lo + step <= hi && yoff[lo + step] <= y) {
-        lo += step;
-        step <<= 1;
-    }
-    hi = lo + step < a.R - 1 ? lo + step : a.R - 1;
-    while (lo < hi) {
-        const i64 mid = (lo + hi + 1) >> 1;
-        if (yoff[mid] <= y) lo = mid; else hi = mid - 1;
-    }
-    const i64 r = lo;
-    const int v1 = a.rv1[r], v2 = a.rv2[r];
-    const int l2 = (int)(y - yoff[r]);
-    const int b1 = a.off[v1], d1 = a.off[v1 + 1] - b1;
-    const int n2 = a.adj[a.off[v2] + l2];
-    const int bq = a.off[n2], nq = a.off[n2 + 1] - bq;
-    const i64 x0 = a.xoff[r] + (i64)l2 * d1;
+// for l1 = 0 .. d1-1, each intersecting N(n1) with N(n2); n1 == n2 is
+// skipped.  Threads of rep r are yoff[r] .. yoff[r+1]-1; ytile_rep gives
+// each block's first rep.  tile_tot must be zero on entry.
+//
+// Two ways to count one thread's d1 expansions, with identical counts:
+//  * 2-hop table (the common case): the block builds, for each distinct v1
+//    among its threads, an LDS hash table of v1's 2-hop neighbourhood,
+//    mid -> bitmask of the l1 whose list N(n1) holds mid.  The thread then
+//    walks N(n2) once, looks every mid up and adds its mask to bit-sliced
+//    per-l1 counters, instead of streaming every N(n1) against N(n2).  Valid
+//    when no list involved holds a value twice (then |N(n1) ∩ N(n2)| is the
+//    merge walk's min(a, b) count), d1 <= 32 and the 2-hop lists fit.
+//  * per expansion (otherwise): N(n2) in registers (or the merge walk /
+//    probes for long lists), each N(n1) streamed against it.
+constexpr int kT5Tables = 16;      // 2-hop tables (distinct v1 runs) per block
+constexpr int kT5SlotBits = 11;
+constexpr int kT5Slots = 1 << kT5SlotBits;  // slots of the block's shared hash table
+constexpr int kT5MaxFill = 1536;   // entries of all tables together at most this (load <= 3/4)
+constexpr int kT5MaxD1 = 32;       // d(v1) at most this (32-bit masks)
+constexpr int kT5RepWin = 256;     // rep edges a block locates its threads among in LDS
+constexpr int kT5MaxNode = 1 << 27;  // node ids below this: (mid, run) fits one key
+
+// The tables of a block share one open-addressing hash table of keys
+// mid * kT5Tables + run.
+__device__ __forceinline__ int t5_hash(int key) {
+    return (int)(((unsigned)key * 2654435761u) >> (32 - kT5SlotBits));
+}
+
+// The per-expansion form for one thread (see above).
+__device__ __forceinline__ void count5_direct(const CycArgs& a, int b1, int d1, int n2, int bq, int nq, i64 x0,
+                                              unsigned char* __restrict__ cnt8, i64* __restrict__ tile_tot,
+                                              i64& mine) {
     int rb[kRegList];
     const bool regs = nq <= kRegList;
     int rmax = -1, kw = kRegList;
@@ -505,8 +512,244 @@ __global__ __launch_bounds__(kBlock) void k_cycle_count5(CycArgs a, const i64* _
         ie = ne;
     }
     acc.flush(tile_tot);
-    xend = x0 + d1;
-    if (a.nslot) slot = a.nslot[v1];
+}
+
+__global__ __launch_bounds__(kBlock) void k_cycle_count5(CycArgs a, const i64* __restrict__ yoff,
+                                                         const int* __restrict__ ytile_rep, i64 Y,
+                                                         unsigned char* __restrict__ cnt8,
+                                                         i64* __restrict__ tile_tot) {
+    __shared__ int tkey[kT5Slots];
+    __shared__ unsigned tmask[kT5Slots];
+    __shared__ int tv1[kT5Tables];
+    __shared__ int tok[kT5Tables];
+    __shared__ int tfill[kT5Tables];
+    __shared__ int sv1[kBlock];
+    __shared__ int wruns[kWaves];
+    const unsigned blk = a.blk0 + xcd_block();
+    const int tid = threadIdx.x, lane = lane_id(), w = tid >> 6;
+    const i64 y = (i64)blk * kCycTile + tid;
+    const bool on = y < Y;
+    i64 mine = 0, xend = 0;
+    int slot = 0;
+    int v1 = -1, b1 = 0, d1 = 0, n2 = 0, bq = 0, nq = 0;
+    i64 x0 = 0;
+    // rep owning each thread: the block's threads belong to reps r0, r0 + 1,
+    // ...; their thread offsets are loaded once (coalesced) and every thread
+    // marks where its rep starts, so a block max-scan (kept as a running max
+    // over the marks) hands each thread its rep.  Blocks whose threads span
+    // more than kT5RepWin reps fall back to a galloping search.
+    const i64 y0 = (i64)blk * kCycTile;
+    const i64 r0 = y0 < Y ? (i64)ytile_rep[blk] : 0;
+    sv1[tid] = -1;
+    __syncthreads();
+    bool wide = false;
+    for (int k = tid; k < kT5RepWin; k += kBlock) {
+        const i64 r = r0 + k;
+        if (r < a.R) {
+            const i64 ys = yoff[r];
+            if (ys >= y0 && ys < y0 + kCycTile && yoff[r + 1] > ys) sv1[ys - y0] = k;
+        }
+    }
+    if (y0 < Y) {
+        const i64 rl = r0 + kT5RepWin - 1;
+        wide = rl < a.R && yoff[rl + 1] < (y0 + kCycTile < Y ? y0 + kCycTile : Y);
+    }
+    __syncthreads();
+    int mark = sv1[tid];
+    // inclusive max-scan of the marks over the block (first thread: its rep
+    // r0 starts before the block, mark 0)
+    if (tid == 0 && mark < 0) mark = 0;
+#pragma unroll
+    for (int d = 1; d < 64; d <<= 1) {
+        const int t = __shfl_up(mark, d, 64);
+        if (lane >= d) mark = max(mark, t);
+    }
+    __shared__ int wmax[kWaves];
+    if (lane == 63) wmax[w] = mark;
+    __syncthreads();
+    for (int q = 0; q < w; ++q) mark = max(mark, wmax[q]);
+    __syncthreads();
+    if (on) {
+        i64 r = r0 + mark;
+        if (wide) {
+            // rep owning thread y (galloping from the block's first rep)
+            i64 lo = r0, hi = a.R - 1, step = 1;
+            while (lo + step <= hi && yoff[lo + step] <= y) {
+                lo += step;
+                step <<= 1;
+            }
+            hi = lo + step < a.R - 1 ? lo + step : a.R - 1;
+            while (lo < hi) {
+                const i64 mid = (lo + hi + 1) >> 1;
+                if (yoff[mid] <= y) lo = mid; else hi = mid - 1;
+            }
+            r = lo;
+        }
+        v1 = a.rv1[r];
+        const int v2 = a.rv2[r];
+        const int l2 = (int)(y - yoff[r]);
+        b1 = a.off[v1];
+        d1 = a.off[v1 + 1] - b1;
+        n2 = a.adj[a.off[v2] + l2];
+        bq = a.off[n2];
+        nq = a.off[n2 + 1] - bq;
+        x0 = a.xoff[r] + (i64)l2 * d1;
+        if (a.nslot) slot = a.nslot[v1];
+    }
+    // runs of equal v1 over the block's threads (rep edges are in row order,
+    // so a run is usually all threads of one v1); run k < kT5Tables gets
+    // table k
+    sv1[tid] = v1;
+    for (int k = tid; k < kT5Slots; k += kBlock) {
+        tkey[k] = -1;
+        tmask[k] = 0u;
+    }
+    __syncthreads();
+    const bool start = on && (tid == 0 || sv1[tid - 1] != v1);
+    const unsigned long long sm = __ballot(start);
+    if (lane == 0) wruns[w] = __popcll(sm);
+    __syncthreads();
+    int before = 0, runs = 0;
+    for (int q = 0; q < kWaves; ++q) {
+        before += q < w ? wruns[q] : 0;
+        runs += wruns[q];
+    }
+    // this thread's run index (its run starts at or before it)
+    const int run = before + __popcll(sm & ((2ull << lane) - 1ull)) - 1;
+    if (start && run < kT5Tables) {
+        tv1[run] = v1;
+        tok[run] = 1;
+    }
+    const int ntab = runs < kT5Tables ? runs : kT5Tables;
+    __syncthreads();
+    // build, in three steps: each wave sizes its tables (w, w + kWaves, ...:
+    // lane l holds the bounds of the list of v1's l-th neighbour n1); the
+    // tables that fit the shared capacity in run order are kept; each wave
+    // inserts its kept tables' (mid, l) pairs, flattened over its 64 lanes
+    constexpr int kPerWave = kT5Tables / kWaves;
+    int s0[kPerWave], s1[kPerWave];
+#pragma unroll
+    for (int i = 0; i < kPerWave; ++i) {
+        const int k = w + i * kWaves;
+        s0[i] = s1[i] = 0;
+        if (k >= ntab) continue;
+        const int v = tv1[k];
+        const int bv = a.off[v], dv = a.off[v + 1] - bv;
+        if (dv <= kT5MaxD1 && lane < dv) {
+            const int n1 = a.adj[bv + lane];
+            s0[i] = a.off[n1];
+            s1[i] = a.off[n1 + 1];
+        }
+        int fill = s1[i] - s0[i];
+#pragma unroll
+        for (int d = 32; d >= 1; d >>= 1) fill += __shfl_xor(fill, d, 64);
+        if (lane == 0) tfill[k] = dv <= kT5MaxD1 && a.tables ? fill : kT5MaxFill + 1;
+    }
+    __syncthreads();
+    if (tid == 0) {
+        int used = 0;
+        for (int k = 0; k < ntab; ++k) {
+            const bool fits = used + tfill[k] <= kT5MaxFill;
+            tok[k] = fits ? 1 : 0;
+            used += fits ? tfill[k] : 0;
+        }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int i = 0; i < kPerWave; ++i) {
+        const int k = w + i * kWaves;
+        if (k >= ntab || !tok[k]) continue;  // wave-uniform
+        bool dup = false;
+        const int fill = tfill[k];
+        const int len = s1[i] - s0[i];
+        int excl = len;  // where list l starts among the table's pairs
+#pragma unroll
+        for (int d = 1; d < 64; d <<= 1) {
+            const int t = __shfl_up(excl, d, 64);
+            if (lane >= d) excl += t;
+        }
+        excl -= len;
+        const int dv = a.off[tv1[k] + 1] - a.off[tv1[k]];
+        for (int f0 = 0; f0 < fill; f0 += 64) {  // wave-uniform: every lane takes part in the shuffles
+            const int f = f0 + lane;
+            int l = 0;
+            for (int j = 1; j < dv; ++j) l = __builtin_amdgcn_readlane(excl, j) <= f ? j : l;
+            const int q = __shfl(s0[i], l, 64) + (f - __shfl(excl, l, 64));
+            if (f >= fill) continue;
+            const int key = a.adj[q] * kT5Tables + k;
+            const unsigned bit = 1u << l;
+            int h = t5_hash(key);
+            while (true) {
+                const int old = atomicCAS(&tkey[h], -1, key);
+                if (old == -1 || old == key) {
+                    dup |= (atomicOr(&tmask[h], bit) & bit) != 0u;
+                    break;
+                }
+                h = (h + 1) & (kT5Slots - 1);
+            }
+        }
+        // a list that repeats a value disables its table
+        if (__ballot(dup) != 0ull && lane == 0) tok[k] = 0;
+    }
+    __syncthreads();
+    if (on) {
+        const bool table = run < ntab && tok[run] && nq <= 255;
+        bool direct = !table;
+        if (table) {
+            // per-l1 counts of N(n2)'s mids, bit-sliced over 8 planes
+            unsigned p[8] = {0u, 0u, 0u, 0u, 0u, 0u, 0u, 0u};
+            int prev = -1;
+            for (int q = bq; q < bq + nq; ++q) {
+                const int m = a.adj[q];
+                if (m == prev) {  // N(n2) repeats a value: the min(a, b) rule needs the direct form
+                    direct = true;
+                    break;
+                }
+                prev = m;
+                const int want = m * kT5Tables + run;
+                int h = t5_hash(want);
+                unsigned mk = 0u;
+                while (true) {
+                    const int key = tkey[h];
+                    if (key == want) {
+                        mk = tmask[h];
+                        break;
+                    }
+                    if (key == -1) break;
+                    h = (h + 1) & (kT5Slots - 1);
+                }
+#pragma unroll
+                for (int b = 0; b < 8; ++b) {
+                    const unsigned carry = p[b] & mk;
+                    p[b] ^= mk;
+                    mk = carry;
+                }
+            }
+            if (!direct) {
+                unsigned nz = 0u;
+#pragma unroll
+                for (int b = 0; b < 8; ++b) nz |= p[b];
+                TileAcc acc;
+                for (int l = 0; l < d1; ++l) {
+                    int c = 0;
+                    if ((nz >> l) & 1u) {
+#pragma unroll
+                        for (int b = 0; b < 8; ++b) c |= (int)((p[b] >> l) & 1u) << b;
+                        if (a.adj[b1 + l] == n2) c = 0;
+                    }
+                    cnt8[x0 + l] = (unsigned char)(c < 255 ? c : 255);
+                    acc.add(tile_tot, x0 + l, 3 * (i64)c);
+                    mine += 3 * (i64)c;
+                }
+                acc.flush(tile_tot);
+            }
+        }
+        if (direct) count5_direct(a, b1, d1, n2, bq, nq, x0, cnt8, tile_tot, mine);
+        xend = x0 + d1;
+        if (a.stats) {
+            atomicAdd(a.stats, 1ull);
+            if (!direct) atomicAdd(a.stats + 1, 1ull);
+        }
     }
     if (a.nslot) wave_add_keyed(a.slot_total, slot, mine, on);
     if (a.counted) block_counted(a.counted, mine, xend);
@@ -767,6 +1010,7 @@ static i64 run_stage(Ctx& ctx, const Csr& G, const int* rv1, const int* rv2, i64
         RAMA_LAUNCHED();
     }
     CycArgs a{OFF, G.adj.get(), rv1, rv2, S > 3 ? xoff.get() : nullptr, S > 3 ? tile_rep.get() : nullptr, R, X};
+    a.tables = G.n < kT5MaxNode ? 1 : 0;
     DBuf<unsigned char> cnt8 = ctx.buf<unsigned char>(X);
     DBuf<i64> tile = ctx.buf<i64>(ntiles + 1);
     RAMA_REQUIRE(ntiles < ((i64)1 << 31), "too many cycle expansions");
@@ -826,6 +1070,13 @@ static i64 run_stage(Ctx& ctx, const Csr& G, const int* rv1, const int* rv2, i64
                 nblk = (unsigned)ytiles;
             }
         }
+        static const bool cyc_stats = std::getenv("RAMA_CYCLE_STATS") != nullptr;
+        DBuf<unsigned long long> st;
+        if (cyc_stats && S == 5) {
+            st = ctx.buf<unsigned long long>(2);
+            ctx.zero(st.get(), 2);
+            a.stats = st.get();
+        }
         const bool early = !ctx.verbose && budget > base;
         const i64 need = budget - base;
         unsigned done = 0, chunk = early ? std::max<unsigned>(nblk / 64, 256u) : nblk;
@@ -867,6 +1118,12 @@ static i64 run_stage(Ctx& ctx, const Csr& G, const int* rv1, const int* rv2, i64
             const double grow = 0.01 * (double)tune::get(tune::CycleChunkPct, X);
             const double want = per_block > 0.0 ? grow * (double)(need - hc[0]) / per_block + 64.0 : 1e30;
             chunk = want >= (double)(nblk - done) ? nblk - done : (unsigned)want;
+        }
+        if (a.stats) {
+            unsigned long long h[2];
+            ctx.read(a.stats, h, 2);
+            std::fprintf(stderr, "5-cycle count: %llu threads, %llu through 2-hop tables (%d blocks)\n", h[0], h[1], (int)done);
+            a.stats = nullptr;
         }
         if (done < nblk) {
             // counted expansions [0, x_end): emit the tiles they touch; the
@@ -927,6 +1184,7 @@ void run_stage_packed(Ctx& ctx, const Csr& G, const int* rv1, const int* rv2, i6
     i64* TOT = slot.get();
     ctx.zero(TOT, Q);
     CycArgs a{OFF, G.adj.get(), rv1, rv2, S > 3 ? xoff.get() : nullptr, S > 3 ? tile_rep.get() : nullptr, R, X};
+    a.tables = G.n < kT5MaxNode ? 1 : 0;
     a.nslot = nslot;
     a.slot_total = TOT;
     a.slot_first = TOT + Q;
