@@ -765,54 +765,27 @@ __device__ __forceinline__ void put_tri(int* t1, int* t2, int* t3, i64 g, i64 bu
     t3[g] = hi;
 }
 
+// One queued expansion per lane (have: the lane holds one), counted c > 0
+// with its first write at `pos` in enumeration order: resolve it, move to the
+// instance's triangle region on packed graphs, then write its triangles by
+// the merge walk, by probing the short list into the long one, or (two long
+// lists) with the whole wave.  Wave-uniform call.
 template <int S>
-__global__ __launch_bounds__(kBlock) void k_cycle_emit(CycArgs a, const unsigned char* __restrict__ cnt8,
-                                                       const i64* __restrict__ tile_off, i64 base, i64 budget,
-                                                       int* __restrict__ t1, int* __restrict__ t2,
-                                                       int* __restrict__ t3) {
-    __shared__ i64 ws[kWaves];
-    // plain block order: tiles past the budget exit at once, and keeping the
-    // working tiles spread over all XCDs balances them
-    const unsigned blk = blockIdx.x;
-    const bool packed = a.nslot != nullptr;
-    if (tile_off[blk + 1] == tile_off[blk]) return;  // uniform: the tile writes nothing
-    const i64 tstart = (packed ? 0 : base) + tile_off[blk];
-    if (!packed && tstart >= budget) return;  // uniform per block
-    const i64 x = (i64)blk * kCycTile + threadIdx.x;
-    int c = 0;
+__device__ __forceinline__ void emit_item(const CycArgs& a, bool have, i64 x, i64 pos, i64 budget, int* t1, int* t2,
+                                          int* t3) {
     int v1 = 0, v2 = 0, n1 = 0, n2 = 0, qa = 0, qb = 0;
-    bool recount = false;
-    if (x < a.X) {
-        c = cnt8[x];
-        if (c > 0) {
-            resolve<S>(a, x, S == 3 ? 0 : a.tile_rep[blk], v1, v2, n1, n2, qa, qb);
-            if (c == 255) {
-                // the count saturated its byte: count again (by the wave
-                // when both lists are long)
-                int sb, se, lb, le;
-                short_long(a.off, qa, qb, sb, se, lb, le);
-                if (se - sb <= kProbe) c = probe_count(a.adj, sb, se, lb, le);
-                else recount = true;
-            }
+    i64 mypos = pos, tofs = 0;  // tofs: start of the instance's triangle region (packed)
+    if (have) {
+        resolve<S>(a, x, S == 3 ? 0 : a.tile_rep[x / kCycTile], v1, v2, n1, n2, qa, qb);
+        if (a.nslot) {
+            // instance q's own position, budget and region
+            const int q = a.nslot[v1];
+            mypos = a.slot_base[q] + (pos - a.slot_first[q]);
+            budget = a.slot_budget[q];
+            tofs = a.slot_out[q];
         }
     }
-    for (unsigned long long hm = __ballot(recount); hm; hm &= hm - 1) {
-        const int h = __builtin_ctzll(hm);
-        const int s = wave_count_common(a.off, a.adj, __builtin_amdgcn_readlane(qa, h),
-                                        __builtin_amdgcn_readlane(qb, h));
-        if (lane_id() == h) c = s;
-    }
-    i64 tot;
-    i64 mypos = tstart + block_excl_scan((i64)c * (S - 2), ws, &tot);
-    i64 tofs = 0;  // start of the instance's triangle region (packed)
-    if (packed && c > 0) {
-        // instance q's own position, budget and region
-        const int q = a.nslot[v1];
-        mypos = a.slot_base[q] + (mypos - a.slot_first[q]);
-        budget = a.slot_budget[q];
-        tofs = a.slot_out[q];
-    }
-    const bool writes = c > 0 && mypos < budget;
+    const bool writes = have && mypos < budget;
     const int i0 = writes ? a.off[qa] : 0, ie = writes ? a.off[qa + 1] : 0;
     const int j0 = writes ? a.off[qb] : 0, je = writes ? a.off[qb + 1] : 0;
     const bool big = writes && (ie - i0) + (je - j0) > kHeavy;
@@ -907,6 +880,104 @@ __global__ __launch_bounds__(kBlock) void k_cycle_emit(CycArgs a, const unsigned
         i += (p <= q);
         j += (q <= p);
     }
+}
+
+__device__ __forceinline__ i64 wave_excl_scan64(i64 v, i64* total) {
+    i64 x = v;
+#pragma unroll
+    for (int d = 1; d < 64; d <<= 1) {
+        const i64 t = __shfl_up(x, d, 64);
+        if (lane_id() >= d) x += t;
+    }
+    *total = readlane64(x, 63);
+    return x - v;
+}
+
+// Emit: one wave per block walks a contiguous range of count tiles in
+// order.  Each tile's 256 counts are read four per lane (expansion order is
+// lane-major), a wave scan turns them into write positions (tile start +
+// the counts before), and the expansions with mids are queued in LDS; every
+// 64 queued expansions are written by the 64 lanes at once (emit_item).
+// Most expansions have no mid, so lanes stay busy instead of idling beside
+// the few that write.  Saturated counts (255) are counted again first.
+constexpr int kEmitQueue = 320;  // at most 63 waiting + 256 from one tile
+constexpr i64 kEmitWaves = 6144;  // emit blocks (one wave each) at most
+
+template <int S>
+__global__ __launch_bounds__(64) void k_cycle_emit(CycArgs a, const unsigned char* __restrict__ cnt8,
+                                                   const i64* __restrict__ tile_off, i64 ntiles, i64 per_block,
+                                                   i64 base, i64 budget, int* __restrict__ t1, int* __restrict__ t2,
+                                                   int* __restrict__ t3) {
+    __shared__ i64 qx[kEmitQueue];
+    __shared__ i64 qp[kEmitQueue];
+    const int lane = lane_id();
+    const bool packed = a.nslot != nullptr;
+    const i64 tb = (i64)blockIdx.x * per_block;
+    const i64 te = tb + per_block < ntiles ? tb + per_block : ntiles;
+    int qn = 0;  // queued expansions (wave-uniform)
+    for (i64 t = tb; t < te; ++t) {
+        const i64 o0 = tile_off[t], o1 = tile_off[t + 1];
+        if (o1 == o0) continue;  // the tile writes nothing (uniform)
+        const i64 tstart = (packed ? 0 : base) + o0;
+        if (!packed && tstart >= budget) break;  // later tiles start later still
+        const i64 x0 = t * kCycTile + 4 * lane;
+        int c[4];
+#pragma unroll
+        for (int k = 0; k < 4; ++k) c[k] = x0 + k < a.X ? (int)cnt8[x0 + k] : 0;
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+            const bool sat = c[k] == 255;
+            if (__ballot(sat) == 0ull) continue;
+            // the count saturated its byte: count again (by the wave when
+            // both lists are long)
+            int v1 = 0, v2 = 0, n1 = 0, n2 = 0, qa = 0, qb = 0;
+            bool wave = false;
+            if (sat) {
+                resolve<S>(a, x0 + k, S == 3 ? 0 : a.tile_rep[t], v1, v2, n1, n2, qa, qb);
+                int sb, se, lb, le;
+                short_long(a.off, qa, qb, sb, se, lb, le);
+                if (se - sb <= kProbe) c[k] = probe_count(a.adj, sb, se, lb, le);
+                else wave = true;
+            }
+            for (unsigned long long hm = __ballot(wave); hm; hm &= hm - 1) {
+                const int h = __builtin_ctzll(hm);
+                const int s = wave_count_common(a.off, a.adj, __builtin_amdgcn_readlane(qa, h),
+                                                __builtin_amdgcn_readlane(qb, h));
+                if (lane == h) c[k] = s;
+            }
+        }
+        i64 tot;
+        i64 p = tstart + wave_excl_scan64((i64)(c[0] + c[1] + c[2] + c[3]) * (S - 2), &tot);
+        const int nz = (c[0] > 0) + (c[1] > 0) + (c[2] > 0) + (c[3] > 0);
+        int zx = nz;
+#pragma unroll
+        for (int d = 1; d < 64; d <<= 1) {
+            const int u = __shfl_up(zx, d, 64);
+            if (lane >= d) zx += u;
+        }
+        const int ztot = __builtin_amdgcn_readlane(zx, 63);
+        int slot = qn + zx - nz;
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+            if (c[k] > 0) {
+                qx[slot] = x0 + k;
+                qp[slot] = p;
+                ++slot;
+            }
+            p += (i64)c[k] * (S - 2);
+        }
+        qn += ztot;
+        __syncthreads();
+        while (qn >= 64) {
+            qn -= 64;
+            const i64 x = qx[qn + lane], pos = qp[qn + lane];
+            __syncthreads();
+            emit_item<S>(a, true, x, pos, budget, t1, t2, t3);
+        }
+    }
+    __syncthreads();
+    const bool have = lane < qn;
+    emit_item<S>(a, have, have ? qx[lane] : 0, have ? qp[lane] : 0, budget, t1, t2, t3);
 }
 
 // tile_rep[t] = rep edge containing the first expansion of tile t.
@@ -1138,8 +1209,9 @@ static i64 run_stage(Ctx& ctx, const Csr& G, const int* rv1, const int* rv2, i64
     const i64 W = ctx.read1(tile.get() + tiles_used);
     if (W > 0 && base < budget) {
         prof::Scope ps(ctx, S == 3 ? "cycles3_emit" : (S == 4 ? "cycles4_emit" : "cycles5_emit"), 0.0);
-        k_cycle_emit<S><<<(unsigned)tiles_used, kBlock, 0, ctx.stream>>>(a, cnt8.get(), tile.get(), base, budget,
-                                                                         T.t1.get(), T.t2.get(), T.t3.get());
+        const i64 nb = std::min<i64>(tiles_used, kEmitWaves), per = (tiles_used + nb - 1) / nb;
+        k_cycle_emit<S><<<(unsigned)((tiles_used + per - 1) / per), 64, 0, ctx.stream>>>(
+            a, cnt8.get(), tile.get(), tiles_used, per, base, budget, T.t1.get(), T.t2.get(), T.t3.get());
         RAMA_LAUNCHED();
     }
     return W;
@@ -1223,7 +1295,9 @@ void run_stage_packed(Ctx& ctx, const Csr& G, const int* rv1, const int* rv2, i6
     i64 W = 0;
     for (int q = 0; q < Q; ++q) W += h_total[q];
     if (W > 0) {
-        k_cycle_emit<S><<<(unsigned)ntiles, kBlock, 0, ctx.stream>>>(a, cnt8.get(), tile.get(), 0, 0, t1, t2, t3);
+        const i64 nb = std::min<i64>(ntiles, kEmitWaves), per = (ntiles + nb - 1) / nb;
+        k_cycle_emit<S><<<(unsigned)((ntiles + per - 1) / per), 64, 0, ctx.stream>>>(a, cnt8.get(), tile.get(), ntiles,
+                                                                                     per, 0, 0, t1, t2, t3);
         RAMA_LAUNCHED();
     }
 }
