@@ -232,8 +232,8 @@ TUNE_SOLVE_KNOBS = {
     "pick_blocks": (1024, 2048, 4096, 8192),
     "match_blocks": (512, 1024, 2048, 4096),
     "mp_fused_max_blocks": (0, 64, 256, 1024),
-    "scan_chunks": (512, 1024, 2048, 4096),
-    "cycle_chunk_pct": (110, 125, 150, 200),
+    "scan_chunks": (1024, 2048, 4096, 6144, 8192),
+    "cycle_chunk_pct": (100, 105, 110, 125, 150),
 }
 
 

@@ -132,7 +132,7 @@ __device__ __forceinline__ u64 scan_word(u32 epoch, u32 flag, u64 v) {
 // and no waiting on other blocks.  One chunk: the second launch only.
 constexpr int kScanItems = 8;
 constexpr int kScanTile = kBlock * kScanItems;
-constexpr unsigned kScanMaxBlocks = 4096;  // the tuning table picks up to this many
+constexpr unsigned kScanMaxBlocks = 8192;  // the tuning table picks up to this many
 
 // Items are evaluated and emitted in striped order (item k*kBlock + t of the
 // tile by thread t), so the loads inside f and the stores inside out are

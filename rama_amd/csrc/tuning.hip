@@ -34,7 +34,7 @@ const Spec& spec(int k) {
         {"sort_path_u64", 0, 2, {{(int64_t)64 * kRsTile, 2}, {-1, 0}}},
         {"sort_path_u32", 0, 2, {{(int64_t)64 * kRsTile, 2}, {-1, 0}}},
         {"sort_threads", 512, 1024, {{-1, 1024}}},
-        {"scan_chunks", 64, 4096, {{-1, 2048}}},
+        {"scan_chunks", 64, 8192, {{-1, 2048}}},
         {"pick_blocks", 64, 65536, {{-1, 2048}}},
         {"match_blocks", 64, 65536, {{-1, 1024}}},
         {"mp_fused_max_blocks", 0, 65536, {{-1, 256}}},
