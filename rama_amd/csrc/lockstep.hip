@@ -471,7 +471,7 @@ void packed_dual_update(Ctx& ctx, Pack& P, int c, int steps, float factor, float
     std::vector<int> tblk((size_t)Q + 1, 0), eblk((size_t)Q + 1, 0), onv((size_t)Q, 0);
     std::vector<i64> pbeg((size_t)Q, 0);
     for (int q = 0; q < Q; ++q) {
-        const int gt = (int)grid_for(tb[q + 1] - tb[q]), ge = (int)grid_for(eb[q + 1] - eb[q]);
+        const int gt = (int)grid_for(tb[q + 1] - tb[q]), ge = (int)mp_edge_blocks(eb[q + 1] - eb[q]);
         tblk[q + 1] = tblk[q] + gt;
         eblk[q + 1] = eblk[q] + ge;
         pbeg[q] = (q ? pbeg[q - 1] + (tblk[q] - tblk[q - 1]) + (eblk[q] - eblk[q - 1]) : 0);

@@ -371,36 +371,69 @@ __device__ __forceinline__ double mp_triangle(i64 t, const int* __restrict__ c12
     return (double)tri_lb(a, b, d);
 }
 
-// One edge's update: with kSweep, cost = sum of the slot messages (edges in
-// no triangle keep their cost); without, edges in a triangle are zeroed
-// (decrease_edge_costs_func, :231-239).  Returns its lower-bound term.
+// The edge update (send_messages_to_edges' sums, decrease_edge_costs_func,
+// :231-239): with kSweep, an edge's cost becomes the sum of its slot
+// messages (edges in no triangle keep their cost); without, edges in a
+// triangle are zeroed.  Its share cost / #triangles is stored beside it.
+//
+// Edges e and e + kBlock (those below end) of one thread: offsets, costs
+// and the first four messages of both are loaded before either is summed;
+// each edge's sum keeps its slot order.  Returns the two lower-bound terms
+// added (e's first).
 template <bool kSweep>
-__device__ __forceinline__ double mp_edge(i64 e, const int* __restrict__ soff, const float* __restrict__ delta,
-                                          float* __restrict__ ec,
-                                          float* __restrict__ share) {
-    float c = ec[e];
-    const int s0 = soff[e], s1 = soff[e + 1];
-    if (s1 > s0) {
-        if (kSweep) {
-            // in slot order; four loads in flight ahead of the additions
-            double s = 0.0;
-            int k = s0;
-            for (; k + 4 <= s1; k += 4) {
-                const float x0 = delta[k], x1 = delta[k + 1], x2 = delta[k + 2], x3 = delta[k + 3];
-                s += (double)x0;
-                s += (double)x1;
-                s += (double)x2;
-                s += (double)x3;
-            }
-            for (; k < s1; ++k) s += (double)delta[k];
-            c = (float)s;
-        } else {
-            c = 0.0f;
-        }
-        ec[e] = c;
-        share[e] = edge_share(c, s1 - s0);
+__device__ __forceinline__ double mp_edge_pair(i64 e, i64 end, const int* __restrict__ soff,
+                                               const float* __restrict__ delta, float* __restrict__ ec,
+                                               float* __restrict__ share) {
+    constexpr int kPre = 4;
+    i64 ee[2] = {e, e + kBlock};
+    bool has[2];
+    int s0[2], s1[2];
+    float c[2];
+    float x[2][kPre];
+#pragma unroll
+    for (int h = 0; h < 2; ++h) {
+        has[h] = ee[h] < end;
+        s0[h] = has[h] ? soff[ee[h]] : 0;
+        s1[h] = has[h] ? soff[ee[h] + 1] : 0;
+        c[h] = has[h] ? ec[ee[h]] : 0.0f;
     }
-    return c < 0.0f ? (double)c : 0.0;
+    if (kSweep) {
+#pragma unroll
+        for (int h = 0; h < 2; ++h)
+#pragma unroll
+            for (int u = 0; u < kPre; ++u) x[h][u] = s0[h] + u < s1[h] ? delta[s0[h] + u] : 0.0f;
+    }
+    double lb = 0.0;
+#pragma unroll
+    for (int h = 0; h < 2; ++h) {
+        if (!has[h]) continue;
+        float cc = c[h];
+        if (s1[h] > s0[h]) {
+            if (kSweep) {
+                // in slot order
+                double sum = 0.0;
+#pragma unroll
+                for (int u = 0; u < kPre; ++u)
+                    if (s0[h] + u < s1[h]) sum += (double)x[h][u];
+                int k = s0[h] + kPre;
+                for (; k + 4 <= s1[h]; k += 4) {
+                    const float y0 = delta[k], y1 = delta[k + 1], y2 = delta[k + 2], y3 = delta[k + 3];
+                    sum += (double)y0;
+                    sum += (double)y1;
+                    sum += (double)y2;
+                    sum += (double)y3;
+                }
+                for (; k < s1[h]; ++k) sum += (double)delta[k];
+                cc = (float)sum;
+            } else {
+                cc = 0.0f;
+            }
+            ec[ee[h]] = cc;
+            share[ee[h]] = edge_share(cc, s1[h] - s0[h]);
+        }
+        lb += cc < 0.0f ? (double)cc : 0.0;
+    }
+    return lb;
 }
 
 template <bool kSweep>
@@ -437,8 +470,8 @@ __global__ __launch_bounds__(kBlock) void k_mp_edges(i64 E, const int* __restric
     __shared__ bool last;
     if (ctl && !ctl->active) return;
     const unsigned blk = xcd_block();
-    const i64 e = (i64)blk * kBlock + threadIdx.x;
-    const double lb = e < E ? mp_edge<kSweep>(e, soff, delta, ec, share) : 0.0;
+    const i64 e = (i64)blk * kMpEdgesPerBlock + threadIdx.x;
+    const double lb = mp_edge_pair<kSweep>(e, E, soff, delta, ec, share);
     const double r = block_reduce(lb, OpSum(), ws);
     if (threadIdx.x == 0) lbpart[blk] = r;
     if (!parts) return;
@@ -468,7 +501,7 @@ static double sum_parts(Ctx& ctx, MPState& S, i64 n) {
 
 static void launch_round(Ctx& ctx, MPState& S, bool sweep, MPControl* ctl = nullptr) {
     const i64 T = S.T, E = S.E;
-    const unsigned gt = grid_for(T), ge = grid_for(E);
+    const unsigned gt = grid_for(T), ge = mp_edge_blocks(E);
     double* P = S.lbparts.get();
     {
         // per triangle: 3 edge ids, 3 share gathers, 3 costs in and out,
@@ -547,8 +580,8 @@ __global__ __launch_bounds__(kBlock) void k_mp_edges_packed(MPPack pk, const int
     const int q = block_owner(pk.eblk, pk.Q, (int)blockIdx.x);
     if (!ctl[q].active) return;
     const int lb = (int)blockIdx.x - pk.eblk[q];
-    const i64 e = pk.ebeg[q] + (i64)lb * kBlock + threadIdx.x;
-    const double v = e < pk.ebeg[q + 1] ? mp_edge<kSweep>(e, soff, delta, ec, share) : 0.0;
+    const i64 e = pk.ebeg[q] + (i64)lb * kMpEdgesPerBlock + threadIdx.x;
+    const double v = mp_edge_pair<kSweep>(e, pk.ebeg[q + 1], soff, delta, ec, share);
     const double r = block_reduce(v, OpSum(), ws);
     if (threadIdx.x == 0) parts[pk.pbeg[q] + (pk.tblk[q + 1] - pk.tblk[q]) + lb] = r;
 }
@@ -680,7 +713,7 @@ void mp_iteration(Ctx& ctx, MPState& S) { launch_round(ctx, S, true); }
 // Lower bound right after mp_iteration / mp_send_to_triplets, from the
 // fused per-block partials.
 double mp_fused_lower_bound(Ctx& ctx, MPState& S) {
-    const i64 n = (i64)grid_for(S.T) + (i64)grid_for(S.E);
+    const i64 n = (i64)grid_for(S.T) + (i64)mp_edge_blocks(S.E);
     return sum_parts(ctx, S, n);
 }
 

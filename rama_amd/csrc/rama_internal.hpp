@@ -122,6 +122,13 @@ void normalize_triangles(Ctx& ctx, Triangles& T);
 // deduplicate (in place); returns the count.  Ids are below n_nodes.
 i64 rearrange_triangles(Ctx& ctx, Triangles& T, i64 valid, int n_nodes);
 
+// The message-passing edge sweep gives each thread two edges (e and
+// e + kBlock of its block's kMpEdgesPerBlock), so both edges' loads are in
+// flight together; its lower-bound partials are per block of that many
+// edges, in the single and the packed launches alike.
+constexpr int kMpEdgesPerBlock = 2 * kBlock;
+inline unsigned mp_edge_blocks(i64 E) { return grid_for(E, kMpEdgesPerBlock); }
+
 // ---- mp.hip
 struct MessagePassing;  // opaque
 struct MPState {
