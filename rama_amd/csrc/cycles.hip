@@ -401,25 +401,47 @@ __global__ __launch_bounds__(kBlock) void k_cycle_count3(CycArgs a, unsigned cha
 // 4-cycles: one thread per repulsive edge; its expansions n1 ∈ N(v1) are
 // xoff[r] .. xoff[r+1]-1 in order, each intersecting N(n1) with N(v2)
 // (held in registers).  tile_tot must be zero on entry.
+// Rep edges whose v1 has at least kWideRep neighbours, or whose v2 has more
+// than kRegList, are counted by the whole wave, one rep edge at a time: the
+// wave puts N(v2) in an LDS hash set (kSetSlots slots, lists of at most
+// kSetMax entries) and lane l takes expansions l, l + 64, ..., streaming
+// N(n1) and probing each entry.  After contraction supernodes have dozens
+// of neighbours: one lane per rep edge would walk long merge walks serially
+// while its wave waits.  A repeated value in either list (the min(a, b)
+// rule) or a longer N(v2) keeps the merge walk for that expansion.
+constexpr int kWideRep = 48;
+constexpr int kSetSlots = 256;
+constexpr int kSetMax = 128;
+
+__device__ __forceinline__ int set_hash(int m) { return (int)(((unsigned)m * 2654435761u) >> 24); }
+
 __global__ __launch_bounds__(kBlock) void k_cycle_count4(CycArgs a, unsigned char* __restrict__ cnt8,
                                                          i64* __restrict__ tile_tot) {
+    __shared__ int wset[kWaves][kSetSlots];
+    const int w = threadIdx.x >> 6;
     const i64 r = (i64)(a.blk0 + xcd_block()) * kBlock + threadIdx.x;
     const bool on = r < a.R;
     i64 mine = 0, xend = 0;
+    int v1 = 0, v2 = 0, b1 = 0, d1 = 0, b2 = 0, n2 = 0;
+    i64 x0 = 0;
     if (on) {
-        const int v1 = a.rv1[r], v2 = a.rv2[r];
-        const int b1 = a.off[v1], d1 = a.off[v1 + 1] - b1;
-        const int b2 = a.off[v2], n2 = a.off[v2 + 1] - b2;
-        const i64 x0 = a.xoff[r];
+        v1 = a.rv1[r];
+        v2 = a.rv2[r];
+        b1 = a.off[v1];
+        d1 = a.off[v1 + 1] - b1;
+        b2 = a.off[v2];
+        n2 = a.off[v2 + 1] - b2;
+        x0 = a.xoff[r];
+        xend = x0 + d1;
+    }
+    const bool wide = on && (d1 >= kWideRep || n2 > kRegList);
+    if (on && !wide) {
         int rb[kRegList];
-        const bool regs = n2 <= kRegList;
-        int rmax = -1, kw = kRegList;
+        int rmax = -1;
         bool rdup = false;
-        if (regs) {
-            load_reg_list(a.adj, b2, n2, rb);
-            reg_list_props(rb, n2, rmax, rdup);
-            kw = reg_width(n2);
-        }
+        load_reg_list(a.adj, b2, n2, rb);
+        reg_list_props(rb, n2, rmax, rdup);
+        const int kw = reg_width(n2);
         TileAcc acc;
         // the next n1's list bounds are loaded while this one is walked
         int ia = 0, ie = 0;
@@ -435,8 +457,7 @@ __global__ __launch_bounds__(kBlock) void k_cycle_count4(CycArgs a, unsigned cha
                 na = a.off[nn];
                 ne = a.off[nn + 1];
             }
-            const int c = regs ? count_vs_regs_fast(a.adj, ia, ie, rb, rmax, rdup, kw)
-                               : lane_count_common(a.adj, ia, ie, b2, n2);
+            const int c = count_vs_regs_fast(a.adj, ia, ie, rb, rmax, rdup, kw);
             cnt8[x0 + l] = (unsigned char)(c < 255 ? c : 255);
             acc.add(tile_tot, x0 + l, 2 * (i64)c);
             mine += 2 * (i64)c;
@@ -444,9 +465,78 @@ __global__ __launch_bounds__(kBlock) void k_cycle_count4(CycArgs a, unsigned cha
             ie = ne;
         }
         acc.flush(tile_tot);
-        xend = x0 + d1;
     }
-    if (a.nslot) wave_add_keyed(a.slot_total, on ? a.nslot[a.rv1[r]] : 0, mine, on);
+    // wide rep edges, one at a time by the whole wave; the attempted writes
+    // are credited to the rep's own lane (its instance, its block total)
+    const int lane = lane_id();
+    for (unsigned long long hm = __ballot(wide); hm; hm &= hm - 1) {
+        const int h = __builtin_ctzll(hm);
+        const int hb1 = __builtin_amdgcn_readlane(b1, h), hd1 = __builtin_amdgcn_readlane(d1, h);
+        const int hb2 = __builtin_amdgcn_readlane(b2, h), hn2 = __builtin_amdgcn_readlane(n2, h);
+        const i64 hx0 = readlane64(x0, h);
+        // N(v2) into the wave's set (a repeated value, or too long a list:
+        // merge walks instead)
+        bool use_set = hn2 <= kSetMax;
+        int vmax = -1;
+        if (use_set) {
+            for (int k = lane; k < kSetSlots; k += 64) wset[w][k] = -1;
+            __builtin_amdgcn_wave_barrier();
+            bool dup = false;
+            for (int k = lane; k < hn2; k += 64) {
+                const int m = a.adj[hb2 + k];
+                dup |= k > 0 && a.adj[hb2 + k - 1] == m;
+                int hs = set_hash(m);
+                while (true) {
+                    const int old = atomicCAS(&wset[w][hs], -1, m);
+                    if (old == -1 || old == m) break;
+                    hs = (hs + 1) & (kSetSlots - 1);
+                }
+            }
+            use_set = __ballot(dup) == 0ull;
+            vmax = hn2 > 0 ? a.adj[hb2 + hn2 - 1] : -1;
+            __builtin_amdgcn_wave_barrier();
+        }
+        TileAcc acc;
+        i64 part = 0;
+        for (int l = lane; l < hd1; l += 64) {
+            const int n1 = a.adj[hb1 + l];
+            const int ia = a.off[n1], ie = a.off[n1 + 1];
+            int c = 0;
+            bool merge = !use_set;
+            if (use_set) {
+                int prev = -1;
+                for (int k = ia; k < ie; ++k) {
+                    const int x = a.adj[k];
+                    if (x > vmax) break;
+                    if (x == prev) {  // N(n1) repeats a value: the merge walk counts it
+                        merge = true;
+                        break;
+                    }
+                    prev = x;
+                    int hs = set_hash(x);
+                    while (true) {
+                        const int key = wset[w][hs];
+                        if (key == x) {
+                            ++c;
+                            break;
+                        }
+                        if (key == -1) break;
+                        hs = (hs + 1) & (kSetSlots - 1);
+                    }
+                }
+            }
+            if (merge) c = lane_count_common(a.adj, ia, ie, hb2, hn2);
+            cnt8[hx0 + l] = (unsigned char)(c < 255 ? c : 255);
+            acc.add(tile_tot, hx0 + l, 2 * (i64)c);
+            part += 2 * (i64)c;
+        }
+        acc.flush(tile_tot);
+#pragma unroll
+        for (int d = 32; d >= 1; d >>= 1) part += __shfl_xor(part, d, 64);
+        if (lane == h) mine += part;
+        __builtin_amdgcn_wave_barrier();
+    }
+    if (a.nslot) wave_add_keyed(a.slot_total, on ? a.nslot[v1] : 0, mine, on);
     if (a.counted) block_counted(a.counted, mine, xend);
 }
 
