@@ -300,6 +300,7 @@ struct Ctx {
     }
     ~Ctx() {
         if (stream || no_alloc) (void)hipStreamSynchronize(stream);
+        drop_graphs();
         if (sync_ev) (void)hipEventDestroy(sync_ev);
         arena.trim();
         if (!no_alloc) pool::give_pinned(pinned);
@@ -375,11 +376,59 @@ struct Ctx {
         }();
         if (by_stream || no_alloc) {
             RAMA_HIP(hipStreamSynchronize(stream));
+            drop_graphs();
             return;
         }
         if (!sync_ev) RAMA_HIP(hipEventCreateWithFlags(&sync_ev, hipEventDisableTiming));
         RAMA_HIP(hipEventRecord(sync_ev, stream));
         RAMA_HIP(hipEventSynchronize(sync_ev));
+        drop_graphs();
+    }
+
+    // Runs `body` `times` times in a row.  body only launches kernels on
+    // this stream (no host waits, no memsets, no allocation).  With
+    // use_graph the first run is captured into a HIP graph and every run
+    // replays it: one launch per run instead of one per kernel.  Capture is
+    // skipped on the null stream, in the allocation-free mode (graph
+    // instantiation allocates) and when the stream is being captured by the
+    // caller.  The graphs live until the stream has been synchronised.
+    template <class F>
+    void repeat(int times, bool use_graph, F&& body) {
+        if (times <= 0) return;
+        hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+        if (use_graph && times > 1 && !no_alloc && stream != nullptr &&
+            hipStreamIsCapturing(stream, &cs) == hipSuccess && cs == hipStreamCaptureStatusNone) {
+            RAMA_HIP(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal));
+            hipGraph_t g = nullptr;
+            try {
+                body();
+            } catch (...) {
+                if (hipStreamEndCapture(stream, &g) == hipSuccess && g) (void)hipGraphDestroy(g);
+                (void)hipGetLastError();
+                throw;
+            }
+            RAMA_HIP(hipStreamEndCapture(stream, &g));
+            hipGraphExec_t ge = nullptr;
+            const hipError_t e = hipGraphInstantiate(&ge, g, nullptr, nullptr, 0);
+            if (e != hipSuccess) {
+                (void)hipGraphDestroy(g);
+                RAMA_HIP(e);
+            }
+            graphs.push_back(g);
+            graph_execs.push_back(ge);
+            for (int k = 0; k < times; ++k) RAMA_HIP(hipGraphLaunch(ge, stream));
+            return;
+        }
+        (void)hipGetLastError();
+        for (int k = 0; k < times; ++k) body();
+    }
+    std::vector<hipGraph_t> graphs;
+    std::vector<hipGraphExec_t> graph_execs;
+    void drop_graphs() {
+        for (hipGraphExec_t ge : graph_execs) (void)hipGraphExecDestroy(ge);
+        for (hipGraph_t g : graphs) (void)hipGraphDestroy(g);
+        graph_execs.clear();
+        graphs.clear();
     }
 
     // Host-to-device upload through a pinned staging ring, asynchronous on

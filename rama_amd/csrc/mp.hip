@@ -17,6 +17,9 @@
 // Lower-bound partials are per block in double.  In the device-driven loop
 // the edge kernel's last block sums them and applies the stall test, so the
 // host reads the outcome once per loop.
+#include <cstdlib>
+#include <cstring>
+
 #include "rama_internal.hpp"
 #include "tuning.hpp"
 
@@ -25,6 +28,17 @@ namespace rama {
 // What an edge in `cnt` triangles passes to each of them
 // (send_messages_to_triplets, src/multicut_message_passing.cu:218-229).
 __device__ __forceinline__ float edge_share(float c, int cnt) { return c / float(cnt); }
+
+// Whether the iterations of a message-passing loop replay one captured HIP
+// graph (every iteration launches the same kernels with the same arguments;
+// the stall test on the device turns the rest into no-ops).
+// RAMA_MP_GRAPH=0 launches them one by one; the kernel profiler's events
+// are not captured, so timing these kernels also launches them directly.
+static bool mp_graphs() {
+    const char* e = std::getenv("RAMA_MP_GRAPH");
+    if (e && std::strcmp(e, "0") == 0) return false;
+    return !prof::wants("mp_triangles") && !prof::wants("mp_edges");
+}
 
 
 // out[i] = lower_bound(a[0..n), q[i]) for ascending queries q[0..m).  With
@@ -637,7 +651,7 @@ void mp_run_packed(Ctx& ctx, MPState& S, const MPPack& pk, int total_tblocks, in
     RAMA_LAUNCHED();
     const i64 nparts = (i64)total_tblocks + total_eblocks;
     DBuf<double> parts = ctx.buf<double>(nparts);
-    for (int it = 0; it < steps; ++it) {
+    ctx.repeat(steps, mp_graphs(), [&] {
         k_mp_triangles_packed<true><<<(unsigned)total_tblocks, kBlock, 0, ctx.stream>>>(
             pk, S.c12.get(), S.c13.get(), S.c23.get(), S.share.get(), S.t12.get(), S.t13.get(), S.t23.get(),
             S.pos.get(), S.delta.get(), parts.get(), ctl.get());
@@ -647,7 +661,7 @@ void mp_run_packed(Ctx& ctx, MPState& S, const MPPack& pk, int total_tblocks, in
         RAMA_LAUNCHED();
         k_mp_finalize_packed<<<(unsigned)Q, kBlock, 0, ctx.stream>>>(pk, parts.get(), ctl.get());
         RAMA_LAUNCHED();
-    }
+    });
     std::vector<MPControl> h((size_t)Q);
     ctx.d2h(h.data(), ctl.get(), Q);
     for (int q = 0; q < Q; ++q) lb_out[q] = h[q].lb;
@@ -698,9 +712,8 @@ double mp_run(Ctx& ctx, MPState& S, int steps) {
         C->max_col = DM ? DM[0] : -1;
         C->last_row = LR ? *LR : -1;
     });
-    for (int it = 0; it < steps; ++it) {
-        launch_round(ctx, S, true, C);
-    }
+    ctx.reduce_ticket();  // created (and cleared) before any capture
+    ctx.repeat(steps, mp_graphs(), [&] { launch_round(ctx, S, true, C); });
     MPControl h;
     ctx.read(C, &h, 1);
     S.max_col = h.max_col;
