@@ -91,6 +91,9 @@ namespace rama {
 
 void set_last_error(const std::string& s) { g_last_error = s; }
 
+static void solve_graph(Ctx& ctx, DCoo& A, const Options& o, const Sanitized& san, DBuf<int>& labels, i64& n_labels,
+                        double* lb_out, int32_t* time_ms, rama_timeline** tl);
+
 // Shared body of the solve entry points; `i`, `j`, `c` are device buffers
 // owned by this call.
 void solve_common(Ctx& ctx, DBuf<int>& i, DBuf<int>& j, DBuf<float>& c, i64 m, const rama_options_c* copts,
@@ -104,6 +107,36 @@ void solve_common(Ctx& ctx, DBuf<int>& i, DBuf<int>& j, DBuf<float>& c, i64 m, c
     i.reset();
     j.reset();
     c.reset();
+    solve_graph(ctx, A, o, san, labels, n_labels, lb_out, time_ms, tl);
+}
+
+// The device entry points: the caller's arrays are read in place (building
+// the graph only reads them); sanitising edits its input, so it works on
+// copies.
+static void solve_view(Ctx& ctx, const int32_t* d_i, const int32_t* d_j, const float* d_c, i64 m,
+                       const rama_options_c* copts, DBuf<int>& labels, i64& n_labels, double* lb_out,
+                       int32_t* time_ms, rama_timeline** tl) {
+    RAMA_REQUIRE(copts != nullptr, "options must not be NULL");
+    RAMA_REQUIRE(m > 0, "the graph has no edges");
+    if (copts->sanitize_graph) {
+        DBuf<int> di = ctx.buf<int>(m), dj = ctx.buf<int>(m);
+        DBuf<float> dc = ctx.buf<float>(m);
+        ctx.d2d(di.get(), d_i, m);
+        ctx.d2d(dj.get(), d_j, m);
+        ctx.d2d(dc.get(), d_c, m);
+        solve_common(ctx, di, dj, dc, m, copts, labels, n_labels, lb_out, time_ms, tl);
+        return;
+    }
+    const Options o = to_options(*copts);
+    // the host entry's argument order: dcoo_from_edges orients every edge as
+    // (min, max), so the reference's dCOO(col=j, row=i) (src/rama_cuda.cu:202)
+    // is the same graph
+    DCoo A = dcoo_from_edges(ctx, d_j, d_i, d_c, m);
+    solve_graph(ctx, A, o, Sanitized{}, labels, n_labels, lb_out, time_ms, tl);
+}
+
+static void solve_graph(Ctx& ctx, DCoo& A, const Options& o, const Sanitized& san, DBuf<int>& labels, i64& n_labels,
+                        double* lb_out, int32_t* time_ms, rama_timeline** tl) {
     const auto t0 = std::chrono::steady_clock::now();
     SolveResult r = solve(ctx, A, o);
     ctx.sync();
@@ -193,17 +226,11 @@ int rama_solve_device(const int32_t* d_i, const int32_t* d_j, const float* d_c, 
         RAMA_REQUIRE(d_i && d_j && d_c && n_labels_out && lb_out, "NULL argument");
         if (tl) *tl = nullptr;
         Ctx ctx(device, static_cast<hipStream_t>(stream), opts && opts->verbose);
-        DBuf<int> di = ctx.buf<int>(m), dj = ctx.buf<int>(m);
-        DBuf<float> dc = ctx.buf<float>(m);
-        ctx.d2d(di.get(), d_i, m);
-        ctx.d2d(dj.get(), d_j, m);
-        ctx.d2d(dc.get(), d_c, m);
         DBuf<int> labels;
         i64 n = 0;
         rama_timeline* t = nullptr;
         std::unique_ptr<rama_timeline> own;
-        // device overload builds dCOO(col=j, row=i) (src/rama_cuda.cu:202); orientation makes both equal
-        solve_common(ctx, di, dj, dc, m, opts, labels, n, lb_out, time_ms_out, tl ? &t : nullptr);
+        solve_view(ctx, d_i, d_j, d_c, m, opts, labels, n, lb_out, time_ms_out, tl ? &t : nullptr);
         own.reset(t);
         *n_labels_out = n;
         if (n > cap) throw Error(RAMA_ERR_INVALID, "label buffer too small: need " + std::to_string(n));
@@ -226,14 +253,9 @@ int rama_solve_device_noalloc(const int32_t* d_i, const int32_t* d_j, const floa
         void* region = noalloc_scratch(device);
         Ctx ctx(device, static_cast<hipStream_t>(stream), opts && opts->verbose, region,
                 (size_t)RAMA_NOALLOC_SCRATCH_BYTES);
-        DBuf<int> di = ctx.buf<int>(m), dj = ctx.buf<int>(m);
-        DBuf<float> dc = ctx.buf<float>(m);
-        ctx.d2d(di.get(), d_i, m);
-        ctx.d2d(dj.get(), d_j, m);
-        ctx.d2d(dc.get(), d_c, m);
         DBuf<int> labels;
         i64 n = 0;
-        solve_common(ctx, di, dj, dc, m, opts, labels, n, lb_out, time_ms_out, nullptr);
+        solve_view(ctx, d_i, d_j, d_c, m, opts, labels, n, lb_out, time_ms_out, nullptr);
         *n_labels_out = n;
         if (n > cap) throw Error(RAMA_ERR_INVALID, "label buffer too small: need " + std::to_string(n));
         if (n > 0) ctx.d2d(d_labels_out, labels.get(), n);

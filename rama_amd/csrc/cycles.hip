@@ -983,7 +983,49 @@ __device__ __forceinline__ i64 wave_excl_scan64(i64 v, i64* total) {
     return x - v;
 }
 
-// Emit: one wave per block walks a contiguous range of count tiles in
+// Dense emit: one thread per expansion (block = one count tile), write
+// positions by a block scan of the counts.  Used when a large share of the
+// expansions have mids (every lane then has one to write).
+template <int S>
+__global__ __launch_bounds__(kBlock) void k_cycle_emit_dense(CycArgs a, const unsigned char* __restrict__ cnt8,
+                                                             const i64* __restrict__ tile_off, i64 base, i64 budget,
+                                                             int* __restrict__ t1, int* __restrict__ t2,
+                                                             int* __restrict__ t3) {
+    __shared__ i64 ws[kWaves];
+    // plain block order: tiles past the budget exit at once, and keeping the
+    // working tiles spread over all XCDs balances them
+    const unsigned blk = blockIdx.x;
+    const bool packed = a.nslot != nullptr;
+    if (tile_off[blk + 1] == tile_off[blk]) return;  // uniform: the tile writes nothing
+    const i64 tstart = (packed ? 0 : base) + tile_off[blk];
+    if (!packed && tstart >= budget) return;  // uniform per block
+    const i64 x = (i64)blk * kCycTile + threadIdx.x;
+    int c = x < a.X ? (int)cnt8[x] : 0;
+    {
+        // a saturated count (255): count again (by the wave when both lists
+        // are long)
+        int v1 = 0, v2 = 0, n1 = 0, n2 = 0, qa = 0, qb = 0;
+        bool wave = false;
+        if (c == 255) {
+            resolve<S>(a, x, S == 3 ? 0 : a.tile_rep[blk], v1, v2, n1, n2, qa, qb);
+            int sb, se, lb, le;
+            short_long(a.off, qa, qb, sb, se, lb, le);
+            if (se - sb <= kProbe) c = probe_count(a.adj, sb, se, lb, le);
+            else wave = true;
+        }
+        for (unsigned long long hm = __ballot(wave); hm; hm &= hm - 1) {
+            const int h = __builtin_ctzll(hm);
+            const int s = wave_count_common(a.off, a.adj, __builtin_amdgcn_readlane(qa, h),
+                                            __builtin_amdgcn_readlane(qb, h));
+            if (lane_id() == h) c = s;
+        }
+    }
+    i64 tot;
+    const i64 pos = tstart + block_excl_scan((i64)c * (S - 2), ws, &tot);
+    emit_item<S>(a, c > 0, x, pos, budget, t1, t2, t3);
+}
+
+// Sparse emit: one wave per block walks a contiguous range of count tiles in
 // order.  Each tile's 256 counts are read four per lane (expansion order is
 // lane-major), a wave scan turns them into write positions (tile start +
 // the counts before), and the expansions with mids are queued in LDS; every
@@ -1068,6 +1110,24 @@ __global__ __launch_bounds__(64) void k_cycle_emit(CycArgs a, const unsigned cha
     __syncthreads();
     const bool have = lane < qn;
     emit_item<S>(a, have, have ? qx[lane] : 0, have ? qp[lane] : 0, budget, t1, t2, t3);
+}
+
+// The emit over `ntiles` count tiles with W attempted writes in all: the
+// per-expansion form when at least one expansion in kEmitDenseShare has a
+// mid (W counts S - 2 writes per mid), else the queued form.
+constexpr i64 kEmitDenseShare = 4;
+template <int S>
+static void launch_emit(Ctx& ctx, const CycArgs& a, const unsigned char* cnt8, const i64* tile_off, i64 ntiles,
+                        i64 W, i64 base, i64 budget, int* t1, int* t2, int* t3) {
+    if (W * kEmitDenseShare >= (i64)(S - 2) * a.X) {
+        k_cycle_emit_dense<S><<<(unsigned)ntiles, kBlock, 0, ctx.stream>>>(a, cnt8, tile_off, base, budget, t1, t2,
+                                                                           t3);
+    } else {
+        const i64 nb = std::min<i64>(ntiles, kEmitWaves), per = (ntiles + nb - 1) / nb;
+        k_cycle_emit<S><<<(unsigned)((ntiles + per - 1) / per), 64, 0, ctx.stream>>>(a, cnt8, tile_off, ntiles, per,
+                                                                                     base, budget, t1, t2, t3);
+    }
+    RAMA_LAUNCHED();
 }
 
 // tile_rep[t] = rep edge containing the first expansion of tile t.
@@ -1299,10 +1359,8 @@ static i64 run_stage(Ctx& ctx, const Csr& G, const int* rv1, const int* rv2, i64
     const i64 W = ctx.read1(tile.get() + tiles_used);
     if (W > 0 && base < budget) {
         prof::Scope ps(ctx, S == 3 ? "cycles3_emit" : (S == 4 ? "cycles4_emit" : "cycles5_emit"), 0.0);
-        const i64 nb = std::min<i64>(tiles_used, kEmitWaves), per = (tiles_used + nb - 1) / nb;
-        k_cycle_emit<S><<<(unsigned)((tiles_used + per - 1) / per), 64, 0, ctx.stream>>>(
-            a, cnt8.get(), tile.get(), tiles_used, per, base, budget, T.t1.get(), T.t2.get(), T.t3.get());
-        RAMA_LAUNCHED();
+        launch_emit<S>(ctx, a, cnt8.get(), tile.get(), tiles_used, W, base, budget, T.t1.get(), T.t2.get(),
+                       T.t3.get());
     }
     return W;
 }
@@ -1385,10 +1443,7 @@ void run_stage_packed(Ctx& ctx, const Csr& G, const int* rv1, const int* rv2, i6
     i64 W = 0;
     for (int q = 0; q < Q; ++q) W += h_total[q];
     if (W > 0) {
-        const i64 nb = std::min<i64>(ntiles, kEmitWaves), per = (ntiles + nb - 1) / nb;
-        k_cycle_emit<S><<<(unsigned)((ntiles + per - 1) / per), 64, 0, ctx.stream>>>(a, cnt8.get(), tile.get(), ntiles,
-                                                                                     per, 0, 0, t1, t2, t3);
-        RAMA_LAUNCHED();
+        launch_emit<S>(ctx, a, cnt8.get(), tile.get(), ntiles, W, 0, 0, t1, t2, t3);
     }
 }
 template void run_stage_packed<3>(Ctx&, const Csr&, const int*, const int*, i64, const int*, int, const i64*,

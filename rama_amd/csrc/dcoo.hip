@@ -106,21 +106,6 @@ DCoo dcoo_from_sorted(Ctx& ctx, DBuf<int>&& row, DBuf<int>&& col, DBuf<float>&& 
     return A;
 }
 
-DCoo dcoo_copy(Ctx& ctx, const DCoo& A) {
-    DCoo B;
-    B.nnz = A.nnz;
-    B.rows = A.rows;
-    B.cols = A.cols;
-    B.max_node = A.max_node;
-    B.row = ctx.buf<int>(A.nnz);
-    B.col = ctx.buf<int>(A.nnz);
-    B.val = ctx.buf<float>(A.nnz);
-    ctx.d2d(B.row.get(), A.row.get(), A.nnz);
-    ctx.d2d(B.col.get(), A.col.get(), A.nnz);
-    ctx.d2d(B.val.get(), A.val.get(), A.nnz);
-    return B;
-}
-
 // Per-segment reduction of the sorted contraction keys (packed instances,
 // which need the segments for their per-instance statistics): one thread
 // per (row, col) segment sums its costs in double in sorted order and

@@ -61,7 +61,6 @@ Options to_options(const rama_options_c& o);
 // ---- dcoo.hip
 DCoo dcoo_from_edges(Ctx& ctx, const int* row_in, const int* col_in, const float* val_in, i64 m);
 DCoo dcoo_from_sorted(Ctx& ctx, DBuf<int>&& row, DBuf<int>&& col, DBuf<float>&& val, i64 m);
-DCoo dcoo_copy(Ctx& ctx, const DCoo& A);
 DCoo contract(Ctx& ctx, const DCoo& A, const int* node_map, int max_label, bool* bad, double* diag_sum);
 // `known_count`: number of edges passing the filter if the caller knows it (-1: count here).
 Csr symmetric_csr(Ctx& ctx, const int* row, const int* col, const float* val, i64 m, int n_nodes,
