@@ -16,6 +16,7 @@
 // mids and writes at enumeration position (rep edge, expansion, mid).  The
 // triangle budget therefore keeps exactly the first `budget` writes.
 #include <cstdlib>
+#include <cstring>
 
 #include "rama_internal.hpp"
 #include "tuning.hpp"
@@ -54,7 +55,27 @@ struct CycArgs {
     unsigned long long* stats = nullptr;
     // 5-cycle count: node ids fit the 2-hop tables' keys (below kT5MaxNode)
     int tables = 0;
+    // 4-cycle count: wave-wide counting of long lists allowed
+    int wide4 = 1;
 };
+
+// Diagnostic switches (tests force each path on the same inputs; every
+// path gives the same counts and triangles):
+//   RAMA_CYCLE_TABLES=0   no 2-hop tables (5-cycles) and no wave-wide sets
+//                         (4-cycles): every expansion counted by its thread
+//   RAMA_EMIT_FORM=dense | queue   the emit form, instead of the choice by
+//                         the share of expansions with mids
+static bool cycle_tables_on() {
+    const char* e = std::getenv("RAMA_CYCLE_TABLES");
+    return !(e && e[0] == '0' && e[1] == 0);
+}
+static int emit_form_forced() {  // 0: choose, 1: dense, 2: queue
+    const char* e = std::getenv("RAMA_EMIT_FORM");
+    if (!e) return 0;
+    if (std::strcmp(e, "dense") == 0) return 1;
+    if (std::strcmp(e, "queue") == 0) return 2;
+    return 0;
+}
 
 // Adds the block's attempted writes to its counter line's [0] and raises
 // its [1] to the end of its expansions.  The counters are spread over
@@ -434,14 +455,17 @@ __global__ __launch_bounds__(kBlock) void k_cycle_count4(CycArgs a, unsigned cha
         x0 = a.xoff[r];
         xend = x0 + d1;
     }
-    const bool wide = on && (d1 >= kWideRep || n2 > kRegList);
+    const bool wide = on && a.wide4 && (d1 >= kWideRep || n2 > kRegList);
     if (on && !wide) {
         int rb[kRegList];
-        int rmax = -1;
+        const bool regs = n2 <= kRegList;
+        int rmax = -1, kw = kRegList;
         bool rdup = false;
-        load_reg_list(a.adj, b2, n2, rb);
-        reg_list_props(rb, n2, rmax, rdup);
-        const int kw = reg_width(n2);
+        if (regs) {
+            load_reg_list(a.adj, b2, n2, rb);
+            reg_list_props(rb, n2, rmax, rdup);
+            kw = reg_width(n2);
+        }
         TileAcc acc;
         // the next n1's list bounds are loaded while this one is walked
         int ia = 0, ie = 0;
@@ -457,7 +481,8 @@ __global__ __launch_bounds__(kBlock) void k_cycle_count4(CycArgs a, unsigned cha
                 na = a.off[nn];
                 ne = a.off[nn + 1];
             }
-            const int c = count_vs_regs_fast(a.adj, ia, ie, rb, rmax, rdup, kw);
+            const int c = regs ? count_vs_regs_fast(a.adj, ia, ie, rb, rmax, rdup, kw)
+                               : lane_count_common(a.adj, ia, ie, b2, n2);
             cnt8[x0 + l] = (unsigned char)(c < 255 ? c : 255);
             acc.add(tile_tot, x0 + l, 2 * (i64)c);
             mine += 2 * (i64)c;
@@ -1119,7 +1144,8 @@ constexpr i64 kEmitDenseShare = 4;
 template <int S>
 static void launch_emit(Ctx& ctx, const CycArgs& a, const unsigned char* cnt8, const i64* tile_off, i64 ntiles,
                         i64 W, i64 base, i64 budget, int* t1, int* t2, int* t3) {
-    if (W * kEmitDenseShare >= (i64)(S - 2) * a.X) {
+    const int forced = emit_form_forced();
+    if (forced == 1 || (forced == 0 && W * kEmitDenseShare >= (i64)(S - 2) * a.X)) {
         k_cycle_emit_dense<S><<<(unsigned)ntiles, kBlock, 0, ctx.stream>>>(a, cnt8, tile_off, base, budget, t1, t2,
                                                                            t3);
     } else {
@@ -1231,7 +1257,8 @@ static i64 run_stage(Ctx& ctx, const Csr& G, const int* rv1, const int* rv2, i64
         RAMA_LAUNCHED();
     }
     CycArgs a{OFF, G.adj.get(), rv1, rv2, S > 3 ? xoff.get() : nullptr, S > 3 ? tile_rep.get() : nullptr, R, X};
-    a.tables = G.n < kT5MaxNode ? 1 : 0;
+    a.tables = G.n < kT5MaxNode && cycle_tables_on() ? 1 : 0;
+    a.wide4 = cycle_tables_on() ? 1 : 0;
     DBuf<unsigned char> cnt8 = ctx.buf<unsigned char>(X);
     DBuf<i64> tile = ctx.buf<i64>(ntiles + 1);
     RAMA_REQUIRE(ntiles < ((i64)1 << 31), "too many cycle expansions");
@@ -1404,7 +1431,8 @@ void run_stage_packed(Ctx& ctx, const Csr& G, const int* rv1, const int* rv2, i6
     i64* TOT = slot.get();
     ctx.zero(TOT, Q);
     CycArgs a{OFF, G.adj.get(), rv1, rv2, S > 3 ? xoff.get() : nullptr, S > 3 ? tile_rep.get() : nullptr, R, X};
-    a.tables = G.n < kT5MaxNode ? 1 : 0;
+    a.tables = G.n < kT5MaxNode && cycle_tables_on() ? 1 : 0;
+    a.wide4 = cycle_tables_on() ? 1 : 0;
     a.nslot = nslot;
     a.slot_total = TOT;
     a.slot_first = TOT + Q;
