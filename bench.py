@@ -24,6 +24,9 @@ Also reported on the same JSON line:
                 loop's measured capacity: solves/s and p50/p99 latency of the
                 serving loop (16 lanes, 200 GB HBM budget) against the plain
                 one-request-at-a-time loop, and whether results are identical;
+  configs       the same grid in D mode (lower bound only) and the 192^3
+                3D grid in PD+ (21,123,072 edges), each through the same
+                device entry: ms per solve, edges/s, objective, bound;
   cpu_baseline  the CPU restatement (oracle, 1 thread) timed on a bounded
                 512x1024 crop of the same generator, plus the 64x64 grid
                 workload in full.
@@ -70,7 +73,48 @@ def parse():
     ap.add_argument("--kernels", type=int, default=8, help="profiled kernels listed on the line")
     ap.add_argument("--no-batch", action="store_true", help="plain mode: leave out the batched block")
     ap.add_argument("--no-serve", action="store_true", help="plain mode: leave out the serving block")
+    ap.add_argument("--no-configs", action="store_true",
+                    help="plain mode: leave out the 3D PD+ grid and the D-mode blocks")
     return ap.parse_args()
+
+
+def config_block(name, gen, preset, steps=2):
+    """One more single-instance configuration, solved through the device
+    entry like the main line (inputs resident, one warm-up solve, `steps`
+    timed solves): ms per solve, edges/s, objective, bound, clusters."""
+    import torch
+
+    from rama_amd import _capi
+
+    i, j, c, _ = gen()
+    E = len(c)
+    dev = torch.device("cuda", 0)
+    di, dj, dc = (torch.from_numpy(x).to(dev) for x in (i, j, c))
+    labels = torch.empty(int(max(i.max(), j.max())) + 2, dtype=torch.int32, device=dev)
+    stream = torch.cuda.current_stream().cuda_stream
+    opts = _capi.options(preset, verbose=0)
+
+    def solve():
+        return _capi.solve_device(di.data_ptr(), dj.data_ptr(), dc.data_ptr(), E, labels.data_ptr(), labels.numel(),
+                                  opts, device=0, stream=stream)
+
+    solve()
+    torch.cuda.synchronize()
+    t = time.perf_counter()
+    for _ in range(steps):
+        n_lab, lb, _, _ = solve()
+    torch.cuda.synchronize()
+    dt = (time.perf_counter() - t) / steps
+    out = {"workload": name, "mode": preset, "edges": E, "steps": steps, "warmup": 1, "ms_per_solve": dt * 1e3,
+           "edges_per_s": E / dt, "lower_bound": lb}
+    if n_lab > 0:
+        lab = labels[:n_lab].cpu().numpy()
+        out["objective"] = _capi.objective(lab, i, j, c)
+        out["clusters"] = int(len(np.unique(lab)))
+    del di, dj, dc, labels
+    torch.cuda.empty_cache()
+    _capi.lib().rama_release_cached(0)
+    return out
 
 
 def cpu_baseline():
@@ -484,6 +528,15 @@ def main():
     del di, dj, dc, labels
     torch.cuda.empty_cache()
     _capi.lib().rama_release_cached(0)
+    if not a.no_configs:
+        out["configs"] = [
+            config_block("grid2d_voronoi 1024x2048 + offsets (9,0),(0,9),(27,0),(0,27); 2000 seeds; D (lower bound only)",
+                         lambda: instances.grid2d_voronoi(H=1024, W=2048, seed=0), "D", steps=3),
+            config_block("grid3d_voronoi 192^3, 6-neighbourhood; 20000 seeds; PD+",
+                         lambda: instances.grid3d_voronoi(D=192, seed=0), "PD+", steps=2),
+        ]
+        progress("configs: D %.2f ms, 3D PD+ %.2f ms" % (out["configs"][0]["ms_per_solve"],
+                                                         out["configs"][1]["ms_per_solve"]))
     if not a.no_batch:
         out["batched"] = batched_block(512, a.workers)
         progress("batched: %.1f solves/s vs %.1f one at a time" % (out["batched"]["batched_solves_per_s"],
