@@ -1058,7 +1058,7 @@ __global__ __launch_bounds__(kBlock) void k_cycle_emit_dense(CycArgs a, const un
 // Most expansions have no mid, so lanes stay busy instead of idling beside
 // the few that write.  Saturated counts (255) are counted again first.
 constexpr int kEmitQueue = 320;  // at most 63 waiting + 256 from one tile
-constexpr i64 kEmitWaves = 6144;  // emit blocks (one wave each) at most
+constexpr i64 kEmitTilesPerWave = 8;  // count tiles a queued-emit wave walks (latency-bound loop: keep it short)
 
 template <int S>
 __global__ __launch_bounds__(64) void k_cycle_emit(CycArgs a, const unsigned char* __restrict__ cnt8,
@@ -1149,7 +1149,7 @@ static void launch_emit(Ctx& ctx, const CycArgs& a, const unsigned char* cnt8, c
         k_cycle_emit_dense<S><<<(unsigned)ntiles, kBlock, 0, ctx.stream>>>(a, cnt8, tile_off, base, budget, t1, t2,
                                                                            t3);
     } else {
-        const i64 nb = std::min<i64>(ntiles, kEmitWaves), per = (ntiles + nb - 1) / nb;
+        const i64 per = kEmitTilesPerWave;
         k_cycle_emit<S><<<(unsigned)((ntiles + per - 1) / per), 64, 0, ctx.stream>>>(a, cnt8, tile_off, ntiles, per,
                                                                                      base, budget, t1, t2, t3);
     }
