@@ -529,24 +529,37 @@ __global__ __launch_bounds__(kBlock) void k_cycle_count4(CycArgs a, unsigned cha
             int c = 0;
             bool merge = !use_set;
             if (use_set) {
+                // four entries per round of loads (past the end reads as
+                // above vmax), walked in order up to the first above vmax
                 int prev = -1;
-                for (int k = ia; k < ie; ++k) {
-                    const int x = a.adj[k];
-                    if (x > vmax) break;
-                    if (x == prev) {  // N(n1) repeats a value: the merge walk counts it
-                        merge = true;
-                        break;
-                    }
-                    prev = x;
-                    int hs = set_hash(x);
-                    while (true) {
-                        const int key = wset[w][hs];
-                        if (key == x) {
-                            ++c;
-                            break;
+                bool stop = false;
+                for (int k = ia; k < ie && !stop; k += 4) {
+                    int xs[4];
+#pragma unroll
+                    for (int u = 0; u < 4; ++u) xs[u] = k + u < ie ? a.adj[k + u] : 0x7fffffff;
+#pragma unroll
+                    for (int u = 0; u < 4; ++u) {
+                        const int x = xs[u];
+                        if (stop || x > vmax) {
+                            stop = true;
+                            continue;
                         }
-                        if (key == -1) break;
-                        hs = (hs + 1) & (kSetSlots - 1);
+                        if (x == prev) {  // N(n1) repeats a value: the merge walk counts it
+                            merge = true;
+                            stop = true;
+                            continue;
+                        }
+                        prev = x;
+                        int hs = set_hash(x);
+                        while (true) {
+                            const int key = wset[w][hs];
+                            if (key == x) {
+                                ++c;
+                                break;
+                            }
+                            if (key == -1) break;
+                            hs = (hs + 1) & (kSetSlots - 1);
+                        }
                     }
                 }
             }
@@ -814,30 +827,38 @@ __global__ __launch_bounds__(kBlock) void k_cycle_count5(CycArgs a, const i64* _
             // per-l1 counts of N(n2)'s mids, bit-sliced over 8 planes
             unsigned p[8] = {0u, 0u, 0u, 0u, 0u, 0u, 0u, 0u};
             int prev = -1;
-            for (int q = bq; q < bq + nq; ++q) {
-                const int m = a.adj[q];
-                if (m == prev) {  // N(n2) repeats a value: the min(a, b) rule needs the direct form
-                    direct = true;
-                    break;
-                }
-                prev = m;
-                const int want = m * kT5Tables + run;
-                int h = t5_hash(want);
-                unsigned mk = 0u;
-                while (true) {
-                    const int key = tkey[h];
-                    if (key == want) {
-                        mk = tmask[h];
-                        break;
-                    }
-                    if (key == -1) break;
-                    h = (h + 1) & (kT5Slots - 1);
-                }
+            // four entries of N(n2) per round of loads
+            for (int q0 = bq; q0 < bq + nq && !direct; q0 += 4) {
+                int ms[4];
 #pragma unroll
-                for (int b = 0; b < 8; ++b) {
-                    const unsigned carry = p[b] & mk;
-                    p[b] ^= mk;
-                    mk = carry;
+                for (int u = 0; u < 4; ++u) ms[u] = q0 + u < bq + nq ? a.adj[q0 + u] : -1;
+#pragma unroll
+                for (int u = 0; u < 4; ++u) {
+                    if (direct || q0 + u >= bq + nq) continue;
+                    const int m = ms[u];
+                    if (m == prev) {  // N(n2) repeats a value: the min(a, b) rule needs the direct form
+                        direct = true;
+                        continue;
+                    }
+                    prev = m;
+                    const int want = m * kT5Tables + run;
+                    int h = t5_hash(want);
+                    unsigned mk = 0u;
+                    while (true) {
+                        const int key = tkey[h];
+                        if (key == want) {
+                            mk = tmask[h];
+                            break;
+                        }
+                        if (key == -1) break;
+                        h = (h + 1) & (kT5Slots - 1);
+                    }
+#pragma unroll
+                    for (int b = 0; b < 8; ++b) {
+                        const unsigned carry = p[b] & mk;
+                        p[b] ^= mk;
+                        mk = carry;
+                    }
                 }
             }
             if (!direct) {
