@@ -438,7 +438,7 @@ __device__ __forceinline__ int set_hash(int m) { return (int)(((unsigned)m * 265
 
 __global__ __launch_bounds__(kBlock) void k_cycle_count4(CycArgs a, unsigned char* __restrict__ cnt8,
                                                          i64* __restrict__ tile_tot) {
-    __shared__ int wset[kWaves][kSetSlots];
+    __shared__ int wset[kWaves][4 * kSetSlots];
     const int w = threadIdx.x >> 6;
     const i64 r = (i64)(a.blk0 + xcd_block()) * kBlock + threadIdx.x;
     const bool on = r < a.R;
@@ -491,39 +491,52 @@ __global__ __launch_bounds__(kBlock) void k_cycle_count4(CycArgs a, unsigned cha
         }
         acc.flush(tile_tot);
     }
-    // wide rep edges, one at a time by the whole wave; the attempted writes
-    // are credited to the rep's own lane (its instance, its block total)
-    const int lane = lane_id();
-    for (unsigned long long hm = __ballot(wide); hm; hm &= hm - 1) {
-        const int h = __builtin_ctzll(hm);
-        const int hb1 = __builtin_amdgcn_readlane(b1, h), hd1 = __builtin_amdgcn_readlane(d1, h);
-        const int hb2 = __builtin_amdgcn_readlane(b2, h), hn2 = __builtin_amdgcn_readlane(n2, h);
-        const i64 hx0 = readlane64(x0, h);
-        // N(v2) into the wave's set (a repeated value, or too long a list:
+    // wide rep edges: the wave works as four groups of 16 lanes, group g
+    // taking the wide rep edges of lanes 16g .. 16g + 15 one after another
+    // (four rep edges in flight per wave); the attempted writes are credited
+    // to the rep's own lane (its instance, its block total)
+    const int lane = lane_id(), g = lane >> 4, gl = lane & 15;
+    const unsigned long long wm = __ballot(wide);
+    unsigned gm = (unsigned)((wm >> (16 * g)) & 0xffffull);  // this group's wide lanes
+    int niter = 0;
+#pragma unroll
+    for (int q = 0; q < 4; ++q) niter = max(niter, __popcll((wm >> (16 * q)) & 0xffffull));
+    int* gset = &wset[w][g * kSetSlots];
+    const unsigned long long gmask = 0xffffull << (16 * g);
+    for (int it = 0; it < niter; ++it) {
+        const bool gact = gm != 0u;  // uniform within the group
+        const int h = gact ? 16 * g + __builtin_ctz(gm) : lane;
+        if (gact) gm &= gm - 1u;
+        const int hb1 = __shfl(b1, h, 64), hd1 = gact ? __shfl(d1, h, 64) : 0;
+        const int hb2 = __shfl(b2, h, 64), hn2 = gact ? __shfl(n2, h, 64) : 0;
+        const i64 hx0 = __shfl(x0, h, 64);
+        // N(v2) into the group's set (a repeated value, or too long a list:
         // merge walks instead)
-        bool use_set = hn2 <= kSetMax;
+        bool use_set = gact && hn2 <= kSetMax;
         int vmax = -1;
+        bool dup = false;
         if (use_set) {
-            for (int k = lane; k < kSetSlots; k += 64) wset[w][k] = -1;
-            __builtin_amdgcn_wave_barrier();
-            bool dup = false;
-            for (int k = lane; k < hn2; k += 64) {
+            for (int k = gl; k < kSetSlots; k += 16) gset[k] = -1;
+        }
+        __builtin_amdgcn_wave_barrier();
+        if (use_set) {
+            for (int k = gl; k < hn2; k += 16) {
                 const int m = a.adj[hb2 + k];
                 dup |= k > 0 && a.adj[hb2 + k - 1] == m;
                 int hs = set_hash(m);
                 while (true) {
-                    const int old = atomicCAS(&wset[w][hs], -1, m);
+                    const int old = atomicCAS(&gset[hs], -1, m);
                     if (old == -1 || old == m) break;
                     hs = (hs + 1) & (kSetSlots - 1);
                 }
             }
-            use_set = __ballot(dup) == 0ull;
             vmax = hn2 > 0 ? a.adj[hb2 + hn2 - 1] : -1;
-            __builtin_amdgcn_wave_barrier();
         }
+        use_set = use_set && (__ballot(dup) & gmask) == 0ull;
+        __builtin_amdgcn_wave_barrier();
         TileAcc acc;
         i64 part = 0;
-        for (int l = lane; l < hd1; l += 64) {
+        for (int l = gl; l < hd1; l += 16) {
             const int n1 = a.adj[hb1 + l];
             const int ia = a.off[n1], ie = a.off[n1 + 1];
             int c = 0;
@@ -552,7 +565,7 @@ __global__ __launch_bounds__(kBlock) void k_cycle_count4(CycArgs a, unsigned cha
                         prev = x;
                         int hs = set_hash(x);
                         while (true) {
-                            const int key = wset[w][hs];
+                            const int key = gset[hs];
                             if (key == x) {
                                 ++c;
                                 break;
@@ -570,8 +583,8 @@ __global__ __launch_bounds__(kBlock) void k_cycle_count4(CycArgs a, unsigned cha
         }
         acc.flush(tile_tot);
 #pragma unroll
-        for (int d = 32; d >= 1; d >>= 1) part += __shfl_xor(part, d, 64);
-        if (lane == h) mine += part;
+        for (int d = 8; d >= 1; d >>= 1) part += __shfl_xor(part, d, 64);  // within the group
+        if (gact && lane == h) mine += part;
         __builtin_amdgcn_wave_barrier();
     }
     if (a.nslot) wave_add_keyed(a.slot_total, on ? a.nslot[v1] : 0, mine, on);
