@@ -305,6 +305,13 @@ const char* rama_tuning_source(void);
  * sort_path_*) and threads per tile (512 or 1024): what the tuner measures. */
 int rama_tune_time_sort(int64_t n, int32_t key_bytes, int32_t bits, int32_t path, int32_t threads, int32_t reps,
                         double* ms_out);
+/* Device-to-device copy rate in GB/s (bytes read + bytes written) of a
+ * streaming copy kernel (16-byte non-temporal loads, grid-stride) over two
+ * buffers of `bytes`: the fastest of eight launch shapes (4, 8 or 16
+ * blocks per CU, or one pass over the whole array; 4 or 8 loads in flight
+ * per thread), each the mean of `reps` timed copies.  The practical HBM rate beside the nominal peak.  No counterpart
+ * in the reference. */
+int rama_stream_copy_gbps(int64_t bytes, int32_t reps, double* gbps_out);
 
 /* ------------------------------------------------------------------------
  * Device memory cache (no counterpart in the reference, whose thrust

@@ -84,6 +84,7 @@ SIGNATURES = {
     "rama_tuning_get": ([ctypes.c_char_p, I64], I64),
     "rama_tuning_source": ([], ctypes.c_char_p),
     "rama_tune_time_sort": ([I64, I32, I32, I32, I32, I32, F64P], I32),
+    "rama_stream_copy_gbps": ([I64, I32, F64P], I32),
     "rama_server_destroy": ([VP], None),
     "rama_profile_enable": ([I32], None),
     "rama_profile_only": ([ctypes.c_char_p], None),

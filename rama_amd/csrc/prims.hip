@@ -116,8 +116,10 @@ __global__ __launch_bounds__(NT) void k_radix_scatter(const K* __restrict__ kin,
                                                       K* __restrict__ kout, u32* __restrict__ vout, i64 n, int shift,
                                                       int bits, const int* __restrict__ offs, int ntiles) {
     RAMA_SC_SHAPE(NT)
+    // one reorder buffer, used for the keys and then for the values, keeps
+    // the tile's LDS at about 50 KB: three tiles per CU instead of two
     __shared__ K sk[kRsTile];
-    __shared__ u32 sv[kRsTile];
+    u32* const sv = reinterpret_cast<u32*>(sk);
     __shared__ int wcnt[kScWaves][256];
     __shared__ int dstart[256];
     __shared__ int gbase[256];
@@ -168,21 +170,37 @@ __global__ __launch_bounds__(NT) void k_radix_scatter(const K* __restrict__ kin,
     }
     __syncthreads();
     scan256(dstart);
+    int pos[kScItems];
 #pragma unroll
     for (int k = 0; k < kScItems; ++k) {
         const u32 d = (u32)(key[k] >> shift) & mask;
-        const int pos = dstart[d] + wcnt[w][d] + rnk[k];
-        sk[pos] = key[k];
-        sv[pos] = val[k];
+        pos[k] = dstart[d] + wcnt[w][d] + rnk[k];
+        sk[pos[k]] = key[k];
     }
     __syncthreads();
     const int nvalid = (int)((n - tile0) < kRsTile ? (n - tile0) : kRsTile);
-    for (int p = tid; p < nvalid; p += kScThreads) {
-        const K k = sk[p];
-        const u32 d = (u32)(k >> shift) & mask;
-        const i64 o = (i64)gbase[d] + (p - dstart[d]);
-        kout[o] = k;
-        vout[o] = sv[p];
+    // the keys in digit runs; each element's output place is kept for its value
+    i64 outp[(kRsTile + kScThreads - 1) / kScThreads];
+#pragma unroll
+    for (int q = 0; q < (kRsTile + kScThreads - 1) / kScThreads; ++q) {
+        const int p = tid + q * kScThreads;
+        outp[q] = -1;
+        if (p < nvalid) {
+            const K k = sk[p];
+            const u32 d = (u32)(k >> shift) & mask;
+            const i64 o = (i64)gbase[d] + (p - dstart[d]);
+            kout[o] = k;
+            outp[q] = o;
+        }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int k = 0; k < kScItems; ++k) sv[pos[k]] = val[k];
+    __syncthreads();
+#pragma unroll
+    for (int q = 0; q < (kRsTile + kScThreads - 1) / kScThreads; ++q) {
+        const int p = tid + q * kScThreads;
+        if (p < nvalid) vout[outp[q]] = sv[p];
     }
 }
 

@@ -176,6 +176,71 @@ __global__ void k_tune_fill(K* keys, u32* vals, i64 n, int bits, u32 rep) {
     }
 }
 
+// Streaming copy: 16-byte non-temporal loads, U of them in flight per
+// thread, grid-stride over whole vectors.  time_copy tries a few grid sizes
+// and unroll depths and keeps the fastest: the practical HBM rate the
+// roofline fractions are read against.
+typedef unsigned int v4u __attribute__((ext_vector_type(4)));
+template <int U>
+__global__ __launch_bounds__(kBlock) void k_stream_copy(const v4u* __restrict__ src, v4u* __restrict__ dst, i64 n16) {
+    const i64 stride = (i64)gridDim.x * kBlock;
+    i64 i = (i64)blockIdx.x * kBlock + threadIdx.x;
+    for (; i + (U - 1) * stride < n16; i += U * stride) {
+        v4u v[U];
+#pragma unroll
+        for (int u = 0; u < U; ++u) v[u] = __builtin_nontemporal_load(&src[i + u * stride]);
+#pragma unroll
+        for (int u = 0; u < U; ++u) dst[i + u * stride] = v[u];
+    }
+    for (; i < n16; i += stride) dst[i] = __builtin_nontemporal_load(&src[i]);
+}
+
+double time_copy(i64 bytes, int reps) {
+    Ctx ctx(0, nullptr, false);
+    const i64 n16 = bytes / 16;
+    DBuf<v4u> a = ctx.buf<v4u>(n16), b = ctx.buf<v4u>(n16);
+    int cus = 0;
+    RAMA_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, ctx.device));
+    hipEvent_t e0, e1;
+    RAMA_HIP(hipEventCreate(&e0));
+    RAMA_HIP(hipEventCreate(&e1));
+    double best = 0.0;
+    try {
+        RAMA_HIP(hipMemsetAsync(a.get(), 1, (size_t)n16 * 16, ctx.stream));
+        for (int shape = 0; shape < 8; ++shape) {
+            // shapes 0-5: bounded grids (4, 8, 16 blocks per CU); 6-7: one
+            // pass, every thread copying U vectors once
+            const int per_cu = (shape % 3 == 0) ? 4 : (shape % 3 == 1 ? 8 : 16);
+            const bool deep = shape >= 6 ? shape == 7 : shape >= 3;
+            const i64 full = (n16 + (i64)kBlock * (deep ? 8 : 4) - 1) / ((i64)kBlock * (deep ? 8 : 4));
+            const unsigned grid = shape >= 6 ? (unsigned)std::min<i64>(std::max<i64>(full, 1), (i64)1 << 30)
+                                             : (unsigned)std::max(1, per_cu * cus);
+            double ms_total = 0.0;
+            for (int r = -1; r < reps; ++r) {  // one untimed run first
+                RAMA_HIP(hipEventRecord(e0, ctx.stream));
+                if (deep)
+                    k_stream_copy<8><<<grid, kBlock, 0, ctx.stream>>>(a.get(), b.get(), n16);
+                else
+                    k_stream_copy<4><<<grid, kBlock, 0, ctx.stream>>>(a.get(), b.get(), n16);
+                RAMA_LAUNCHED();
+                RAMA_HIP(hipEventRecord(e1, ctx.stream));
+                RAMA_HIP(hipEventSynchronize(e1));
+                float ms = 0.0f;
+                RAMA_HIP(hipEventElapsedTime(&ms, e0, e1));
+                if (r >= 0) ms_total += ms;
+            }
+            best = std::max(best, 2.0 * (double)n16 * 16.0 * reps / (ms_total * 1e-3) / 1e9);
+        }
+    } catch (...) {
+        (void)hipEventDestroy(e0);
+        (void)hipEventDestroy(e1);
+        throw;
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    return best;
+}
+
 template <class K>
 double time_sort(i64 n, int bits, int path, int threads, int reps) {
     Ctx ctx(0, nullptr, false);
@@ -287,6 +352,20 @@ int rama_tune_time_sort(int64_t n, int32_t key_bytes, int32_t bits, int32_t path
         }
         *ms_out = key_bytes == 8 ? time_sort<u64>(n, bits, path, threads, reps)
                                  : time_sort<u32>(n, bits, path, threads, reps);
+    });
+}
+
+int rama_stream_copy_gbps(int64_t bytes, int32_t reps, double* gbps_out) {
+    using namespace rama;
+    return guarded_call([&] {
+        RAMA_REQUIRE(gbps_out != nullptr, "NULL argument");
+        RAMA_REQUIRE(bytes >= 16 && reps >= 1 && reps <= 1000, "bytes or reps out of range");
+        int nd = 0;
+        if (hipGetDeviceCount(&nd) != hipSuccess || nd == 0) {
+            (void)hipGetLastError();
+            throw Error(RAMA_ERR_NO_DEVICE, "no HIP device available");
+        }
+        *gbps_out = time_copy(bytes, reps);
     });
 }
 
