@@ -75,8 +75,9 @@ inline unsigned grid_for(i64 n, int block = 256) {
 // RAMA_CACHE_BYTES (default 32 GiB per device); rama_release_cached() and
 // allocation failures empty it.
 namespace pool {
-constexpr size_t kPinnedBytes = 4096;
-void* take_pinned();
+constexpr size_t kPinnedBytes = 4096;  // read-back staging
+constexpr size_t kMirrorBytes = 256;   // device-written result slots (Ctx::mirror)
+void* take_pinned();                   // kPinnedBytes + kMirrorBytes, mapped into every device
 void give_pinned(void* p);
 void* take(int device, size_t cls);
 void give(int device, size_t cls, void* p);  // frees when over the bound
@@ -273,6 +274,7 @@ struct Ctx {
     Arena arena;
     bool verbose = false;
     void* pinned = nullptr;  // small pinned host area for scalar read-back
+    void* mirror_dev = nullptr;  // device address of the slots after it (null: allocation-free mode)
     // The last conflicted-cycle search on this context found no cycle (a
     // hint that the next dual pass leaves its graph unchanged).
     bool last_cycles_empty = false;
@@ -286,6 +288,9 @@ struct Ctx {
             own_stream = true;
         }
         pinned = pool::take_pinned();
+        void* base = nullptr;
+        RAMA_HIP(hipHostGetDevicePointer(&base, pinned, 0));
+        mirror_dev = static_cast<unsigned char*>(base) + pool::kPinnedBytes;
     }
     // Allocation-free context: device memory comes only from the fixed
     // region [region, region + region_bytes), work runs on the caller's
@@ -471,6 +476,40 @@ struct Ctx {
         std::memcpy(slot, h, bytes);
         RAMA_HIP(hipMemcpyAsync(d, slot, bytes, hipMemcpyHostToDevice, stream));
         up_off += need;
+    }
+
+    // Result slots in pinned host memory mapped into the device's address
+    // space.  A kernel whose result the host reads next stores it here as
+    // well as in device memory, so the read-back is a wait with no copy
+    // behind it (device stores to this memory are visible to the host once
+    // the stream has been synchronised).  Null in the allocation-free mode,
+    // whose host scalars are ordinary memory: callers then read the device
+    // copy.  One set of slots per context: a result is read before the next
+    // kernel that stores one is queued.
+    template <class T>
+    T* mirror() {
+        static_assert(sizeof(T) <= pool::kMirrorBytes, "mirror slot too small");
+        static const bool off = [] {  // RAMA_READ_MIRROR=0: read every result back by a copy
+            const char* e = std::getenv("RAMA_READ_MIRROR");
+            return e && std::strcmp(e, "0") == 0;
+        }();
+        return off ? nullptr : static_cast<T*>(mirror_dev);
+    }
+    // Host address of the mirror slots (for a value the host sets itself
+    // when no kernel runs to store it).
+    void* mirror_host() { return static_cast<unsigned char*>(pinned) + pool::kPinnedBytes; }
+    // The n values a kernel stored at m = mirror<T>() (synchronises), or,
+    // when m is null, the n device values at d.
+    template <class T>
+    void read_mirror(const T* d, const T* m, T* h, int n = 1, const char* fn = __builtin_FUNCTION(),
+                     int line = __builtin_LINE()) {
+        if (!m) {
+            read(d, h, n, fn, line);
+            return;
+        }
+        RAMA_REQUIRE(sizeof(T) * (size_t)n <= pool::kMirrorBytes, "mirror read too large");
+        sync(fn, line);
+        std::memcpy(h, mirror_host(), sizeof(T) * n);
     }
 
     // Read `n` device scalars (contiguous) into host memory (synchronises).

@@ -1268,9 +1268,10 @@ static i64 run_stage(Ctx& ctx, const Csr& G, const int* rv1, const int* rv2, i64
     const int* OFF = G.off.get();
     DBuf<i64> xoff;  // 3-cycles: expansion = rep edge, no offsets
     if (S > 3) xoff = ctx.buf<i64>(R + 1);
+    i64* xm = ctx.mirror<i64>();
     if (S == 4) {
         scan_exclusive<i64>(ctx, [=] __device__(i64 r) { const int v = rv1[r]; return (i64)(OFF[v + 1] - OFF[v]); },
-                            xoff.get(), R);
+                            xoff.get(), R, xm);
     } else if (S == 5) {
         scan_exclusive<i64>(
             ctx,
@@ -1278,9 +1279,10 @@ static i64 run_stage(Ctx& ctx, const Csr& G, const int* rv1, const int* rv2, i64
                 const int v = rv1[r], u = rv2[r];
                 return (i64)(OFF[v + 1] - OFF[v]) * (i64)(OFF[u + 1] - OFF[u]);
             },
-            xoff.get(), R);
+            xoff.get(), R, xm);
     }
-    const i64 X = S == 3 ? R : ctx.read1(xoff.get() + R);
+    i64 X = R;
+    if (S > 3) ctx.read_mirror(xoff.get() + R, xm, &X);
     if (ctx.verbose && S > 3) std::printf("%d-cycles: number of expansions: %lld\n", S, (long long)X);
     if (X == 0) return 0;
     const i64 ntiles = (X + kCycTile - 1) / kCycTile;
@@ -1339,10 +1341,11 @@ static i64 run_stage(Ctx& ctx, const Csr& G, const int* rv1, const int* rv2, i64
         } else {
             // thread space: one thread per (rep edge, neighbour of v2)
             yoff = ctx.buf<i64>(R + 1);
+            i64* ym = ctx.mirror<i64>();
             scan_exclusive<i64>(ctx,
                                 [=] __device__(i64 r) { const int u = rv2[r]; return (i64)(OFF[u + 1] - OFF[u]); },
-                                yoff.get(), R);
-            Y = ctx.read1(yoff.get() + R);
+                                yoff.get(), R, ym);
+            ctx.read_mirror(yoff.get() + R, ym, &Y);
             if (Y > 0) {
                 const i64 ytiles = (Y + kCycTile - 1) / kCycTile;
                 RAMA_REQUIRE(ytiles < ((i64)1 << 31), "too many cycle expansions");
@@ -1416,8 +1419,10 @@ static i64 run_stage(Ctx& ctx, const Csr& G, const int* rv1, const int* rv2, i64
             if (tail > 0) ctx.zero(cnt8.get() + x_end, tail);
         }
     }
-    scan_exclusive_arr<i64>(ctx, tile.get(), tile.get(), tiles_used);
-    const i64 W = ctx.read1(tile.get() + tiles_used);
+    i64* wm = ctx.mirror<i64>();
+    scan_exclusive_arr<i64>(ctx, tile.get(), tile.get(), tiles_used, wm);
+    i64 W;
+    ctx.read_mirror(tile.get() + tiles_used, wm, &W);
     if (W > 0 && base < budget) {
         prof::Scope ps(ctx, S == 3 ? "cycles3_emit" : (S == 4 ? "cycles4_emit" : "cycles5_emit"), 0.0);
         launch_emit<S>(ctx, a, cnt8.get(), tile.get(), tiles_used, W, base, budget, T.t1.get(), T.t2.get(),
@@ -1439,9 +1444,10 @@ void run_stage_packed(Ctx& ctx, const Csr& G, const int* rv1, const int* rv2, i6
     const int* OFF = G.off.get();
     DBuf<i64> xoff;  // 3-cycles: expansion = rep edge, no offsets
     if (S > 3) xoff = ctx.buf<i64>(R + 1);
+    i64* xm = ctx.mirror<i64>();
     if (S == 4) {
         scan_exclusive<i64>(ctx, [=] __device__(i64 r) { const int v = rv1[r]; return (i64)(OFF[v + 1] - OFF[v]); },
-                            xoff.get(), R);
+                            xoff.get(), R, xm);
     } else if (S == 5) {
         scan_exclusive<i64>(
             ctx,
@@ -1449,9 +1455,10 @@ void run_stage_packed(Ctx& ctx, const Csr& G, const int* rv1, const int* rv2, i6
                 const int v = rv1[r], u = rv2[r];
                 return (i64)(OFF[v + 1] - OFF[v]) * (i64)(OFF[u + 1] - OFF[u]);
             },
-            xoff.get(), R);
+            xoff.get(), R, xm);
     }
-    const i64 X = S == 3 ? R : ctx.read1(xoff.get() + R);
+    i64 X = R;
+    if (S > 3) ctx.read_mirror(xoff.get() + R, xm, &X);
     if (X == 0) return;
     const i64 ntiles = (X + kCycTile - 1) / kCycTile;
     RAMA_REQUIRE(ntiles < ((i64)1 << 31), "too many cycle expansions");
@@ -1559,8 +1566,10 @@ i64 rearrange_triangles(Ctx& ctx, Triangles& T, i64 valid, int n_nodes) {
                 B[o] = (int)((k >> kb) & m1);
                 C[o] = (int)(k & m1);
             },
-            CNT);
-        return ctx.read1(CNT);
+            CNT, ctx.mirror<int>());
+        int h;
+        ctx.read_mirror(CNT, ctx.mirror<int>(), &h);
+        return h;
     }
     // ids too wide for one key: two stable passes, by third id and then by
     // (first, second), on the ordered triples kept in S1..S3 (-1: repeated vertex)
@@ -1604,8 +1613,10 @@ i64 rearrange_triangles(Ctx& ctx, Triangles& T, i64 valid, int n_nodes) {
             B[o] = S2[a];
             C[o] = S3[a];
         },
-        CNT);
-    return ctx.read1(CNT);
+        CNT, ctx.mirror<int>());
+    int h;
+    ctx.read_mirror(CNT, ctx.mirror<int>(), &h);
+    return h;
 }
 
 void normalize_triangles(Ctx& ctx, Triangles& T) { T.n = rearrange_triangles(ctx, T, T.n, 1 << 30); }
@@ -1628,16 +1639,18 @@ Triangles conflicted_cycles(Ctx& ctx, const DCoo& A, int max_len, float factor, 
     };
     DBuf<Sizes> sz = ctx.buf<Sizes>(1);
     Sizes* SZ = sz.get();
-    reduce_to<float>(ctx, [=] __device__(i64 e) { return W[e]; }, A.nnz, 3.402823466e38f, OpMin(), &SZ->mn);
+    Sizes* SM = ctx.mirror<Sizes>();
+    reduce_to<float>(ctx, [=] __device__(i64 e) { return W[e]; }, A.nnz, 3.402823466e38f, OpMin(), &SZ->mn,
+                     SM ? &SM->mn : nullptr);
     reduce_to<u64>(
         ctx,
         [=] __device__(i64 e) {
             const float t = SZ->mn * tol;
             return ((u64)(W[e] > -1.0f * t) << 32) | (u64)(W[e] < t);
         },
-        A.nnz, (u64)0, OpSum(), &SZ->counts);
+        A.nnz, (u64)0, OpSum(), &SZ->counts, SM ? &SM->counts : nullptr);
     Sizes h;
-    ctx.read(SZ, &h, 1);
+    ctx.read_mirror(SZ, SM, &h);
     const float thr = h.mn * tol;
     const i64 npos = (i64)(h.counts >> 32);
     const i64 R = (i64)(h.counts & 0xffffffffull);

@@ -206,7 +206,7 @@ __global__ __launch_bounds__(kBlock) void k_contract_stats(const u64* __restrict
 __global__ __launch_bounds__(kBlock) void k_contract_stats_combine(const u64* __restrict__ K, i64 n, int kb,
                                                                   const int* __restrict__ part,
                                                                   const double* __restrict__ dpart, unsigned nb,
-                                                                  int* out, double* diag) {
+                                                                  int* sc, double* diag, int* mirror) {
     __shared__ int ws[kWaves];
     __shared__ double wd[kWaves];
     int all = -1, kept = -1, bad = 0;
@@ -223,11 +223,19 @@ __global__ __launch_bounds__(kBlock) void k_contract_stats_combine(const u64* __
     bad = block_reduce(bad, OpMax(), ws);
     d = block_reduce(d, OpSum(), wd);
     if (threadIdx.x == 0) {
-        out[1] = all;
-        out[0] = kept;
-        out[2] = (int)(K[n - 1] >> kb);
-        out[3] = bad;
+        const int last = (int)(K[n - 1] >> kb);
+        sc[2] = kept;
+        sc[3] = all;
+        sc[4] = last;
+        sc[5] = bad;
         *diag = d;
+        if (mirror) {  // sc[1], the kept edge count, is final: the merge's scan ran before
+            mirror[1] = sc[1];
+            mirror[2] = kept;
+            mirror[3] = all;
+            mirror[4] = last;
+            mirror[5] = bad;
+        }
     }
 }
 
@@ -254,6 +262,7 @@ DCoo contract(Ctx& ctx, const DCoo& A, const int* node_map, int max_label, bool*
     DBuf<int> sc = ctx.buf<int>(8);
     DBuf<double> dg = ctx.buf<double>(1);
     int* SC = sc.get();
+    int* SM = ctx.mirror<int>();
     double* DG = dg.get();
     for_each(ctx, n, [=] __device__(i64 e) {
         const int a = node_map[R[e]], b = node_map[C[e]];
@@ -309,11 +318,11 @@ DCoo contract(Ctx& ctx, const DCoo& A, const int* node_map, int max_label, bool*
         prof::Scope ps(ctx, "contract_stats", 12.0 * n);
         k_contract_stats<<<nb, kBlock, 0, ctx.stream>>>(K, V, n, kb, part, dpart);
         RAMA_LAUNCHED();
-        k_contract_stats_combine<<<1, kBlock, 0, ctx.stream>>>(K, n, kb, part, dpart, nb, SC + 2, DG);
+        k_contract_stats_combine<<<1, kBlock, 0, ctx.stream>>>(K, n, kb, part, dpart, nb, SC, DG, SM);
         RAMA_LAUNCHED();
     }
     int h[8];
-    ctx.read(SC, h, 8);
+    ctx.read_mirror(SC, SM, h, 8);
     B.nnz = h[1];
     B.cols = h[3] + 1;
     B.rows = h[4] + 1;

@@ -187,11 +187,19 @@ __global__ __launch_bounds__(kBlock) void k_match(int n, float mult, int round, 
     if (threadIdx.x == 0 && tot) atomicAdd(&st->part[round][(blockIdx.x % kParts) * kLine], tot);
 }
 
-// head.newly[r] = round r's count, for the host (one wave).
-__global__ void k_match_totals(MatchLoop* st) {
+// head.newly[r] = round r's count, for the host (one wave); the whole head
+// is also stored at `mirror` when that is set.
+__global__ void k_match_totals(MatchLoop* st, MatchHead* mirror) {
     for (int r = 0; r < kMaxRounds; ++r) {
         const int x = loop_newly(st, r);
-        if (threadIdx.x == 0) st->head.newly[r] = x;
+        if (threadIdx.x == 0) {
+            st->head.newly[r] = x;
+            if (mirror) mirror->newly[r] = x;
+        }
+    }
+    if (mirror && threadIdx.x == 0) {
+        mirror->pos_sum = st->head.pos_sum;
+        mirror->pos_cnt = st->head.pos_cnt;
     }
 }
 
@@ -252,9 +260,10 @@ int matching_vertex_based(Ctx& ctx, const Csr& U, float mean_multiplier, DBuf<in
         return false;
     };
     auto read_head = [&](MatchHead& h) {
-        k_match_totals<<<1, 64, 0, ctx.stream>>>(st);
+        MatchHead* m = ctx.mirror<MatchHead>();
+        k_match_totals<<<1, 64, 0, ctx.stream>>>(st, m);
         RAMA_LAUNCHED();
-        ctx.read(&st->head, &h, 1);
+        ctx.read_mirror(&st->head, m, &h);
     };
     MatchHead h;
     bool have = false;  // h holds the final state

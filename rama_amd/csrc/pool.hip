@@ -79,7 +79,7 @@ void* take_pinned() {
         }
     }
     void* p = nullptr;
-    RAMA_HIP(hipHostMalloc(&p, kPinnedBytes, hipHostMallocDefault));
+    RAMA_HIP(hipHostMalloc(&p, kPinnedBytes + kMirrorBytes, hipHostMallocPortable | hipHostMallocMapped));
     return p;
 }
 

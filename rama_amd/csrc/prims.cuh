@@ -165,7 +165,7 @@ __global__ __launch_bounds__(kBlock) void k_scan_up(F f, i64 n, i64 chunk, T* __
 // null) receives the sum of all values.
 template <class T, class F, class Out>
 __global__ __launch_bounds__(kBlock) void k_scan_down(F f, i64 n, i64 chunk, Out out,
-                                                      const T* __restrict__ partials, T* total) {
+                                                      const T* __restrict__ partials, T* total, T* mirror) {
     __shared__ T ws[kWaves];
     __shared__ T sv[kBlock * kScanPad];
     T p = T(0);
@@ -209,7 +209,10 @@ __global__ __launch_bounds__(kBlock) void k_scan_down(F f, i64 n, i64 chunk, Out
         run += agg;
         __syncthreads();  // the next tile overwrites sv
     }
-    if (total && threadIdx.x == 0 && blockIdx.x == gridDim.x - 1) *total = run;
+    if (threadIdx.x == 0 && blockIdx.x == gridDim.x - 1) {
+        if (total) *total = run;
+        if (mirror) *mirror = run;
+    }
 }
 
 template <class T>
@@ -224,11 +227,13 @@ struct StoreExcl {
     __device__ __forceinline__ void operator()(i64 i, T run, T) const { p[i] = run; }
 };
 
-// Generic scan: out(i, prefix, value) for all i < n; *total on device.
+// Generic scan: out(i, prefix, value) for all i < n; *total on device, and
+// at `mirror` (a Ctx::mirror slot) when that is set.
 template <class T, class F, class Out>
-void scan_apply(Ctx& ctx, F f, i64 n, Out out, T* total) {
+void scan_apply(Ctx& ctx, F f, i64 n, Out out, T* total, T* mirror = nullptr) {
     if (n <= 0) {
         if (total) ctx.zero(total, 1);
+        if (mirror) *static_cast<T*>(ctx.mirror_host()) = T(0);  // no kernel stores the empty total
         return;
     }
     static_assert(std::is_integral<T>::value, "integer scans only: the chunking must not change the sums");
@@ -244,7 +249,7 @@ void scan_apply(Ctx& ctx, F f, i64 n, Out out, T* total) {
         k_scan_up<T, F><<<g, kBlock, 0, ctx.stream>>>(f, n, chunk, partials);
         RAMA_LAUNCHED();
     }
-    k_scan_down<T, F, Out><<<g, kBlock, 0, ctx.stream>>>(f, n, chunk, out, partials, total);
+    k_scan_down<T, F, Out><<<g, kBlock, 0, ctx.stream>>>(f, n, chunk, out, partials, total, mirror);
     RAMA_LAUNCHED();
 }
 
@@ -252,17 +257,18 @@ void scan_apply(Ctx& ctx, F f, i64 n, Out out, T* total) {
 // alias the storage f reads (item i is written only by the block that owns
 // it, after the first launch and after its tile has read all its items).
 template <class T, class F>
-void scan_exclusive(Ctx& ctx, F f, T* out, i64 n) {
+void scan_exclusive(Ctx& ctx, F f, T* out, i64 n, T* mirror = nullptr) {
     if (n <= 0) {
         ctx.zero(out, 1);
+        if (mirror) *static_cast<T*>(ctx.mirror_host()) = T(0);
         return;
     }
-    scan_apply<T>(ctx, f, n, StoreExcl<T>{out}, out + n);
+    scan_apply<T>(ctx, f, n, StoreExcl<T>{out}, out + n, mirror);
 }
 
 template <class T>
-void scan_exclusive_arr(Ctx& ctx, const T* in, T* out, i64 n) {
-    scan_exclusive<T>(ctx, LoadFn<T>{in}, out, n);
+void scan_exclusive_arr(Ctx& ctx, const T* in, T* out, i64 n, T* mirror = nullptr) {
+    scan_exclusive<T>(ctx, LoadFn<T>{in}, out, n, mirror);
 }
 
 // Raises if any look-back gave up (never expected; guards against a hang).
@@ -303,7 +309,7 @@ __device__ __forceinline__ T combine_partials(const T* partials, unsigned nb, T 
 }
 
 template <class T, class Op, class F>
-__global__ __launch_bounds__(kBlock) void k_reduce1(F f, i64 n, T ident, Op op, T* partials, T* dst) {
+__global__ __launch_bounds__(kBlock) void k_reduce1(F f, i64 n, T ident, Op op, T* partials, T* dst, T* mirror) {
     __shared__ T ws[kWaves];
     __shared__ T own;
     T acc = ident;
@@ -316,27 +322,34 @@ __global__ __launch_bounds__(kBlock) void k_reduce1(F f, i64 n, T ident, Op op, 
     if (threadIdx.x == 0) own = r;
     __syncthreads();
     const T total = combine_partials(&own, 1u, ident, op, ws);
-    if (threadIdx.x == 0) *dst = total;
+    if (threadIdx.x == 0) {
+        *dst = total;
+        if (mirror) *mirror = total;
+    }
 }
 
 template <class T, class Op>
 __global__ __launch_bounds__(kBlock) void k_reduce_combine(const T* __restrict__ partials, unsigned nb, T ident,
-                                                           Op op, T* dst) {
+                                                           Op op, T* dst, T* mirror) {
     __shared__ T ws[kWaves];
     const T total = combine_partials(partials, nb, ident, op, ws);
-    if (threadIdx.x == 0) *dst = total;
+    if (threadIdx.x == 0) {
+        *dst = total;
+        if (mirror) *mirror = total;
+    }
 }
 
-// Deterministic reduction (fixed grid, fixed tree); result stays on device in *dst.
+// Deterministic reduction (fixed grid, fixed tree); result stays on device in
+// *dst, and is also stored at `mirror` (a Ctx::mirror slot) when that is set.
 template <class T, class Op, class F>
-void reduce_to(Ctx& ctx, F f, i64 n, T ident, Op op, T* dst) {
+void reduce_to(Ctx& ctx, F f, i64 n, T ident, Op op, T* dst, T* mirror = nullptr) {
     const unsigned nb = (unsigned)std::min<i64>(std::max<i64>((n + kBlock - 1) / kBlock, 1), kReduceMaxBlocks);
     static_assert(sizeof(T) <= 8, "reduction type");
     T* scratch = static_cast<T*>(ctx.reduce_scratch(kReduceMaxBlocks * 8));
-    k_reduce1<T, Op, F><<<nb, kBlock, 0, ctx.stream>>>(f, n, ident, op, scratch, dst);
+    k_reduce1<T, Op, F><<<nb, kBlock, 0, ctx.stream>>>(f, n, ident, op, scratch, dst, mirror);
     RAMA_LAUNCHED();
     if (nb > 1) {
-        k_reduce_combine<T, Op><<<1, kBlock, 0, ctx.stream>>>(scratch, nb, ident, op, dst);
+        k_reduce_combine<T, Op><<<1, kBlock, 0, ctx.stream>>>(scratch, nb, ident, op, dst, mirror);
         RAMA_LAUNCHED();
     }
 }
@@ -404,8 +417,11 @@ void reduce_pair_to(Ctx& ctx, F f, i64 n, double* dst_a, double* dst_b) {
 template <class T, class Op, class F>
 T reduce(Ctx& ctx, F f, i64 n, T ident, Op op) {
     DBuf<T> r = ctx.buf<T>(1);
-    reduce_to(ctx, f, n, ident, op, r.get());
-    return ctx.read1(r.get());
+    T* m = ctx.mirror<T>();
+    reduce_to(ctx, f, n, ident, op, r.get(), m);
+    T h;
+    ctx.read_mirror(r.get(), m, &h);
+    return h;
 }
 
 // ------------------------------------------------ segmented reduction
@@ -603,8 +619,8 @@ struct SelectOut {
 // Stable compaction in one launch: emit(i, position) for every i with
 // pred(i); the number selected is left on the device in *d_count.
 template <class P, class Emit>
-void select_into(Ctx& ctx, i64 n, P pred, Emit emit, int* d_count) {
-    scan_apply<int>(ctx, FlagFn<P>{pred}, n, SelectOut<Emit>{emit}, d_count);
+void select_into(Ctx& ctx, i64 n, P pred, Emit emit, int* d_count, int* mirror = nullptr) {
+    scan_apply<int>(ctx, FlagFn<P>{pred}, n, SelectOut<Emit>{emit}, d_count, mirror);
 }
 
 // ------------------------------------------------------------ radix sort
