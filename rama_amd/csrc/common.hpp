@@ -512,6 +512,25 @@ struct Ctx {
         std::memcpy(h, mirror_host(), sizeof(T) * n);
     }
 
+    // A flag that the next kernel raises (threads that find something store
+    // 1) and the host then reads with flag_value: the mirror slot, cleared
+    // by the host (no memset launch, no copy back), or in the
+    // allocation-free mode the device word d, cleared on the stream.
+    int* flag_slot(int* d) {
+        int* m = mirror<int>();
+        if (!m) {
+            zero(d, 1);
+            return d;
+        }
+        *static_cast<int*>(mirror_host()) = 0;
+        return m;
+    }
+    int flag_value(const int* d, const int* f, const char* fn = __builtin_FUNCTION(), int line = __builtin_LINE()) {
+        int h;
+        read_mirror(d, f == d ? nullptr : f, &h, 1, fn, line);
+        return h;
+    }
+
     // Read `n` device scalars (contiguous) into host memory (synchronises).
     template <class T>
     void read(const T* d, T* h, int n = 1, const char* fn = __builtin_FUNCTION(), int line = __builtin_LINE()) {

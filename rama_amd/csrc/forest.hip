@@ -102,7 +102,7 @@ i64 max_spanning_forest(Ctx& ctx, const int* r, const int* c, const float* w, i6
             atomicMax((unsigned long long*)&BEST[cu], (unsigned long long)k);
             atomicMax((unsigned long long*)&BEST[cv], (unsigned long long)k);
         });
-        ctx.zero(FL, 1);
+        int* fl = ctx.flag_slot(FL);
         for_each(ctx, n, [=] __device__(i64 v) {
             PAR[v] = (int)v;
             if (CMP[v] != (int)v) return;  // only component roots act
@@ -114,20 +114,20 @@ i64 max_spanning_forest(Ctx& ctx, const int* r, const int* c, const float* w, i6
             const bool mutual = BEST[other] == k;
             if (!(mutual && (int)v < other)) PAR[v] = other;
             INF[e] = 1;
-            FL[0] = 1;
+            fl[0] = 1;
         });
-        if (ctx.read1(FL) == 0) break;
+        if (ctx.flag_value(FL, fl) == 0) break;
         // pointer doubling on the component forest until every root is found
         for (int it = 0; it < 64; ++it) {
-            ctx.zero(FL, 1);
+            int* fl2 = ctx.flag_slot(FL);
             for_each(ctx, n, [=] __device__(i64 v) {
                 const int p = PAR[v], pp = PAR[p];
                 if (p != pp) {
                     PAR[v] = pp;
-                    FL[0] = 1;
+                    fl2[0] = 1;
                 }
             });
-            if (ctx.read1(FL) == 0) break;
+            if (ctx.flag_value(FL, fl2) == 0) break;
         }
         for_each(ctx, n, [=] __device__(i64 v) { CMP[v] = PAR[CMP[v]]; });
         // keep only edges that still join two components
@@ -283,7 +283,6 @@ static bool check_triangles(Ctx& ctx, WocState& S) {
     if (S.R == 0) return false;
     DBuf<int> rm = ctx.buf<int>(S.F), any = ctx.buf<int>(1);
     ctx.zero(rm.get(), S.F);
-    ctx.zero(any.get(), 1);
     int* RM = rm.get();
     int* ANY = any.get();
     const int* OFF = S.off.get();
@@ -292,6 +291,7 @@ static bool check_triangles(Ctx& ctx, WocState& S) {
     const float* FW = S.fw.get();
     const int* RR = S.rr.get();
     const int* RC = S.rc.get();
+    int* anyf = ctx.flag_slot(ANY);  // raised by the kernel below
     for_each(ctx, S.R, [=] __device__(i64 r) {
         const int v1 = RR[r], v2 = RC[r];
         int i1 = OFF[v1], i2 = OFF[v2];
@@ -299,10 +299,10 @@ static bool check_triangles(Ctx& ctx, WocState& S) {
         if (mid >= 0) {
             const int k = FW[i1 - 1] < FW[i2 - 1] ? i1 - 1 : i2 - 1;
             flag_pair(RM, OFF, FR, ADJ, k);
-            ANY[0] = 1;
+            anyf[0] = 1;
         }
     });
-    if (ctx.read1(ANY) == 0) return false;
+    if (ctx.flag_value(ANY, anyf) == 0) return false;
     forest_compact(ctx, S, RM);
     return true;
 }
@@ -317,17 +317,19 @@ static bool check_quadrangles(Ctx& ctx, WocState& S) {
     const int* RR = S.rr.get();
     const int* RC = S.rc.get();
     DBuf<i64> xoff = ctx.buf<i64>(S.R + 1);
+    i64* xm = ctx.mirror<i64>();
     scan_exclusive<i64>(ctx, [=] __device__(i64 r) { const int v = RR[r]; return (i64)(OFF[v + 1] - OFF[v]); },
-                        xoff.get(), S.R);
-    const i64 X = ctx.read1(xoff.get() + S.R);
+                        xoff.get(), S.R, xm);
+    i64 X;
+    ctx.read_mirror(xoff.get() + S.R, xm, &X);
     if (X == 0) return false;
     DBuf<int> rm = ctx.buf<int>(S.F), any = ctx.buf<int>(1);
     ctx.zero(rm.get(), S.F);
-    ctx.zero(any.get(), 1);
     int* RM = rm.get();
     int* ANY = any.get();
     const i64* XO = xoff.get();
     const i64 R = S.R;
+    int* anyf = ctx.flag_slot(ANY);  // raised by the kernel below
     for_each(ctx, X, [=] __device__(i64 x) {
         i64 lo = 0, hi = R - 1;  // largest r with XO[r] <= x
         while (lo < hi) {
@@ -343,10 +345,10 @@ static bool check_quadrangles(Ctx& ctx, WocState& S) {
             const float a = FW[e1], b = FW[i1 - 1], c = FW[i2 - 1];
             const int k = (a < b && a < c) ? e1 : (b < c ? i1 - 1 : i2 - 1);
             flag_pair(RM, OFF, FR, ADJ, k);
-            ANY[0] = 1;
+            anyf[0] = 1;
         }
     });
-    if (ctx.read1(ANY) == 0) return false;
+    if (ctx.flag_value(ANY, anyf) == 0) return false;
     forest_compact(ctx, S, RM);
     return true;
 }
@@ -355,12 +357,13 @@ static bool check_quadrangles(Ctx& ctx, WocState& S) {
 static bool filter_by_thresholding(Ctx& ctx, WocState& S, float mult) {
     const float* FW = S.fw.get();
     DBuf<double> s = ctx.buf<double>(2);
+    double* sm = ctx.mirror<double>();
     reduce_to<double>(ctx, [=] __device__(i64 e) { const float x = FW[e]; return x >= 0.0f ? (double)x : 0.0; }, S.F,
-                      0.0, OpSum(), s.get());
+                      0.0, OpSum(), s.get(), sm);
     reduce_to<double>(ctx, [=] __device__(i64 e) { return FW[e] >= 0.0f ? 1.0 : 0.0; }, S.F, 0.0, OpSum(),
-                      s.get() + 1);
+                      s.get() + 1, sm ? sm + 1 : nullptr);
     double h[2];
-    ctx.read(s.get(), h, 2);
+    ctx.read_mirror(s.get(), sm, h, 2);
     const float sf = (float)h[0];
     const float thr = mult * sf / (float)(i64)h[1];
     DBuf<int> rm = ctx.buf<int>(S.F);
@@ -380,10 +383,13 @@ static void expand(Ctx& ctx, WocState& S, WocState::Frontier& f) {
     const int* ND = f.node.get();
     const int* PA = f.parent.get();
     DBuf<int> xo = ctx.buf<int>(f.size + 1);
+    int* xm = ctx.mirror<int>();
     scan_exclusive<int>(
         ctx, [=] __device__(i64 q) { const int v = ND[q]; return OFF[v + 1] - OFF[v] - (PA[q] != -1 ? 1 : 0); },
-        xo.get(), f.size);
-    const i64 X = ctx.read1(xo.get() + f.size);
+        xo.get(), f.size, xm);
+    int xh;
+    ctx.read_mirror(xo.get() + f.size, xm, &xh);
+    const i64 X = xh;
     WocState::Frontier g;
     g.size = X;
     g.node = ctx.buf<int>(X);
@@ -453,7 +459,6 @@ static bool filter_cycles(Ctx& ctx, WocState& S) {
     frontier_sorted(ctx, S.colf, rb, kc, oc, kb);
     DBuf<int> rm = ctx.buf<int>(S.F), any = ctx.buf<int>(1);
     ctx.zero(rm.get(), S.F);
-    ctx.zero(any.get(), 1);
     int* RM = rm.get();
     int* ANY = any.get();
     const u64* KR = kr.get();
@@ -468,6 +473,7 @@ static bool filter_cycles(Ctx& ctx, WocState& S) {
     const int* OFF = S.off.get();
     const int* FR = S.frow.get();
     const int* ADJ = S.fadj.get();
+    int* anyf = ctx.flag_slot(ANY);  // raised by the kernel below
     for_each(ctx, S.rowf.size, [=] __device__(i64 q) {
         const u64 k = KR[q];
         i64 lo = 0, hi = nc - 1;
@@ -479,9 +485,9 @@ static bool filter_cycles(Ctx& ctx, WocState& S) {
         const u32 a = OR[q], b = OC[lo];
         const int idx = RBV[a] < CBV[b] ? RBI[a] : CBI[b];
         flag_pair(RM, OFF, FR, ADJ, idx);
-        ANY[0] = 1;
+        anyf[0] = 1;
     });
-    if (ctx.read1(ANY) == 0) return false;
+    if (ctx.flag_value(ANY, anyf) == 0) return false;
     forest_compact(ctx, S, RM);
     return true;
 }
