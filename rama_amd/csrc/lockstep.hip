@@ -468,10 +468,15 @@ void packed_dual_update(Ctx& ctx, Pack& P, int c, int steps, float factor, float
     ctx.d2h(eb.data(), deb.get(), Q + 1);
     eb[Q] = S.E;
     ctx.upload(deb.get() + Q, &eb[Q], 1);
+    // each slot's edges in a triangle (the edge launch covers only these)
+    DBuf<i64> dab = ctx.buf<i64>(Q + 1);
+    mp_active_ranges(ctx, S, deb.get(), Q, dab.get());
+    std::vector<i64> ab((size_t)Q + 1);
+    ctx.d2h(ab.data(), dab.get(), Q + 1);
     std::vector<int> tblk((size_t)Q + 1, 0), eblk((size_t)Q + 1, 0), onv((size_t)Q, 0);
     std::vector<i64> pbeg((size_t)Q, 0);
     for (int q = 0; q < Q; ++q) {
-        const int gt = (int)grid_for(tb[q + 1] - tb[q]), ge = (int)mp_edge_blocks(eb[q + 1] - eb[q]);
+        const int gt = (int)grid_for(tb[q + 1] - tb[q]), ge = (int)mp_edge_blocks(ab[q + 1] - ab[q]);
         tblk[q + 1] = tblk[q] + gt;
         eblk[q + 1] = eblk[q] + ge;
         pbeg[q] = (q ? pbeg[q - 1] + (tblk[q] - tblk[q - 1]) + (eblk[q] - eblk[q - 1]) : 0);
@@ -479,7 +484,7 @@ void packed_dual_update(Ctx& ctx, Pack& P, int c, int steps, float factor, float
     }
     DBuf<int> dtblk = upload_vec(ctx, tblk), deblk = upload_vec(ctx, eblk), don = upload_vec(ctx, onv);
     DBuf<i64> dpbeg = upload_vec(ctx, pbeg);
-    MPPack pk{Q, dtb.get(), deb.get(), dtblk.get(), deblk.get(), dpbeg.get()};
+    MPPack pk{Q, dtb.get(), deb.get(), dab.get(), dtblk.get(), deblk.get(), dpbeg.get()};
     std::vector<double> lb((size_t)Q, 0.0);
     mp_run_packed(ctx, S, pk, tblk[Q], eblk[Q], steps, tb.data(), eb.data(), don.get(), lb.data());
     for (int q = 0; q < Q; ++q)

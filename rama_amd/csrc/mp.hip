@@ -242,6 +242,21 @@ static void mp_build_impl(Ctx& ctx, const DCoo& A, Triangles& tri, MPState& S, i
         (s == 0 ? C12 : (s == 1 ? C13 : C23))[t] = u;
         POS[v] = (u32)q;  // slot 3t+s sends to place q, inside its edge's run [SOFF[u], SOFF[u+1])
     });
+    // the edges in a triangle, ascending: one per run of equal sorted items,
+    // at the run head's place in the merged list (as above, a head is never
+    // a repeat); the edge sweep visits only these
+    S.act = ctx.buf<int>(N3 > 0 ? N3 : 1);
+    {
+        int* ACT = S.act.get();
+        DBuf<int> na = ctx.buf<int>(1);
+        int* am = ctx.mirror<int>();
+        select_into(
+            ctx, N3, [=] __device__(i64 q) { return q == 0 || KT[q] != KT[q - 1]; },
+            [=] __device__(i64 q, int o) { ACT[o] = (int)(PA[q] + CP[q]); }, na.get(), am);
+        int h = 0;
+        ctx.read_mirror(na.get(), am, &h);
+        S.Ea = h;
+    }
 }
 
 void mp_build(Ctx& ctx, const DCoo& A, Triangles& tri, MPState& S) {
@@ -273,6 +288,7 @@ struct MPControl {
     int iterations;
     int max_col;   // copied from MPState::dims so one read returns both
     int last_row;
+    double linact;  // bound term of the edges in no triangle, added to every iteration's sum
 };
 
 // Thread t's share of a fixed-order sum: p[t] + p[t + kBlock] + ... in
@@ -297,8 +313,9 @@ __device__ __forceinline__ double strided_sum(const double* __restrict__ p, i64 
 __global__ __launch_bounds__(kBlock) void k_mp_finalize(const double* __restrict__ parts, i64 n, MPControl* ctl) {
     __shared__ double ws[kWaves];
     if (!ctl->active) return;
-    const double lb = block_reduce(strided_sum(parts, n), OpSum(), ws);
+    const double sum = block_reduce(strided_sum(parts, n), OpSum(), ws);
     if (threadIdx.x == 0) {
+        const double lb = sum + ctl->linact;
         ctl->prev = ctl->lb;
         ctl->lb = lb;
         ctl->iterations += 1;
@@ -386,27 +403,34 @@ __device__ __forceinline__ double mp_triangle(i64 t, const int* __restrict__ c12
 }
 
 // The edge update (send_messages_to_edges' sums, decrease_edge_costs_func,
-// :231-239): with kSweep, an edge's cost becomes the sum of its slot
-// messages (edges in no triangle keep their cost); without, edges in a
-// triangle are zeroed.  Its share cost / #triangles is stored beside it.
+// :231-239) for the edges in a triangle (act, ascending; the others keep
+// their cost, and their constant lower-bound term is added by the
+// finaliser): with kSweep, an edge's cost becomes the sum of its slot
+// messages; without, it is zeroed.  Its share cost / #triangles is stored
+// beside it.
 //
-// Edges e and e + kBlock (those below end) of one thread: offsets, costs
-// and the first four messages of both are loaded before either is summed;
-// each edge's sum keeps its slot order.  Returns the two lower-bound terms
-// added (e's first).
+// Active edges k and k + kBlock (those below end) of one thread: offsets,
+// costs and the first four messages of both are loaded before either is
+// summed; each edge's sum keeps its slot order.  Returns the two
+// lower-bound terms added (k's first).
 template <bool kSweep>
-__device__ __forceinline__ double mp_edge_pair(i64 e, i64 end, const int* __restrict__ soff,
-                                               const float* __restrict__ delta, float* __restrict__ ec,
-                                               float* __restrict__ share) {
+__device__ __forceinline__ double mp_edge_pair(i64 k, i64 end, const int* __restrict__ act,
+                                               const int* __restrict__ soff, const float* __restrict__ delta,
+                                               float* __restrict__ ec, float* __restrict__ share) {
     constexpr int kPre = 4;
-    i64 ee[2] = {e, e + kBlock};
+    i64 ee[2];
     bool has[2];
     int s0[2], s1[2];
     float c[2];
     float x[2][kPre];
 #pragma unroll
     for (int h = 0; h < 2; ++h) {
-        has[h] = ee[h] < end;
+        const i64 kk = k + (i64)h * kBlock;
+        has[h] = kk < end;
+        ee[h] = has[h] ? (i64)act[kk] : 0;
+    }
+#pragma unroll
+    for (int h = 0; h < 2; ++h) {
         s0[h] = has[h] ? soff[ee[h]] : 0;
         s1[h] = has[h] ? soff[ee[h] + 1] : 0;
         c[h] = has[h] ? ec[ee[h]] : 0.0f;
@@ -475,7 +499,7 @@ __global__ __launch_bounds__(kBlock) void k_mp_triangles(i64 T, const int* __res
 // Each block's ticket carries a release, whose cost grows with the grid, so
 // large grids finalise in a separate one-block launch instead.
 template <bool kSweep>
-__global__ __launch_bounds__(kBlock) void k_mp_edges(i64 E, const int* __restrict__ soff,
+__global__ __launch_bounds__(kBlock) void k_mp_edges(i64 Ea, const int* __restrict__ act, const int* __restrict__ soff,
                                                      const float* __restrict__ delta, float* __restrict__ ec,
                                                      float* __restrict__ share, double* __restrict__ lbpart,
                                                      MPControl* ctl, const double* parts, i64 nparts,
@@ -484,8 +508,8 @@ __global__ __launch_bounds__(kBlock) void k_mp_edges(i64 E, const int* __restric
     __shared__ bool last;
     if (ctl && !ctl->active) return;
     const unsigned blk = xcd_block();
-    const i64 e = (i64)blk * kMpEdgesPerBlock + threadIdx.x;
-    const double lb = mp_edge_pair<kSweep>(e, E, soff, delta, ec, share);
+    const i64 k = (i64)blk * kMpEdgesPerBlock + threadIdx.x;
+    const double lb = mp_edge_pair<kSweep>(k, Ea, act, soff, delta, ec, share);
     const double r = block_reduce(lb, OpSum(), ws);
     if (threadIdx.x == 0) lbpart[blk] = r;
     if (!parts) return;
@@ -496,8 +520,9 @@ __global__ __launch_bounds__(kBlock) void k_mp_edges(i64 E, const int* __restric
     __syncthreads();
     if (!last) return;
     __threadfence();  // acquire: the other blocks' partials are visible to plain loads
-    const double tot = block_reduce(strided_sum(parts, nparts), OpSum(), ws);
+    const double sum = block_reduce(strided_sum(parts, nparts), OpSum(), ws);
     if (threadIdx.x == 0) {
+        const double tot = sum + ctl->linact;
         ctl->prev = ctl->lb;
         ctl->lb = tot;
         ctl->iterations += 1;
@@ -514,8 +539,8 @@ static double sum_parts(Ctx& ctx, MPState& S, i64 n) {
 }
 
 static void launch_round(Ctx& ctx, MPState& S, bool sweep, MPControl* ctl = nullptr) {
-    const i64 T = S.T, E = S.E;
-    const unsigned gt = grid_for(T), ge = mp_edge_blocks(E);
+    const i64 T = S.T, Ea = S.Ea;
+    const unsigned gt = grid_for(T), ge = mp_edge_blocks(Ea);
     double* P = S.lbparts.get();
     {
         // per triangle: 3 edge ids, 3 share gathers, 3 costs in and out,
@@ -533,19 +558,22 @@ static void launch_round(Ctx& ctx, MPState& S, bool sweep, MPControl* ctl = null
     }
     // the device-driven loop (ctl set) finalises each iteration in the edge
     // kernel on small grids, in a separate launch on large ones
-    const bool fused = ctl && ge <= (unsigned)tune::get(tune::MpFusedMaxBlocks, E);
+    const bool fused = ctl && ge <= (unsigned)tune::get(tune::MpFusedMaxBlocks, Ea);
     const double* parts = fused ? P : nullptr;
     const i64 nparts = (i64)gt + (i64)ge;
     unsigned* ticket = fused ? ctx.reduce_ticket() : nullptr;
     {
-        // per edge: offsets, cost in/out, share out; per slot: its message
-        prof::Scope ps(ctx, sweep ? "mp_edges" : "mp_edges_zero", 16.0 * E + (sweep ? 12.0 * T : 0.0));
+        // per edge in a triangle: its index, offsets, cost in/out, share
+        // out; per slot: its message
+        prof::Scope ps(ctx, sweep ? "mp_edges" : "mp_edges_zero", 20.0 * Ea + (sweep ? 12.0 * T : 0.0));
         if (sweep)
-            k_mp_edges<true><<<ge, kBlock, 0, ctx.stream>>>(E, S.soff.get(), S.delta.get(), S.ec.get(),
-                                                             S.share.get(), P + gt, ctl, parts, nparts, ticket);
+            k_mp_edges<true><<<ge, kBlock, 0, ctx.stream>>>(Ea, S.act.get(), S.soff.get(), S.delta.get(),
+                                                             S.ec.get(), S.share.get(), P + gt, ctl, parts, nparts,
+                                                             ticket);
         else
-            k_mp_edges<false><<<ge, kBlock, 0, ctx.stream>>>(E, S.soff.get(), S.delta.get(), S.ec.get(),
-                                                              S.share.get(), P + gt, ctl, parts, nparts, ticket);
+            k_mp_edges<false><<<ge, kBlock, 0, ctx.stream>>>(Ea, S.act.get(), S.soff.get(), S.delta.get(),
+                                                              S.ec.get(), S.share.get(), P + gt, ctl, parts, nparts,
+                                                              ticket);
         RAMA_LAUNCHED();
     }
     if (ctl && !fused) {
@@ -556,13 +584,15 @@ static void launch_round(Ctx& ctx, MPState& S, bool sweep, MPControl* ctl = null
 
 // ------------------------------------------------------------------
 // Several instances in one message-passing state (packed batch).  Instance
-// q owns triangles [tbeg[q], tbeg[q+1]) and edges [ebeg[q], ebeg[q+1]); its
-// blocks are [tblk[q], tblk[q+1]) of the triangle launch and [eblk[q],
-// eblk[q+1]) of the edge launch, each covering 256 consecutive elements from
-// the instance's start exactly as its own launches would.  Its lower-bound
+// q owns triangles [tbeg[q], tbeg[q+1]), edges [ebeg[q], ebeg[q+1]) and,
+// of those, the edges in a triangle act[abeg[q] .. abeg[q+1]); its blocks
+// are [tblk[q], tblk[q+1]) of the triangle launch and [eblk[q], eblk[q+1])
+// of the edge launch, each covering consecutive elements from the
+// instance's start exactly as its own launches would.  Its lower-bound
 // partials sit at parts + pbeg[q] (triangle blocks, then edge blocks) and
-// are summed in the same order as k_mp_finalize sums them, so every
-// instance's bound and stall test equal those of a solve on its own.
+// are summed in the same order as k_mp_finalize sums them, plus its own
+// bound term of the edges in no triangle, so every instance's bound and
+// stall test equal those of a solve on its own.
 template <bool kSweep>
 __global__ __launch_bounds__(kBlock) void k_mp_triangles_packed(MPPack pk, const int* __restrict__ c12,
                                                                 const int* __restrict__ c13,
@@ -586,7 +616,8 @@ __global__ __launch_bounds__(kBlock) void k_mp_triangles_packed(MPPack pk, const
 }
 
 template <bool kSweep>
-__global__ __launch_bounds__(kBlock) void k_mp_edges_packed(MPPack pk, const int* __restrict__ soff,
+__global__ __launch_bounds__(kBlock) void k_mp_edges_packed(MPPack pk, const int* __restrict__ act,
+                                                            const int* __restrict__ soff,
                                                             const float* __restrict__ delta, float* __restrict__ ec,
                                                             float* __restrict__ share, double* __restrict__ parts,
                                                             const MPControl* __restrict__ ctl) {
@@ -594,8 +625,8 @@ __global__ __launch_bounds__(kBlock) void k_mp_edges_packed(MPPack pk, const int
     const int q = block_owner(pk.eblk, pk.Q, (int)blockIdx.x);
     if (!ctl[q].active) return;
     const int lb = (int)blockIdx.x - pk.eblk[q];
-    const i64 e = pk.ebeg[q] + (i64)lb * kMpEdgesPerBlock + threadIdx.x;
-    const double v = mp_edge_pair<kSweep>(e, pk.ebeg[q + 1], soff, delta, ec, share);
+    const i64 k = pk.abeg[q] + (i64)lb * kMpEdgesPerBlock + threadIdx.x;
+    const double v = mp_edge_pair<kSweep>(k, pk.abeg[q + 1], act, soff, delta, ec, share);
     const double r = block_reduce(v, OpSum(), ws);
     if (threadIdx.x == 0) parts[pk.pbeg[q] + (pk.tblk[q + 1] - pk.tblk[q]) + lb] = r;
 }
@@ -608,8 +639,9 @@ __global__ __launch_bounds__(kBlock) void k_mp_finalize_packed(MPPack pk, const 
     MPControl* c = ctl + q;
     if (!c->active) return;
     const i64 n = (i64)(pk.tblk[q + 1] - pk.tblk[q]) + (pk.eblk[q + 1] - pk.eblk[q]);
-    const double lb = block_reduce(strided_sum(parts + pk.pbeg[q], n), OpSum(), ws);
+    const double sum = block_reduce(strided_sum(parts + pk.pbeg[q], n), OpSum(), ws);
     if (threadIdx.x == 0) {
+        const double lb = sum + c->linact;
         c->prev = c->lb;
         c->lb = lb;
         c->iterations += 1;
@@ -618,12 +650,13 @@ __global__ __launch_bounds__(kBlock) void k_mp_finalize_packed(MPPack pk, const 
 }
 
 // Per instance: initial bound = edge part + triangle part (as mp_run's
-// first two reductions), loop state cleared; `on` marks the instances whose
-// loop runs.
+// first two reductions), the constant part of the edges in no triangle,
+// loop state cleared; `on` marks the instances whose loop runs.
 __global__ void k_mp_init_packed(int Q, const double* __restrict__ p_edges, const double* __restrict__ p_tris,
-                                 const int* __restrict__ on, MPControl* ctl) {
+                                 const double* __restrict__ p_rest, const int* __restrict__ on, MPControl* ctl) {
     const int q = blockIdx.x * blockDim.x + threadIdx.x;
     if (q >= Q) return;
+    ctl[q].linact = p_rest[q];
     ctl[q].lb = p_edges[q] + p_tris[q];
     ctl[q].prev = 0.0;
     ctl[q].active = on[q];
@@ -636,9 +669,11 @@ void mp_run_packed(Ctx& ctx, MPState& S, const MPPack& pk, int total_tblocks, in
                    const i64* seg_tbeg_host, const i64* seg_ebeg_host, const int* d_on,
                    double* lb_out) {
     const int Q = pk.Q;
-    // initial bound per instance, reduced as reduce_to reduces each part alone
-    DBuf<double> pe = ctx.buf<double>(2 * (i64)Q);
+    // initial bound per instance, reduced as reduce_to reduces each part
+    // alone; the edges in no triangle as mp_run's pair reduction sums them
+    DBuf<double> pe = ctx.buf<double>(3 * (i64)Q);
     const float* EC = S.ec.get();
+    const int* SO = S.soff.get();
     const float* A = S.t12.get();
     const float* B = S.t13.get();
     const float* D = S.t23.get();
@@ -646,8 +681,12 @@ void mp_run_packed(Ctx& ctx, MPState& S, const MPPack& pk, int total_tblocks, in
                        seg_ebeg_host, pk.ebeg, Q, 0.0, OpSum(), pe.get(), 1);
     seg_reduce<double>(ctx, [=] __device__(i64 t) { return (double)tri_lb(A[t], B[t], D[t]); }, seg_tbeg_host,
                        pk.tbeg, Q, 0.0, OpSum(), pe.get() + Q, 1);
+    seg_reduce<double>(
+        ctx, [=] __device__(i64 e) { const float c = EC[e]; return SO[e + 1] == SO[e] && c < 0.0f ? (double)c : 0.0; },
+        seg_ebeg_host, pk.ebeg, Q, 0.0, OpSum(), pe.get() + 2 * (i64)Q, 1);
     DBuf<MPControl> ctl = ctx.buf<MPControl>(Q);
-    k_mp_init_packed<<<grid_for(Q), kBlock, 0, ctx.stream>>>(Q, pe.get(), pe.get() + Q, d_on, ctl.get());
+    k_mp_init_packed<<<grid_for(Q), kBlock, 0, ctx.stream>>>(Q, pe.get(), pe.get() + Q, pe.get() + 2 * (i64)Q, d_on,
+                                                             ctl.get());
     RAMA_LAUNCHED();
     const i64 nparts = (i64)total_tblocks + total_eblocks;
     DBuf<double> parts = ctx.buf<double>(nparts);
@@ -657,7 +696,7 @@ void mp_run_packed(Ctx& ctx, MPState& S, const MPPack& pk, int total_tblocks, in
             S.pos.get(), S.delta.get(), parts.get(), ctl.get());
         RAMA_LAUNCHED();
         k_mp_edges_packed<true><<<(unsigned)total_eblocks, kBlock, 0, ctx.stream>>>(
-            pk, S.soff.get(), S.delta.get(), S.ec.get(), S.share.get(), parts.get(), ctl.get());
+            pk, S.act.get(), S.soff.get(), S.delta.get(), S.ec.get(), S.share.get(), parts.get(), ctl.get());
         RAMA_LAUNCHED();
         k_mp_finalize_packed<<<(unsigned)Q, kBlock, 0, ctx.stream>>>(pk, parts.get(), ctl.get());
         RAMA_LAUNCHED();
@@ -665,6 +704,24 @@ void mp_run_packed(Ctx& ctx, MPState& S, const MPPack& pk, int total_tblocks, in
     std::vector<MPControl> h((size_t)Q);
     ctx.d2h(h.data(), ctl.get(), Q);
     for (int q = 0; q < Q; ++q) lb_out[q] = h[q].lb;
+}
+
+__global__ void k_active_ranges(const int* __restrict__ act, i64 n, const i64* __restrict__ ebeg, int Q,
+                                i64* __restrict__ abeg) {
+    const int q = blockIdx.x * blockDim.x + threadIdx.x;
+    if (q > Q) return;
+    const i64 x = ebeg[q];
+    i64 lo = 0, hi = n;
+    while (lo < hi) {
+        const i64 mid = (lo + hi) >> 1;
+        if ((i64)act[mid] < x) lo = mid + 1; else hi = mid;
+    }
+    abeg[q] = lo;
+}
+
+void mp_active_ranges(Ctx& ctx, const MPState& S, const i64* ebeg, int Q, i64* abeg) {
+    k_active_ranges<<<grid_for((i64)Q + 1), kBlock, 0, ctx.stream>>>(S.act.get(), S.Ea, ebeg, Q, abeg);
+    RAMA_LAUNCHED();
 }
 
 // lower_bound() (:201-216) of the current state.
@@ -697,8 +754,17 @@ double mp_run(Ctx& ctx, MPState& S, int steps) {
     const float* B = S.t13.get();
     const float* D = S.t23.get();
     const i64 E = S.E;
-    reduce_to<double>(ctx, [=] __device__(i64 e) { const float c = EC[e]; return c < 0.0f ? (double)c : 0.0; }, E, 0.0,
-                      OpSum(), P);
+    const int* SO = S.soff.get();
+    // the edges' bound term, and that of the edges in no triangle (constant
+    // over the iterations), each summed as reduce_to sums it
+    reduce_pair_to(
+        ctx,
+        [=] __device__(i64 e, double& a, double& b) {
+            const float c = EC[e];
+            a = c < 0.0f ? (double)c : 0.0;
+            b = SO[e + 1] == SO[e] ? a : 0.0;
+        },
+        E, P, P + 2);
     reduce_to<double>(ctx, [=] __device__(i64 t) { return (double)tri_lb(A[t], B[t], D[t]); }, S.T, 0.0, OpSum(),
                       P + 1);
     MPControl* C = ctl.get();
@@ -711,6 +777,7 @@ double mp_run(Ctx& ctx, MPState& S, int steps) {
         C->iterations = 0;
         C->max_col = DM ? DM[0] : -1;
         C->last_row = LR ? *LR : -1;
+        C->linact = P[2];
     });
     ctx.reduce_ticket();  // created (and cleared) before any capture
     ctx.repeat(steps, mp_graphs(), [&] { launch_round(ctx, S, true, C); });
@@ -726,8 +793,15 @@ void mp_iteration(Ctx& ctx, MPState& S) { launch_round(ctx, S, true); }
 // Lower bound right after mp_iteration / mp_send_to_triplets, from the
 // fused per-block partials.
 double mp_fused_lower_bound(Ctx& ctx, MPState& S) {
-    const i64 n = (i64)grid_for(S.T) + (i64)mp_edge_blocks(S.E);
-    return sum_parts(ctx, S, n);
+    const i64 n = (i64)grid_for(S.T) + (i64)mp_edge_blocks(S.Ea);
+    const double sum = sum_parts(ctx, S, n);
+    // plus the bound term of the edges in no triangle
+    const float* EC = S.ec.get();
+    const int* SO = S.soff.get();
+    const double rest = reduce<double>(
+        ctx, [=] __device__(i64 e) { const float c = EC[e]; return SO[e + 1] == SO[e] && c < 0.0f ? (double)c : 0.0; },
+        S.E, 0.0, OpSum());
+    return sum + rest;
 }
 
 // reparametrized_edge_costs (:356-360) as a sorted dCOO.

@@ -142,19 +142,25 @@ struct MPState {
     DBuf<float> delta;           // per-slot messages (3T), edge-major
     DBuf<double> lbparts;        // partial lower bounds
     DBuf<int> dims;              // [0] max col of (ei, ej), on device
+    DBuf<int> act;               // the edges in at least one triangle, ascending (Ea of them)
+    DBuf<double> linact;         // lower-bound term of the other edges (their costs never change)
     int max_col = -1, last_row = -1;  // host copies once read back (-1: not yet)
-    i64 E = 0, T = 0;
+    i64 E = 0, T = 0, Ea = 0;
 };
 void mp_build(Ctx& ctx, const DCoo& A, Triangles& tri, MPState& S);
 // Packed instances in one message-passing state (see mp_run_packed).
 struct MPPack {
     int Q;
     const i64* tbeg;  // Q+1, device
-    const i64* ebeg;  // Q+1, device
+    const i64* ebeg;  // Q+1, device: edges of each instance
+    const i64* abeg;  // Q+1, device: its edges in a triangle, as positions in MPState::act
     const int* tblk;  // Q+1, device: prefix of per-instance triangle blocks
-    const int* eblk;  // Q+1, device: prefix of per-instance edge blocks
+    const int* eblk;  // Q+1, device: prefix of per-instance edge blocks (over its edges in a triangle)
     const i64* pbeg;  // Q, device: start of each instance's partials
 };
+// abeg[q] = position in S.act of the first edge of instance q with an
+// index of at least ebeg[q] (ebeg: Q+1 device values).
+void mp_active_ranges(Ctx& ctx, const MPState& S, const i64* ebeg, int Q, i64* abeg);
 // Runs `steps` iterations for every instance with d_on[q] != 0; lb_out[q]
 // (host) receives each instance's final bound.  seg_*beg_host are the host
 // copies of tbeg/ebeg.
