@@ -397,36 +397,13 @@ static Csr symmetric_csr_impl(Ctx& ctx, const int* row_in, const int* col_in, co
                               int n_nodes, bool filter, Keep keep, i64 known_count) {
     Csr G;
     G.n = n_nodes;
-    const int* row = row_in;
-    const int* col = col_in;
-    const float* val = val_in;
-    DBuf<int> fr, fc;
-    DBuf<float> fv;
     i64 P = m;
     if (filter) {
-        DBuf<int> cnt = ctx.buf<int>(1);
         if (known_count < 0) {
             P = reduce<i64>(ctx, [=] __device__(i64 e) { return (i64)keep(e); }, m, (i64)0, OpSum());
         } else {
             P = known_count;
         }
-        fr = ctx.buf<int>(P);
-        fc = ctx.buf<int>(P);
-        fv = ctx.buf<float>(P);
-        int* FR = fr.get();
-        int* FC = fc.get();
-        float* FV = fv.get();
-        select_into(
-            ctx, m, [=] __device__(i64 e) { return keep(e); },
-            [=] __device__(i64 e, int q) {
-                FR[q] = row_in[e];
-                FC[q] = col_in[e];
-                FV[q] = val_in[e];
-            },
-            cnt.get());
-        row = FR;
-        col = FC;
-        val = FV;
     }
     const i64 N2 = 2 * P;
     G.nnz = N2;
@@ -437,16 +414,42 @@ static Csr symmetric_csr_impl(Ctx& ctx, const int* row_in, const int* col_in, co
         ctx.zero(G.off.get(), (i64)n_nodes + 1);
         return G;
     }
-    // transpose: stable sort of (col, edge) by col
+    // the kept edges as (row, cost) pairs in one 8-byte word each (one
+    // gather per transposed entry instead of two) and their cols, and the
+    // transpose's sort input: (col, kept edge index)
+    DBuf<u64> rvb = ctx.buf<u64>(P);
+    DBuf<int> fc;
     DBuf<u32> tk = ctx.buf<u32>(P), tv = ctx.buf<u32>(P);
+    u64* RV = rvb.get();
     u32* TK = tk.get();
     u32* TV = tv.get();
-    for_each(ctx, P, [=] __device__(i64 e) {
-        TK[e] = (u32)col[e];
-        TV[e] = (u32)e;
-    });
-    prof::Site site_("csr");
-    radix_sort_pairs<u32>(ctx, tk, tv, P, bits_for(n_nodes > 1 ? n_nodes - 1 : 1));
+    const int* col = col_in;
+    if (filter) {
+        fc = ctx.buf<int>(P);
+        int* FC = fc.get();
+        DBuf<int> cnt = ctx.buf<int>(1);
+        select_into(
+            ctx, m, [=] __device__(i64 e) { return keep(e); },
+            [=] __device__(i64 e, int q) {
+                const int c = col_in[e];
+                RV[q] = ((u64)__float_as_uint(val_in[e]) << 32) | (u64)(u32)row_in[e];
+                FC[q] = c;
+                TK[q] = (u32)c;
+                TV[q] = (u32)q;
+            },
+            cnt.get());
+        col = FC;
+    } else {
+        for_each(ctx, P, [=] __device__(i64 e) {
+            RV[e] = ((u64)__float_as_uint(val_in[e]) << 32) | (u64)(u32)row_in[e];
+            TK[e] = (u32)col_in[e];
+            TV[e] = (u32)e;
+        });
+    }
+    {
+        prof::Site site_("csr");
+        radix_sort_pairs<u32>(ctx, tk, tv, P, bits_for(n_nodes > 1 ? n_nodes - 1 : 1));
+    }
     TK = tk.get();  // an odd pass count hands back the scratch buffers
     TV = tv.get();
     DBuf<int> offF = ctx.buf<int>((i64)n_nodes + 1), offT = ctx.buf<int>((i64)n_nodes + 1);
@@ -455,7 +458,7 @@ static Csr symmetric_csr_impl(Ctx& ctx, const int* row_in, const int* col_in, co
     // row offsets of the forward part (edge rows) and of the backward part
     // (sorted cols), one launch
     for_each(ctx, P + 1, [=] __device__(i64 i) {
-        const int pr = i > 0 ? row[i - 1] : -1, cr = i < P ? row[i] : n_nodes;
+        const int pr = i > 0 ? (int)(u32)RV[i - 1] : -1, cr = i < P ? (int)(u32)RV[i] : n_nodes;
         for (int r = pr + 1; r <= cr && r <= n_nodes; ++r) OFw[r] = (int)i;
         const int pc = i > 0 ? (int)TK[i - 1] : -1, cc = i < P ? (int)TK[i] : n_nodes;
         for (int r = pc + 1; r <= cc && r <= n_nodes; ++r) OTw[r] = (int)i;
@@ -472,16 +475,17 @@ static Csr symmetric_csr_impl(Ctx& ctx, const int* row_in, const int* col_in, co
         if (i < P) {
             {
                 const int r = (int)TK[i];
-                const u32 e = TV[i];
+                const u64 rv = RV[TV[i]];
                 const i64 o = (i64)OF[r] + OT[r] + (i - OT[r]);
-                ADJ[o] = row[e];
-                WW[o] = val[e];
+                ADJ[o] = (int)(u32)rv;
+                WW[o] = __uint_as_float((u32)(rv >> 32));
             }
             {
-                const int r = row[i];
+                const u64 rv = RV[i];
+                const int r = (int)(u32)rv;
                 const i64 o = (i64)OF[r] + OT[r] + (OT[r + 1] - OT[r]) + (i - OF[r]);
                 ADJ[o] = col[i];
-                WW[o] = val[i];
+                WW[o] = __uint_as_float((u32)(rv >> 32));
             }
         }
     });
