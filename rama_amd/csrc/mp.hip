@@ -125,7 +125,17 @@ static void mp_build_impl(Ctx& ctx, const DCoo& A, Triangles& tri, MPState& S, i
     const int* T1 = tri.t1.get();
     const int* T2 = tri.t2.get();
     const int* T3 = tri.t3.get();
+    // triangle costs start at zero, cleared in the same launch
+    S.t12 = ctx.buf<float>(T);
+    S.t13 = ctx.buf<float>(T);
+    S.t23 = ctx.buf<float>(T);
+    float* TC12 = S.t12.get();
+    float* TC13 = S.t13.get();
+    float* TC23 = S.t23.get();
     for_each(ctx, T, [=] __device__(i64 t) {
+        TC12[t] = 0.0f;
+        TC13[t] = 0.0f;
+        TC23[t] = 0.0f;
         const K a = (u32)T1[t], b = (u32)T2[t], c = (u32)T3[t];
         KT[3 * t + 0] = (a << kb) | b;
         KT[3 * t + 1] = (a << kb) | c;
@@ -177,9 +187,6 @@ static void mp_build_impl(Ctx& ctx, const DCoo& A, Triangles& tri, MPState& S, i
     S.c12 = ctx.buf<int>(T);
     S.c13 = ctx.buf<int>(T);
     S.c23 = ctx.buf<int>(T);
-    S.t12 = ctx.buf<float>(T);
-    S.t13 = ctx.buf<float>(T);
-    S.t23 = ctx.buf<float>(T);
     S.delta = ctx.buf<float>(N3 > 0 ? N3 : 1);
     S.pos = ctx.buf<u32>(N3 > 0 ? N3 : 1);
     int* EI = S.ei.get();
@@ -210,6 +217,14 @@ static void mp_build_impl(Ctx& ctx, const DCoo& A, Triangles& tri, MPState& S, i
         SOFF[u] = CH[c];
     });
     if (E == 0) for_each(ctx, 1, [=] __device__(i64) { SOFF[Em] = (int)N3; });
+    // largest col of the merged list: A's (its max_node) or a chord's; all
+    // cols are read when A's is not known
+    S.dims = ctx.buf<int>(2);
+    if (A.max_node >= 0 || E == 0)
+        reduce_to<int>(ctx, [=] __device__(i64 c) { return (int)((u64)CK[c] & mask); }, Nc, A.max_node, OpMax(),
+                       S.dims.get());
+    else
+        reduce_to<int>(ctx, [=] __device__(i64 u) { return EJ[u]; }, Em, -1, OpMax(), S.dims.get());
     S.share = ctx.buf<float>(Em);
     float* SH = S.share.get();
     for_each(ctx, Em, [=] __device__(i64 u) {
@@ -221,16 +236,7 @@ static void mp_build_impl(Ctx& ctx, const DCoo& A, Triangles& tri, MPState& S, i
     int* C12 = S.c12.get();
     int* C13 = S.c13.get();
     int* C23 = S.c23.get();
-    float* TC12 = S.t12.get();
-    float* TC13 = S.t13.get();
-    float* TC23 = S.t23.get();
     u32* POS = S.pos.get();
-    // triangle costs start at zero (coalesced, not per sorted item)
-    for_each(ctx, T, [=] __device__(i64 t) {
-        TC12[t] = 0.0f;
-        TC13[t] = 0.0f;
-        TC23[t] = 0.0f;
-    });
     for_each(ctx, N3, [=] __device__(i64 q) {
         const K k = KT[q];
         // chord heads before q, less the head of q's own run when that run
@@ -270,13 +276,7 @@ void mp_build(Ctx& ctx, const DCoo& A, Triangles& tri, MPState& S) {
         mp_build_impl<u64>(ctx, A, tri, S, kb);
     const i64 nb = (std::max(S.T, S.E) + kBlock - 1) / kBlock + 2;
     S.lbparts = ctx.buf<double>(2 * nb + 2);
-    // dimensions of the reparametrised graph, read back with the final bound
-    S.dims = ctx.buf<int>(2);
-    S.max_col = S.last_row = -1;
-    if (S.E > 0) {
-        const int* EJ = S.ej.get();
-        reduce_to<int>(ctx, [=] __device__(i64 e) { return EJ[e]; }, S.E, -1, OpMax(), S.dims.get());
-    }
+    S.max_col = S.last_row = -1;  // read back with the final bound
 }
 
 // Device-side state of the dual loop (src/dual_solver.cu:17-26): the

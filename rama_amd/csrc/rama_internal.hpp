@@ -141,7 +141,7 @@ struct MPState {
     DBuf<float> t12, t13, t23;   // triangle costs
     DBuf<float> delta;           // per-slot messages (3T), edge-major
     DBuf<double> lbparts;        // partial lower bounds
-    DBuf<int> dims;              // [0] max col of (ei, ej), on device
+    DBuf<int> dims;              // [0] max col of (ei, ej), on device (from A.max_node, A's largest col, and the chords)
     DBuf<int> act;               // the edges in at least one triangle, ascending (Ea of them)
     DBuf<double> linact;         // lower-bound term of the other edges (their costs never change)
     int max_col = -1, last_row = -1;  // host copies once read back (-1: not yet)
