@@ -109,6 +109,12 @@ static void sorted_lb(Ctx& ctx, const K* a, i64 n, const K* q, i64 m, int* out) 
 // triangles all go to its first entry (set_union_by_key and
 // set_intersection_by_key in src/multicut_message_passing.cu:46-50,131-156).
 template <class K>
+struct RunHead {  // item q starts a run of equal sorted keys
+    const K* kt;
+    __device__ __forceinline__ bool operator()(i64 q) const { return q == 0 || kt[q] != kt[q - 1]; }
+};
+
+template <class K>
 static void mp_build_impl(Ctx& ctx, const DCoo& A, Triangles& tri, MPState& S, int kb) {
     const i64 E = A.nnz, T = tri.n, N3 = 3 * T;
     S.T = T;
@@ -182,7 +188,6 @@ static void mp_build_impl(Ctx& ctx, const DCoo& A, Triangles& tri, MPState& S, i
     S.ei = ctx.buf<int>(Em);
     S.ej = ctx.buf<int>(Em);
     S.ec = ctx.buf<float>(Em);
-    S.ecnt = ctx.buf<int>(Em);
     S.soff = ctx.buf<int>(Em + 1);
     S.c12 = ctx.buf<int>(T);
     S.c13 = ctx.buf<int>(T);
@@ -192,7 +197,6 @@ static void mp_build_impl(Ctx& ctx, const DCoo& A, Triangles& tri, MPState& S, i
     int* EI = S.ei.get();
     int* EJ = S.ej.get();
     float* EC = S.ec.get();
-    int* ECNT = S.ecnt.get();
     int* SOFF = S.soff.get();
     const u64 mask = ((u64)1 << kb) - 1;
     // Places in the merged list come from the scan of the chord flags (CP,
@@ -225,44 +229,52 @@ static void mp_build_impl(Ctx& ctx, const DCoo& A, Triangles& tri, MPState& S, i
                        S.dims.get());
     else
         reduce_to<int>(ctx, [=] __device__(i64 u) { return EJ[u]; }, Em, -1, OpMax(), S.dims.get());
-    S.share = ctx.buf<float>(Em);
-    float* SH = S.share.get();
-    for_each(ctx, Em, [=] __device__(i64 u) {
-        const int cnt = SOFF[u + 1] - SOFF[u];
-        ECNT[u] = cnt;
-        SH[u] = cnt > 0 ? edge_share(EC[u], cnt) : 0.0f;
-    });
-    // triangle -> edge correspondences
+    // The edges in a triangle, ascending, are numbered k = 0 .. Ea-1, one
+    // per run of equal sorted items: act[k] is the run head's place in the
+    // merged list (as above; a head is never a repeat) and its messages are
+    // the places hq[k] .. hq[k+1]-1.  One scan over the items numbers the
+    // runs and gives every triangle slot its edge number k (the triangle
+    // sweep gathers share[k]) and its place q (where it sends its message).
+    S.act = ctx.buf<int>(N3 > 0 ? N3 : 1);
+    S.hq = ctx.buf<int>(N3 + 1);
     int* C12 = S.c12.get();
     int* C13 = S.c13.get();
     int* C23 = S.c23.get();
     u32* POS = S.pos.get();
-    for_each(ctx, N3, [=] __device__(i64 q) {
-        const K k = KT[q];
-        // chord heads before q, less the head of q's own run when that run
-        // is a chord and q is not its head
-        const bool dup = q > 0 && KT[q - 1] == k;
-        const bool chord = PA[q] == E || KA[PA[q]] != k;
-        const int u = (int)(PA[q] + CP[q] - (dup && chord ? 1 : 0));
-        const u32 v = VT[q], t = v / 3, s = v % 3;
-        (s == 0 ? C12 : (s == 1 ? C13 : C23))[t] = u;
-        POS[v] = (u32)q;  // slot 3t+s sends to place q, inside its edge's run [SOFF[u], SOFF[u+1])
-    });
-    // the edges in a triangle, ascending: one per run of equal sorted items,
-    // at the run head's place in the merged list (as above, a head is never
-    // a repeat); the edge sweep visits only these
-    S.act = ctx.buf<int>(N3 > 0 ? N3 : 1);
+    int* ACT = S.act.get();
+    int* HQ = S.hq.get();
     {
-        int* ACT = S.act.get();
         DBuf<int> na = ctx.buf<int>(1);
         int* am = ctx.mirror<int>();
-        select_into(
-            ctx, N3, [=] __device__(i64 q) { return q == 0 || KT[q] != KT[q - 1]; },
-            [=] __device__(i64 q, int o) { ACT[o] = (int)(PA[q] + CP[q]); }, na.get(), am);
+        scan_apply<int>(
+            ctx, FlagFn<RunHead<K>>{RunHead<K>{KT}}, N3,
+            [=] __device__(i64 q, int before, int head) {
+                const int k = before + head - 1;
+                if (head) {
+                    ACT[k] = (int)(PA[q] + CP[q]);
+                    HQ[k] = (int)q;
+                }
+                const u32 v = VT[q], t = v / 3, s = v % 3;
+                (s == 0 ? C12 : (s == 1 ? C13 : C23))[t] = k;
+                POS[v] = (u32)q;  // slot 3t+s sends to place q, inside its edge's run
+            },
+            na.get(), am);
         int h = 0;
         ctx.read_mirror(na.get(), am, &h);
         S.Ea = h;
     }
+    // what each triangle absorbs from edge k (cost / #triangles)
+    const i64 Ea = S.Ea;
+    S.share = ctx.buf<float>(Ea > 0 ? Ea : 1);
+    float* SH = S.share.get();
+    for_each(ctx, Ea + 1, [=] __device__(i64 k) {
+        if (k == Ea) {
+            HQ[Ea] = (int)N3;
+            return;
+        }
+        const int q0 = HQ[k], q1 = k + 1 < Ea ? HQ[k + 1] : (int)N3;
+        SH[k] = edge_share(EC[ACT[k]], q1 - q0);
+    });
 }
 
 void mp_build(Ctx& ctx, const DCoo& A, Triangles& tri, MPState& S) {
@@ -415,26 +427,24 @@ __device__ __forceinline__ double mp_triangle(i64 t, const int* __restrict__ c12
 // lower-bound terms added (k's first).
 template <bool kSweep>
 __device__ __forceinline__ double mp_edge_pair(i64 k, i64 end, const int* __restrict__ act,
-                                               const int* __restrict__ soff, const float* __restrict__ delta,
+                                               const int* __restrict__ hq, const float* __restrict__ delta,
                                                float* __restrict__ ec, float* __restrict__ share) {
     constexpr int kPre = 4;
-    i64 ee[2];
+    i64 kk[2], ee[2];
     bool has[2];
     int s0[2], s1[2];
     float c[2];
     float x[2][kPre];
 #pragma unroll
     for (int h = 0; h < 2; ++h) {
-        const i64 kk = k + (i64)h * kBlock;
-        has[h] = kk < end;
-        ee[h] = has[h] ? (i64)act[kk] : 0;
+        kk[h] = k + (i64)h * kBlock;
+        has[h] = kk[h] < end;
+        ee[h] = has[h] ? (i64)act[kk[h]] : 0;
+        s0[h] = has[h] ? hq[kk[h]] : 0;
+        s1[h] = has[h] ? hq[kk[h] + 1] : 0;
     }
 #pragma unroll
-    for (int h = 0; h < 2; ++h) {
-        s0[h] = has[h] ? soff[ee[h]] : 0;
-        s1[h] = has[h] ? soff[ee[h] + 1] : 0;
-        c[h] = has[h] ? ec[ee[h]] : 0.0f;
-    }
+    for (int h = 0; h < 2; ++h) c[h] = has[h] ? ec[ee[h]] : 0.0f;
     if (kSweep) {
 #pragma unroll
         for (int h = 0; h < 2; ++h)
@@ -467,7 +477,7 @@ __device__ __forceinline__ double mp_edge_pair(i64 k, i64 end, const int* __rest
                 cc = 0.0f;
             }
             ec[ee[h]] = cc;
-            share[ee[h]] = edge_share(cc, s1[h] - s0[h]);
+            share[kk[h]] = edge_share(cc, s1[h] - s0[h]);
         }
         lb += cc < 0.0f ? (double)cc : 0.0;
     }
@@ -499,7 +509,7 @@ __global__ __launch_bounds__(kBlock) void k_mp_triangles(i64 T, const int* __res
 // Each block's ticket carries a release, whose cost grows with the grid, so
 // large grids finalise in a separate one-block launch instead.
 template <bool kSweep>
-__global__ __launch_bounds__(kBlock) void k_mp_edges(i64 Ea, const int* __restrict__ act, const int* __restrict__ soff,
+__global__ __launch_bounds__(kBlock) void k_mp_edges(i64 Ea, const int* __restrict__ act, const int* __restrict__ hq,
                                                      const float* __restrict__ delta, float* __restrict__ ec,
                                                      float* __restrict__ share, double* __restrict__ lbpart,
                                                      MPControl* ctl, const double* parts, i64 nparts,
@@ -509,7 +519,7 @@ __global__ __launch_bounds__(kBlock) void k_mp_edges(i64 Ea, const int* __restri
     if (ctl && !ctl->active) return;
     const unsigned blk = xcd_block();
     const i64 k = (i64)blk * kMpEdgesPerBlock + threadIdx.x;
-    const double lb = mp_edge_pair<kSweep>(k, Ea, act, soff, delta, ec, share);
+    const double lb = mp_edge_pair<kSweep>(k, Ea, act, hq, delta, ec, share);
     const double r = block_reduce(lb, OpSum(), ws);
     if (threadIdx.x == 0) lbpart[blk] = r;
     if (!parts) return;
@@ -567,11 +577,11 @@ static void launch_round(Ctx& ctx, MPState& S, bool sweep, MPControl* ctl = null
         // out; per slot: its message
         prof::Scope ps(ctx, sweep ? "mp_edges" : "mp_edges_zero", 20.0 * Ea + (sweep ? 12.0 * T : 0.0));
         if (sweep)
-            k_mp_edges<true><<<ge, kBlock, 0, ctx.stream>>>(Ea, S.act.get(), S.soff.get(), S.delta.get(),
+            k_mp_edges<true><<<ge, kBlock, 0, ctx.stream>>>(Ea, S.act.get(), S.hq.get(), S.delta.get(),
                                                              S.ec.get(), S.share.get(), P + gt, ctl, parts, nparts,
                                                              ticket);
         else
-            k_mp_edges<false><<<ge, kBlock, 0, ctx.stream>>>(Ea, S.act.get(), S.soff.get(), S.delta.get(),
+            k_mp_edges<false><<<ge, kBlock, 0, ctx.stream>>>(Ea, S.act.get(), S.hq.get(), S.delta.get(),
                                                               S.ec.get(), S.share.get(), P + gt, ctl, parts, nparts,
                                                               ticket);
         RAMA_LAUNCHED();
@@ -617,7 +627,7 @@ __global__ __launch_bounds__(kBlock) void k_mp_triangles_packed(MPPack pk, const
 
 template <bool kSweep>
 __global__ __launch_bounds__(kBlock) void k_mp_edges_packed(MPPack pk, const int* __restrict__ act,
-                                                            const int* __restrict__ soff,
+                                                            const int* __restrict__ hq,
                                                             const float* __restrict__ delta, float* __restrict__ ec,
                                                             float* __restrict__ share, double* __restrict__ parts,
                                                             const MPControl* __restrict__ ctl) {
@@ -626,7 +636,7 @@ __global__ __launch_bounds__(kBlock) void k_mp_edges_packed(MPPack pk, const int
     if (!ctl[q].active) return;
     const int lb = (int)blockIdx.x - pk.eblk[q];
     const i64 k = pk.abeg[q] + (i64)lb * kMpEdgesPerBlock + threadIdx.x;
-    const double v = mp_edge_pair<kSweep>(k, pk.abeg[q + 1], act, soff, delta, ec, share);
+    const double v = mp_edge_pair<kSweep>(k, pk.abeg[q + 1], act, hq, delta, ec, share);
     const double r = block_reduce(v, OpSum(), ws);
     if (threadIdx.x == 0) parts[pk.pbeg[q] + (pk.tblk[q + 1] - pk.tblk[q]) + lb] = r;
 }
@@ -696,7 +706,7 @@ void mp_run_packed(Ctx& ctx, MPState& S, const MPPack& pk, int total_tblocks, in
             S.pos.get(), S.delta.get(), parts.get(), ctl.get());
         RAMA_LAUNCHED();
         k_mp_edges_packed<true><<<(unsigned)total_eblocks, kBlock, 0, ctx.stream>>>(
-            pk, S.act.get(), S.soff.get(), S.delta.get(), S.ec.get(), S.share.get(), parts.get(), ctl.get());
+            pk, S.act.get(), S.hq.get(), S.delta.get(), S.ec.get(), S.share.get(), parts.get(), ctl.get());
         RAMA_LAUNCHED();
         k_mp_finalize_packed<<<(unsigned)Q, kBlock, 0, ctx.stream>>>(pk, parts.get(), ctl.get());
         RAMA_LAUNCHED();

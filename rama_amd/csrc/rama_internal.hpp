@@ -133,17 +133,16 @@ struct MessagePassing;  // opaque
 struct MPState {
     DBuf<int> ei, ej;            // message-passing edges (sorted, unique)
     DBuf<float> ec;              // edge costs
-    DBuf<float> share;           // ec / ecnt for edges in a triangle (what each triangle absorbs)
-    DBuf<int> ecnt;              // #triangles per edge
+    DBuf<float> share;           // per edge in a triangle (k): its cost / #triangles, what each triangle absorbs
     DBuf<int> soff;              // edge -> its run of messages in delta (E+1)
     DBuf<u32> pos;               // slot 3t+k -> its place in delta (runs by edge, (triangle, slot) order)
-    DBuf<int> c12, c13, c23;     // triangle -> edge index
+    DBuf<int> c12, c13, c23;     // triangle -> number k of its edge among the edges in a triangle
     DBuf<float> t12, t13, t23;   // triangle costs
     DBuf<float> delta;           // per-slot messages (3T), edge-major
     DBuf<double> lbparts;        // partial lower bounds
     DBuf<int> dims;              // [0] max col of (ei, ej), on device (from A.max_node, A's largest col, and the chords)
-    DBuf<int> act;               // the edges in at least one triangle, ascending (Ea of them)
-    DBuf<double> linact;         // lower-bound term of the other edges (their costs never change)
+    DBuf<int> act;               // the edges in at least one triangle, ascending (Ea of them): act[k] = edge index
+    DBuf<int> hq;                // Ea + 1: edge k's messages are delta[hq[k] .. hq[k+1])
     int max_col = -1, last_row = -1;  // host copies once read back (-1: not yet)
     i64 E = 0, T = 0, Ea = 0;
 };
