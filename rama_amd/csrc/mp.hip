@@ -263,9 +263,12 @@ static void mp_build_impl(Ctx& ctx, const DCoo& A, Triangles& tri, MPState& S, i
         ctx.read_mirror(na.get(), am, &h);
         S.Ea = h;
     }
-    // what each triangle absorbs from edge k (cost / #triangles)
+    // edge k's cost while the iterations run, and what each triangle
+    // absorbs from it (cost / #triangles)
     const i64 Ea = S.Ea;
+    S.eca = ctx.buf<float>(Ea > 0 ? Ea : 1);
     S.share = ctx.buf<float>(Ea > 0 ? Ea : 1);
+    float* ECA = S.eca.get();
     float* SH = S.share.get();
     for_each(ctx, Ea + 1, [=] __device__(i64 k) {
         if (k == Ea) {
@@ -273,7 +276,9 @@ static void mp_build_impl(Ctx& ctx, const DCoo& A, Triangles& tri, MPState& S, i
             return;
         }
         const int q0 = HQ[k], q1 = k + 1 < Ea ? HQ[k + 1] : (int)N3;
-        SH[k] = edge_share(EC[ACT[k]], q1 - q0);
+        const float c = EC[ACT[k]];
+        ECA[k] = c;
+        SH[k] = edge_share(c, q1 - q0);
     });
 }
 
@@ -426,11 +431,11 @@ __device__ __forceinline__ double mp_triangle(i64 t, const int* __restrict__ c12
 // summed; each edge's sum keeps its slot order.  Returns the two
 // lower-bound terms added (k's first).
 template <bool kSweep>
-__device__ __forceinline__ double mp_edge_pair(i64 k, i64 end, const int* __restrict__ act,
-                                               const int* __restrict__ hq, const float* __restrict__ delta,
-                                               float* __restrict__ ec, float* __restrict__ share) {
+__device__ __forceinline__ double mp_edge_pair(i64 k, i64 end, const int* __restrict__ hq,
+                                               const float* __restrict__ delta, float* __restrict__ eca,
+                                               float* __restrict__ share) {
     constexpr int kPre = 4;
-    i64 kk[2], ee[2];
+    i64 kk[2];
     bool has[2];
     int s0[2], s1[2];
     float c[2];
@@ -439,12 +444,10 @@ __device__ __forceinline__ double mp_edge_pair(i64 k, i64 end, const int* __rest
     for (int h = 0; h < 2; ++h) {
         kk[h] = k + (i64)h * kBlock;
         has[h] = kk[h] < end;
-        ee[h] = has[h] ? (i64)act[kk[h]] : 0;
         s0[h] = has[h] ? hq[kk[h]] : 0;
         s1[h] = has[h] ? hq[kk[h] + 1] : 0;
+        c[h] = has[h] ? eca[kk[h]] : 0.0f;
     }
-#pragma unroll
-    for (int h = 0; h < 2; ++h) c[h] = has[h] ? ec[ee[h]] : 0.0f;
     if (kSweep) {
 #pragma unroll
         for (int h = 0; h < 2; ++h)
@@ -476,7 +479,7 @@ __device__ __forceinline__ double mp_edge_pair(i64 k, i64 end, const int* __rest
             } else {
                 cc = 0.0f;
             }
-            ec[ee[h]] = cc;
+            eca[kk[h]] = cc;
             share[kk[h]] = edge_share(cc, s1[h] - s0[h]);
         }
         lb += cc < 0.0f ? (double)cc : 0.0;
@@ -509,8 +512,8 @@ __global__ __launch_bounds__(kBlock) void k_mp_triangles(i64 T, const int* __res
 // Each block's ticket carries a release, whose cost grows with the grid, so
 // large grids finalise in a separate one-block launch instead.
 template <bool kSweep>
-__global__ __launch_bounds__(kBlock) void k_mp_edges(i64 Ea, const int* __restrict__ act, const int* __restrict__ hq,
-                                                     const float* __restrict__ delta, float* __restrict__ ec,
+__global__ __launch_bounds__(kBlock) void k_mp_edges(i64 Ea, const int* __restrict__ hq,
+                                                     const float* __restrict__ delta, float* __restrict__ eca,
                                                      float* __restrict__ share, double* __restrict__ lbpart,
                                                      MPControl* ctl, const double* parts, i64 nparts,
                                                      unsigned* ticket) {
@@ -519,7 +522,7 @@ __global__ __launch_bounds__(kBlock) void k_mp_edges(i64 Ea, const int* __restri
     if (ctl && !ctl->active) return;
     const unsigned blk = xcd_block();
     const i64 k = (i64)blk * kMpEdgesPerBlock + threadIdx.x;
-    const double lb = mp_edge_pair<kSweep>(k, Ea, act, hq, delta, ec, share);
+    const double lb = mp_edge_pair<kSweep>(k, Ea, hq, delta, eca, share);
     const double r = block_reduce(lb, OpSum(), ws);
     if (threadIdx.x == 0) lbpart[blk] = r;
     if (!parts) return;
@@ -577,13 +580,11 @@ static void launch_round(Ctx& ctx, MPState& S, bool sweep, MPControl* ctl = null
         // out; per slot: its message
         prof::Scope ps(ctx, sweep ? "mp_edges" : "mp_edges_zero", 20.0 * Ea + (sweep ? 12.0 * T : 0.0));
         if (sweep)
-            k_mp_edges<true><<<ge, kBlock, 0, ctx.stream>>>(Ea, S.act.get(), S.hq.get(), S.delta.get(),
-                                                             S.ec.get(), S.share.get(), P + gt, ctl, parts, nparts,
-                                                             ticket);
+            k_mp_edges<true><<<ge, kBlock, 0, ctx.stream>>>(Ea, S.hq.get(), S.delta.get(), S.eca.get(),
+                                                             S.share.get(), P + gt, ctl, parts, nparts, ticket);
         else
-            k_mp_edges<false><<<ge, kBlock, 0, ctx.stream>>>(Ea, S.act.get(), S.hq.get(), S.delta.get(),
-                                                              S.ec.get(), S.share.get(), P + gt, ctl, parts, nparts,
-                                                              ticket);
+            k_mp_edges<false><<<ge, kBlock, 0, ctx.stream>>>(Ea, S.hq.get(), S.delta.get(), S.eca.get(),
+                                                              S.share.get(), P + gt, ctl, parts, nparts, ticket);
         RAMA_LAUNCHED();
     }
     if (ctl && !fused) {
@@ -626,9 +627,8 @@ __global__ __launch_bounds__(kBlock) void k_mp_triangles_packed(MPPack pk, const
 }
 
 template <bool kSweep>
-__global__ __launch_bounds__(kBlock) void k_mp_edges_packed(MPPack pk, const int* __restrict__ act,
-                                                            const int* __restrict__ hq,
-                                                            const float* __restrict__ delta, float* __restrict__ ec,
+__global__ __launch_bounds__(kBlock) void k_mp_edges_packed(MPPack pk, const int* __restrict__ hq,
+                                                            const float* __restrict__ delta, float* __restrict__ eca,
                                                             float* __restrict__ share, double* __restrict__ parts,
                                                             const MPControl* __restrict__ ctl) {
     __shared__ double ws[kWaves];
@@ -636,7 +636,7 @@ __global__ __launch_bounds__(kBlock) void k_mp_edges_packed(MPPack pk, const int
     if (!ctl[q].active) return;
     const int lb = (int)blockIdx.x - pk.eblk[q];
     const i64 k = pk.abeg[q] + (i64)lb * kMpEdgesPerBlock + threadIdx.x;
-    const double v = mp_edge_pair<kSweep>(k, pk.abeg[q + 1], act, hq, delta, ec, share);
+    const double v = mp_edge_pair<kSweep>(k, pk.abeg[q + 1], hq, delta, eca, share);
     const double r = block_reduce(v, OpSum(), ws);
     if (threadIdx.x == 0) parts[pk.pbeg[q] + (pk.tblk[q + 1] - pk.tblk[q]) + lb] = r;
 }
@@ -706,11 +706,12 @@ void mp_run_packed(Ctx& ctx, MPState& S, const MPPack& pk, int total_tblocks, in
             S.pos.get(), S.delta.get(), parts.get(), ctl.get());
         RAMA_LAUNCHED();
         k_mp_edges_packed<true><<<(unsigned)total_eblocks, kBlock, 0, ctx.stream>>>(
-            pk, S.act.get(), S.hq.get(), S.delta.get(), S.ec.get(), S.share.get(), parts.get(), ctl.get());
+            pk, S.hq.get(), S.delta.get(), S.eca.get(), S.share.get(), parts.get(), ctl.get());
         RAMA_LAUNCHED();
         k_mp_finalize_packed<<<(unsigned)Q, kBlock, 0, ctx.stream>>>(pk, parts.get(), ctl.get());
         RAMA_LAUNCHED();
     });
+    mp_store_costs(ctx, S);
     std::vector<MPControl> h((size_t)Q);
     ctx.d2h(h.data(), ctl.get(), Q);
     for (int q = 0; q < Q; ++q) lb_out[q] = h[q].lb;
@@ -734,8 +735,16 @@ void mp_active_ranges(Ctx& ctx, const MPState& S, const i64* ebeg, int Q, i64* a
     RAMA_LAUNCHED();
 }
 
+void mp_store_costs(Ctx& ctx, MPState& S) {
+    float* EC = S.ec.get();
+    const float* ECA = S.eca.get();
+    const int* ACT = S.act.get();
+    for_each(ctx, S.Ea, [=] __device__(i64 k) { EC[ACT[k]] = ECA[k]; });
+}
+
 // lower_bound() (:201-216) of the current state.
 double mp_lower_bound(Ctx& ctx, MPState& S) {
+    mp_store_costs(ctx, S);
     const float* EC = S.ec.get();
     const float* A = S.t12.get();
     const float* B = S.t13.get();
@@ -816,6 +825,7 @@ double mp_fused_lower_bound(Ctx& ctx, MPState& S) {
 
 // reparametrized_edge_costs (:356-360) as a sorted dCOO.
 DCoo mp_reparametrized(Ctx& ctx, MPState& S) {
+    mp_store_costs(ctx, S);
     if (S.E > 0 && S.max_col >= 0) {
         DCoo A;
         A.row = std::move(S.ei);

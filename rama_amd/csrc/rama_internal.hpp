@@ -143,6 +143,7 @@ struct MPState {
     DBuf<int> dims;              // [0] max col of (ei, ej), on device (from A.max_node, A's largest col, and the chords)
     DBuf<int> act;               // the edges in at least one triangle, ascending (Ea of them): act[k] = edge index
     DBuf<int> hq;                // Ea + 1: edge k's messages are delta[hq[k] .. hq[k+1])
+    DBuf<float> eca;             // Ea: cost of edge k while the iterations run (stored into ec by mp_store_costs)
     int max_col = -1, last_row = -1;  // host copies once read back (-1: not yet)
     i64 E = 0, T = 0, Ea = 0;
 };
@@ -166,6 +167,9 @@ void mp_active_ranges(Ctx& ctx, const MPState& S, const i64* ebeg, int Q, i64* a
 void mp_run_packed(Ctx& ctx, MPState& S, const MPPack& pk, int total_tblocks, int total_eblocks, int steps,
                    const i64* seg_tbeg_host, const i64* seg_ebeg_host, const int* d_on,
                    double* lb_out);
+// ec[act[k]] = eca[k]: the iterations update the costs of the edges in a
+// triangle in eca; ec receives them before it is read.
+void mp_store_costs(Ctx& ctx, MPState& S);
 double mp_lower_bound(Ctx& ctx, MPState& S);
 double mp_fused_lower_bound(Ctx& ctx, MPState& S);
 void mp_send_to_triplets(Ctx& ctx, MPState& S);
