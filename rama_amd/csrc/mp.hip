@@ -576,9 +576,9 @@ static void launch_round(Ctx& ctx, MPState& S, bool sweep, MPControl* ctl = null
     const i64 nparts = (i64)gt + (i64)ge;
     unsigned* ticket = fused ? ctx.reduce_ticket() : nullptr;
     {
-        // per edge in a triangle: its index, offsets, cost in/out, share
+        // per edge in a triangle: its message range, cost in/out, share
         // out; per slot: its message
-        prof::Scope ps(ctx, sweep ? "mp_edges" : "mp_edges_zero", 20.0 * Ea + (sweep ? 12.0 * T : 0.0));
+        prof::Scope ps(ctx, sweep ? "mp_edges" : "mp_edges_zero", 16.0 * Ea + (sweep ? 12.0 * T : 0.0));
         if (sweep)
             k_mp_edges<true><<<ge, kBlock, 0, ctx.stream>>>(Ea, S.hq.get(), S.delta.get(), S.eca.get(),
                                                              S.share.get(), P + gt, ctl, parts, nparts, ticket);
