@@ -154,15 +154,10 @@ static void mp_build_impl(Ctx& ctx, const DCoo& A, Triangles& tri, MPState& S, i
     radix_sort_pairs<K>(ctx, kt, vt, N3, 2 * kb);
     KT = kt.get();  // an odd pass count hands back the scratch buffers
     VT = vt.get();
-    // position of every sorted triangle item among the graph keys, and of
-    // every graph key among the sorted triangle items
-    DBuf<int> pa = ctx.buf<int>(N3 > 0 ? N3 : 1), pt = ctx.buf<int>(E > 0 ? E : 1);
+    // position of every sorted triangle item among the graph keys
+    DBuf<int> pa = ctx.buf<int>(N3 > 0 ? N3 : 1);
     int* PA = pa.get();
-    int* PT = pt.get();
     sorted_lb<K>(ctx, KA, E, KT, N3, PA);
-    // a repeat of a graph pair searches past the pair's items: its run of
-    // slots is empty and the first entry owns them
-    sorted_lb<K, true>(ctx, KT, N3, KA, E, PT);
     // chords: first item of a key run whose key is not a graph edge
     auto is_chord = [=] __device__(i64 q) {
         if (q > 0 && KT[q] == KT[q - 1]) return 0;
@@ -183,12 +178,19 @@ static void mp_build_impl(Ctx& ctx, const DCoo& A, Triangles& tri, MPState& S, i
                 CH[CP[q]] = (int)q;
             }
         });
+    // chords below every graph key (none to search when there are none)
+    DBuf<int> cbelow;
+    if (Nc > 0) {
+        cbelow = ctx.buf<int>(E > 0 ? E : 1);
+        sorted_lb<K>(ctx, CK, Nc, KA, E, cbelow.get());
+    }
+    const int* CB = Nc > 0 ? cbelow.get() : nullptr;
     const i64 Em = E + Nc;
     S.E = Em;
     S.ei = ctx.buf<int>(Em);
     S.ej = ctx.buf<int>(Em);
     S.ec = ctx.buf<float>(Em);
-    S.soff = ctx.buf<int>(Em + 1);
+    S.inact = ctx.buf<unsigned char>(Em > 0 ? Em : 1);
     S.c12 = ctx.buf<int>(T);
     S.c13 = ctx.buf<int>(T);
     S.c23 = ctx.buf<int>(T);
@@ -197,30 +199,27 @@ static void mp_build_impl(Ctx& ctx, const DCoo& A, Triangles& tri, MPState& S, i
     int* EI = S.ei.get();
     int* EJ = S.ej.get();
     float* EC = S.ec.get();
-    int* SOFF = S.soff.get();
+    unsigned char* INACT = S.inact.get();
     const u64 mask = ((u64)1 << kb) - 1;
-    // Places in the merged list come from the scan of the chord flags (CP,
-    // N3 + 1 entries): the chords below a key are the chord heads before its
-    // first triangle item.
-    // graph edges: the chords below KA[e] are the heads before PT[e] (no
-    // chord head lies inside the items of a graph key)
+    // Places in the merged list: a graph edge's index plus the chords below
+    // its key; a chord's index among the chords plus the graph keys below
+    // it (PA of its head).  Every place starts out in no triangle (the
+    // numbering scan below marks those in one), chords excepted.
     for_each(ctx, E, [=] __device__(i64 e) {
-        const i64 u = e + CP[PT[e]];
+        const i64 u = e + (CB ? CB[e] : 0);
         EI[u] = AR[e];
         EJ[u] = AC[e];
         EC[u] = AV[e];
-        SOFF[u] = PT[e];
-        if (e == 0) SOFF[Em] = (int)N3;
+        INACT[u] = 1;
     });
-    // chords (cost 0): the graph keys below a chord's key are PA of its head
+    // chords (cost 0)
     for_each(ctx, Nc, [=] __device__(i64 c) {
         const i64 u = c + PA[CH[c]];
         EI[u] = (int)((u64)CK[c] >> kb);
         EJ[u] = (int)((u64)CK[c] & mask);
         EC[u] = 0.0f;
-        SOFF[u] = CH[c];
+        INACT[u] = 0;
     });
-    if (E == 0) for_each(ctx, 1, [=] __device__(i64) { SOFF[Em] = (int)N3; });
     // largest col of the merged list: A's (its max_node) or a chord's; all
     // cols are read when A's is not known
     S.dims = ctx.buf<int>(2);
@@ -251,8 +250,10 @@ static void mp_build_impl(Ctx& ctx, const DCoo& A, Triangles& tri, MPState& S, i
             [=] __device__(i64 q, int before, int head) {
                 const int k = before + head - 1;
                 if (head) {
-                    ACT[k] = (int)(PA[q] + CP[q]);
+                    const int u = (int)(PA[q] + CP[q]);
+                    ACT[k] = u;
                     HQ[k] = (int)q;
+                    INACT[u] = 0;
                 }
                 const u32 v = VT[q], t = v / 3, s = v % 3;
                 (s == 0 ? C12 : (s == 1 ? C13 : C23))[t] = k;
@@ -683,7 +684,7 @@ void mp_run_packed(Ctx& ctx, MPState& S, const MPPack& pk, int total_tblocks, in
     // alone; the edges in no triangle as mp_run's pair reduction sums them
     DBuf<double> pe = ctx.buf<double>(3 * (i64)Q);
     const float* EC = S.ec.get();
-    const int* SO = S.soff.get();
+    const unsigned char* NI = S.inact.get();
     const float* A = S.t12.get();
     const float* B = S.t13.get();
     const float* D = S.t23.get();
@@ -692,7 +693,7 @@ void mp_run_packed(Ctx& ctx, MPState& S, const MPPack& pk, int total_tblocks, in
     seg_reduce<double>(ctx, [=] __device__(i64 t) { return (double)tri_lb(A[t], B[t], D[t]); }, seg_tbeg_host,
                        pk.tbeg, Q, 0.0, OpSum(), pe.get() + Q, 1);
     seg_reduce<double>(
-        ctx, [=] __device__(i64 e) { const float c = EC[e]; return SO[e + 1] == SO[e] && c < 0.0f ? (double)c : 0.0; },
+        ctx, [=] __device__(i64 e) { const float c = EC[e]; return NI[e] && c < 0.0f ? (double)c : 0.0; },
         seg_ebeg_host, pk.ebeg, Q, 0.0, OpSum(), pe.get() + 2 * (i64)Q, 1);
     DBuf<MPControl> ctl = ctx.buf<MPControl>(Q);
     k_mp_init_packed<<<grid_for(Q), kBlock, 0, ctx.stream>>>(Q, pe.get(), pe.get() + Q, pe.get() + 2 * (i64)Q, d_on,
@@ -773,7 +774,7 @@ double mp_run(Ctx& ctx, MPState& S, int steps) {
     const float* B = S.t13.get();
     const float* D = S.t23.get();
     const i64 E = S.E;
-    const int* SO = S.soff.get();
+    const unsigned char* NI = S.inact.get();
     // the edges' bound term, and that of the edges in no triangle (constant
     // over the iterations), each summed as reduce_to sums it
     reduce_pair_to(
@@ -781,7 +782,7 @@ double mp_run(Ctx& ctx, MPState& S, int steps) {
         [=] __device__(i64 e, double& a, double& b) {
             const float c = EC[e];
             a = c < 0.0f ? (double)c : 0.0;
-            b = SO[e + 1] == SO[e] ? a : 0.0;
+            b = NI[e] ? a : 0.0;
         },
         E, P, P + 2);
     reduce_to<double>(ctx, [=] __device__(i64 t) { return (double)tri_lb(A[t], B[t], D[t]); }, S.T, 0.0, OpSum(),
@@ -816,10 +817,10 @@ double mp_fused_lower_bound(Ctx& ctx, MPState& S) {
     const double sum = sum_parts(ctx, S, n);
     // plus the bound term of the edges in no triangle
     const float* EC = S.ec.get();
-    const int* SO = S.soff.get();
+    const unsigned char* NI = S.inact.get();
     const double rest = reduce<double>(
-        ctx, [=] __device__(i64 e) { const float c = EC[e]; return SO[e + 1] == SO[e] && c < 0.0f ? (double)c : 0.0; },
-        S.E, 0.0, OpSum());
+        ctx, [=] __device__(i64 e) { const float c = EC[e]; return NI[e] && c < 0.0f ? (double)c : 0.0; }, S.E, 0.0,
+        OpSum());
     return sum + rest;
 }
 

@@ -134,7 +134,7 @@ struct MPState {
     DBuf<int> ei, ej;            // message-passing edges (sorted, unique)
     DBuf<float> ec;              // edge costs
     DBuf<float> share;           // per edge in a triangle (k): its cost / #triangles, what each triangle absorbs
-    DBuf<int> soff;              // edge -> its run of messages in delta (E+1)
+    DBuf<unsigned char> inact;   // per edge: 1 when it is in no triangle (its cost never changes)
     DBuf<u32> pos;               // slot 3t+k -> its place in delta (runs by edge, (triangle, slot) order)
     DBuf<int> c12, c13, c23;     // triangle -> number k of its edge among the edges in a triangle
     DBuf<float> t12, t13, t23;   // triangle costs
