@@ -339,15 +339,30 @@ DCoo contract(Ctx& ctx, const DCoo& A, const int* node_map, int max_label, bool*
 // leave a long run of empty rows, as a sparse forest over many nodes has,
 // to a single thread.)
 constexpr int kOffWin = 2048;
-__device__ __forceinline__ i64 lower_bound_ids(const int* __restrict__ a, i64 lo, i64 hi, int x) {
+// Sorted ids read through an accessor: an int array, the cols of the
+// transpose's sort (u32) or the rows packed in the low half of 8-byte words.
+struct IntIds {
+    const int* p;
+    __device__ __forceinline__ int operator[](i64 i) const { return p[i]; }
+};
+struct U32Ids {
+    const u32* p;
+    __device__ __forceinline__ int operator[](i64 i) const { return (int)p[i]; }
+};
+struct LowIds {
+    const u64* p;
+    __device__ __forceinline__ int operator[](i64 i) const { return (int)(u32)p[i]; }
+};
+template <class Ids>
+__device__ __forceinline__ i64 lower_bound_ids(Ids a, i64 lo, i64 hi, int x) {
     while (lo < hi) {
         const i64 mid = (lo + hi) >> 1;
         if (a[mid] < x) lo = mid + 1; else hi = mid;
     }
     return lo;
 }
-__global__ __launch_bounds__(kBlock) void k_offsets_rows(const int* __restrict__ ids, i64 n, int end_value,
-                                                         int* __restrict__ out) {
+template <class Ids>
+__global__ __launch_bounds__(kBlock) void k_offsets_rows(Ids ids, i64 n, int end_value, int* __restrict__ out) {
     __shared__ i64 range[2];
     __shared__ int sa[kOffWin];
     const i64 r0 = (i64)blockIdx.x * kBlock;
@@ -374,10 +389,15 @@ __global__ __launch_bounds__(kBlock) void k_offsets_rows(const int* __restrict__
     }
 }
 
-void compute_offsets(Ctx& ctx, const int* sorted_ids, i64 n, int max_value, int* out) {
+template <class Ids>
+static void offsets_of(Ctx& ctx, Ids ids, i64 n, int max_value, int* out) {
     const int end_value = max_value + 1;
-    k_offsets_rows<<<grid_for((i64)end_value + 1), kBlock, 0, ctx.stream>>>(sorted_ids, n, end_value, out);
+    k_offsets_rows<Ids><<<grid_for((i64)end_value + 1), kBlock, 0, ctx.stream>>>(ids, n, end_value, out);
     RAMA_LAUNCHED();
+}
+
+void compute_offsets(Ctx& ctx, const int* sorted_ids, i64 n, int max_value, int* out) {
+    offsets_of(ctx, IntIds{sorted_ids}, n, max_value, out);
 }
 
 // to_undirected + dCOO(symmetric) + compute_row_offsets
@@ -394,7 +414,7 @@ void compute_offsets(Ctx& ctx, const int* sorted_ids, i64 n, int max_value, int*
 // reference's stable sort of the doubled list does.
 template <class Keep>
 static Csr symmetric_csr_impl(Ctx& ctx, const int* row_in, const int* col_in, const float* val_in, i64 m,
-                              int n_nodes, bool filter, Keep keep, i64 known_count) {
+                              int n_nodes, bool filter, Keep keep, i64 known_count, bool sparse_rows = false) {
     Csr G;
     G.n = n_nodes;
     i64 P = m;
@@ -456,13 +476,22 @@ static Csr symmetric_csr_impl(Ctx& ctx, const int* row_in, const int* col_in, co
     int* OFw = offF.get();
     int* OTw = offT.get();
     // row offsets of the forward part (edge rows) and of the backward part
-    // (sorted cols), one launch
-    for_each(ctx, P + 1, [=] __device__(i64 i) {
-        const int pr = i > 0 ? (int)(u32)RV[i - 1] : -1, cr = i < P ? (int)(u32)RV[i] : n_nodes;
-        for (int r = pr + 1; r <= cr && r <= n_nodes; ++r) OFw[r] = (int)i;
-        const int pc = i > 0 ? (int)TK[i - 1] : -1, cc = i < P ? (int)TK[i] : n_nodes;
-        for (int r = pc + 1; r <= cc && r <= n_nodes; ++r) OTw[r] = (int)i;
-    });
+    // (sorted cols).  Most rows hold entries: one launch over the entries,
+    // each filling the rows from its predecessor's to its own.  The rows of
+    // a spanning forest (its smaller ends gather at low ids), or far more
+    // rows than entries, leave long runs of empty rows that would fall to
+    // single threads: one thread per row there.
+    if (sparse_rows || (i64)n_nodes > 4 * P) {
+        offsets_of(ctx, LowIds{RV}, P, n_nodes - 1, OFw);
+        offsets_of(ctx, U32Ids{TK}, P, n_nodes - 1, OTw);
+    } else {
+        for_each(ctx, P + 1, [=] __device__(i64 i) {
+            const int pr = i > 0 ? (int)(u32)RV[i - 1] : -1, cr = i < P ? (int)(u32)RV[i] : n_nodes;
+            for (int r = pr + 1; r <= cr && r <= n_nodes; ++r) OFw[r] = (int)i;
+            const int pc = i > 0 ? (int)TK[i - 1] : -1, cc = i < P ? (int)TK[i] : n_nodes;
+            for (int r = pc + 1; r <= cc && r <= n_nodes; ++r) OTw[r] = (int)i;
+        });
+    }
     int* OFF = G.off.get();
     const int* OF = OFw;
     const int* OT = OTw;
@@ -496,6 +525,10 @@ Csr symmetric_csr(Ctx& ctx, const int* row, const int* col, const float* val, i6
                   float keep_above, i64 known_count) {
     return symmetric_csr_impl(ctx, row, col, val, m, n_nodes, filter,
                               [=] __device__(i64 e) { return val[e] > keep_above; }, known_count);
+}
+
+Csr symmetric_csr_forest(Ctx& ctx, const int* row, const int* col, const float* val, i64 m, int n_nodes) {
+    return symmetric_csr_impl(ctx, row, col, val, m, n_nodes, false, [=] __device__(i64) { return true; }, -1, true);
 }
 
 // Stable filter of a CSR's entries: position of every kept entry by one

@@ -547,7 +547,7 @@ int woc_mapping(Ctx& ctx, const DCoo& A, DBuf<int>& map, i64& map_size) {
             FC[q] = PC[ID[q]];
             FWp[q] = PW[ID[q]];
         });
-        Csr F = symmetric_csr(ctx, fr.get(), fc.get(), fw.get(), k, S.n, false, 0.0f);
+        Csr F = symmetric_csr_forest(ctx, fr.get(), fc.get(), fw.get(), k, S.n);
         S.F = F.nnz;
         S.fadj = std::move(F.adj);
         S.fw = std::move(F.w);

@@ -65,6 +65,9 @@ DCoo contract(Ctx& ctx, const DCoo& A, const int* node_map, int max_label, bool*
 // `known_count`: number of edges passing the filter if the caller knows it (-1: count here).
 Csr symmetric_csr(Ctx& ctx, const int* row, const int* col, const float* val, i64 m, int n_nodes,
                   bool filter, float keep_above, i64 known_count = -1);
+// symmetric_csr of every edge of a spanning forest (row offsets found per row:
+// its rows leave long runs of empty rows).
+Csr symmetric_csr_forest(Ctx& ctx, const int* row, const int* col, const float* val, i64 m, int n_nodes);
 // The entries of F with weight > keep_above (kept of them), row order kept.
 Csr csr_filter(Ctx& ctx, const Csr& F, float keep_above, i64 kept);
 Csr symmetric_csr_slots(Ctx& ctx, const int* row, const int* col, const float* val, i64 m, int n_nodes,
