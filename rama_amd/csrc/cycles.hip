@@ -1364,7 +1364,10 @@ static i64 run_stage(Ctx& ctx, const Csr& G, const int* rv1, const int* rv2, i64
         }
         const bool early = !ctx.verbose && budget > base;
         const i64 need = budget - base;
-        unsigned done = 0, chunk = early ? std::max<unsigned>(nblk / 64, 256u) : nblk;
+        // the first chunk: a sixteenth of the blocks (a count that runs to
+        // the end pays this launch's tail once; smaller first chunks cost D
+        // mode on the 2D grid 5 ms, larger ones overshoot budgets)
+        unsigned done = 0, chunk = early ? std::max<unsigned>(nblk / 16, 256u) : nblk;
         i64 hc[2] = {0, X};
         while (done < nblk) {
             const unsigned n = std::min(chunk, nblk - done);
