@@ -60,17 +60,27 @@ __device__ __forceinline__ i64 bound_dev(const K* a, i64 lo, i64 hi, K x, bool u
     }
     return lo;
 }
+// The answers of each block's first and last query, one thread per search
+// (all blocks' searches run side by side instead of two per block ahead of
+// its own work).
+template <class K, bool kRepUpper>
+__global__ __launch_bounds__(kBlock) void k_sorted_lb_ends(const K* __restrict__ a, i64 n, const K* __restrict__ q,
+                                                           i64 m, i64* __restrict__ ends) {
+    const i64 t = (i64)blockIdx.x * kBlock + threadIdx.x;
+    const i64 nb = (m + kBlock - 1) / kBlock;
+    if (t >= 2 * nb) return;
+    const i64 b0 = (t >> 1) * kBlock;
+    const i64 qi = (t & 1) ? (b0 + kBlock < m ? b0 + kBlock : m) - 1 : b0;
+    ends[t] = bound_dev(a, 0, n, q[qi], lb_upper<K, kRepUpper>(q, qi));
+}
+
 template <class K, bool kRepUpper>
 __global__ __launch_bounds__(kBlock) void k_sorted_lb(const K* __restrict__ a, i64 n, const K* __restrict__ q, i64 m,
-                                                      int* __restrict__ out) {
-    __shared__ i64 range[2];
+                                                      const i64* __restrict__ ends, int* __restrict__ out) {
     __shared__ K sa[kLbWin];
     const i64 b0 = (i64)blockIdx.x * kBlock;
     const i64 b1 = (b0 + kBlock < m ? b0 + kBlock : m) - 1;
-    if (threadIdx.x == 0) range[0] = bound_dev(a, 0, n, q[b0], lb_upper<K, kRepUpper>(q, b0));
-    if (threadIdx.x == 64) range[1] = bound_dev(a, 0, n, q[b1], lb_upper<K, kRepUpper>(q, b1));
-    __syncthreads();
-    const i64 r0 = range[0], r1 = range[1];
+    const i64 r0 = ends[2 * (i64)blockIdx.x], r1 = ends[2 * (i64)blockIdx.x + 1];
     const i64 i = b0 + threadIdx.x;
     if (r1 - r0 <= kLbWin) {  // uniform
         const int w = (int)(r1 - r0);
@@ -94,7 +104,11 @@ __global__ __launch_bounds__(kBlock) void k_sorted_lb(const K* __restrict__ a, i
 template <class K, bool kRepUpper = false>
 static void sorted_lb(Ctx& ctx, const K* a, i64 n, const K* q, i64 m, int* out) {
     if (m <= 0) return;
-    k_sorted_lb<K, kRepUpper><<<grid_for(m), kBlock, 0, ctx.stream>>>(a, n, q, m, out);
+    const unsigned nb = grid_for(m);
+    DBuf<i64> ends = ctx.buf<i64>(2 * (i64)nb);
+    k_sorted_lb_ends<K, kRepUpper><<<grid_for(2 * (i64)nb), kBlock, 0, ctx.stream>>>(a, n, q, m, ends.get());
+    RAMA_LAUNCHED();
+    k_sorted_lb<K, kRepUpper><<<nb, kBlock, 0, ctx.stream>>>(a, n, q, m, ends.get(), out);
     RAMA_LAUNCHED();
 }
 
