@@ -4,19 +4,24 @@
 // Construction sorts the 3T triangle edges and merges them into the graph's
 // sorted edge list: graph edges keep their cost, new chords cost 0, and each
 // edge's run of triangle items is its slot list in (triangle, slot) order.
+// The edges in a triangle are numbered k in edge order; the iterations
+// work in that numbering (their costs, shares and message runs) and leave
+// the other edges alone.
 //
 // One iteration is two kernels:
-//   k_mp_triangles  per triangle: gather the three edge costs / counts, add
-//                   c/count to the triangle costs (send_messages_to_triplets),
-//                   run the six min-marginal steps, store the three messages
-//                   at their places in the edge-major message array and add
-//                   the triangle's lower-bound term;
-//   k_mp_edges      per edge: cost = sum of its run of messages, read
-//                   contiguously in (triangle, slot) order (replaces float
-//                   atomicAdd), plus the edge's lower-bound term.
-// Lower-bound partials are per block in double.  In the device-driven loop
-// the edge kernel's last block sums them and applies the stall test, so the
-// host reads the outcome once per loop.
+//   k_mp_triangles  per triangle: gather the shares (cost / #triangles) of
+//                   its three edges and add them to the triangle costs
+//                   (send_messages_to_triplets), run the six min-marginal
+//                   steps, store the three messages at their places in the
+//                   edge-major message array and add the triangle's
+//                   lower-bound term;
+//   k_mp_edges      per edge in a triangle: cost = sum of its run of
+//                   messages, read contiguously in (triangle, slot) order
+//                   (replaces float atomicAdd), plus its lower-bound term.
+// Lower-bound partials are per block in double; the edges in no triangle
+// add a constant term.  In the device-driven loop the edge kernel's last
+// block sums them and applies the stall test, so the host reads the outcome
+// once per loop.
 #include <cstdlib>
 #include <cstring>
 
