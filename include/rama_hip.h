@@ -157,6 +157,19 @@ int rama_stage_contract(const int32_t* row, const int32_t* col, const float* val
 /* compute_offsets(sorted ids, max_value) (include/rama_utils.h:403-413); out has max_value+2 entries. */
 int rama_stage_compute_offsets(const int32_t* ids, int64_t n, int32_t max_value, int32_t* out);
 
+/* to_undirected + dCOO(symmetric) + compute_row_offsets
+ * (include/rama_utils.h:107-131, src/dCOO.cu:144-147,164-172,
+ * src/conflicted_cycles_cuda.cu:208-217) of a (row, col)-sorted edge list
+ * with row < col: the symmetric adjacency with n_nodes rows, each row's
+ * neighbours ascending (a repeated pair keeps its input order).  mode 0:
+ * every edge; 1: the edges with cost > keep_above (is_positive_edge); 2:
+ * every edge, row offsets found one thread per row (the spanning-forest
+ * form).  out_off has n_nodes+1 entries, out_adj / out_w capacity cap;
+ * out_nnz = 2 x the edges taken. */
+int rama_stage_symmetric_csr(const int32_t* row, const int32_t* col, const float* val, int64_t m,
+                             int32_t n_nodes, int32_t mode, float keep_above, int32_t* out_off,
+                             int32_t* out_adj, float* out_w, int64_t cap, int64_t* out_nnz);
+
 /* compress_label_sequence(data, max_label) (include/rama_utils.h:160-177). */
 int rama_stage_compress_labels(const int32_t* data, int64_t n, int32_t max_label, int32_t* out);
 

@@ -63,6 +63,7 @@ SIGNATURES = {
     "rama_stage_dcoo_init": ([I32P, I32P, F32P, I64, I32P, I32P, F32P, I32P], I32),
     "rama_stage_contract": ([I32P, I32P, F32P, I64, I32P, I64, I32P, I32P, F32P, I64P, I32P, I32P, F64P], I32),
     "rama_stage_compute_offsets": ([I32P, I64, I32, I32P], I32),
+    "rama_stage_symmetric_csr": ([I32P, I32P, F32P, I64, I32, I32, F32, I32P, I32P, F32P, I64, I64P], I32),
     "rama_stage_compress_labels": ([I32P, I64, I32, I32P], I32),
     "rama_stage_matching": ([I32P, I32P, F32P, I64, I32, I32, F32, I32P, I64P, I32P], I32),
     "rama_stage_conflicted_cycles": ([I32P, I32P, F32P, I64, I32, I32, I32, F32, F32, I32P, I32P, I32P, I64, I64P,
@@ -249,6 +250,23 @@ def compute_offsets(ids, max_value):
     out = np.zeros(max_value + 2, np.int32)
     _check(lib().rama_stage_compute_offsets(_p(ids, I32P), len(ids), max_value, _p(out, I32P)))
     return out
+
+
+def symmetric_csr(row, col, val, n_nodes, mode=0, keep_above=0.0):
+    """Symmetric CSR (offsets, neighbours, costs) of a (row, col)-sorted
+    edge list with row < col; mode 1 keeps the edges with cost > keep_above,
+    mode 2 finds the row offsets one thread per row."""
+    row, col, val = _i32(row), _i32(col), _f32(val)
+    off = np.zeros(n_nodes + 1, np.int32)
+    cap = 2 * len(val) + 1
+    adj = np.zeros(cap, np.int32)
+    w = np.zeros(cap, np.float32)
+    nnz = ctypes.c_int64()
+    _check(lib().rama_stage_symmetric_csr(_p(row, I32P), _p(col, I32P), _p(val, F32P), len(val), n_nodes, mode,
+                                          keep_above, _p(off, I32P), _p(adj, I32P), _p(w, F32P), cap,
+                                          ctypes.byref(nnz)))
+    k = nnz.value
+    return off, adj[:k].copy(), w[:k].copy()
 
 
 def compress_labels(data, max_label):

@@ -343,6 +343,24 @@ int rama_stage_compute_offsets(const int32_t* ids, int64_t n, int32_t max_value,
     });
 }
 
+int rama_stage_symmetric_csr(const int32_t* row, const int32_t* col, const float* val, int64_t m,
+                             int32_t n_nodes, int32_t mode, float keep_above, int32_t* out_off,
+                             int32_t* out_adj, float* out_w, int64_t cap, int64_t* out_nnz) {
+    return guarded([&] {
+        if (m < 0 || n_nodes < 0 || mode < 0 || mode > 2) throw Error(RAMA_ERR_INVALID, "bad size or mode");
+        Ctx ctx(pick_device(), nullptr, false);
+        DBuf<int> r = upload(ctx, row, m), c = upload(ctx, col, m);
+        DBuf<float> v = upload(ctx, val, m);
+        Csr G = mode == 2 ? symmetric_csr_forest(ctx, r.get(), c.get(), v.get(), m, n_nodes)
+                          : symmetric_csr(ctx, r.get(), c.get(), v.get(), m, n_nodes, mode == 1, keep_above);
+        *out_nnz = G.nnz;
+        if (G.nnz > cap) throw Error(RAMA_ERR_INVALID, "adjacency buffer too small");
+        download(ctx, out_off, G.off.get(), (i64)n_nodes + 1);
+        download(ctx, out_adj, G.adj.get(), G.nnz);
+        download(ctx, out_w, G.w.get(), G.nnz);
+    });
+}
+
 int rama_stage_compress_labels(const int32_t* data, int64_t n, int32_t max_label, int32_t* out) {
     return guarded([&] {
         Ctx ctx(pick_device(), nullptr, false);
