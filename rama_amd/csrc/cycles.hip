@@ -1662,10 +1662,11 @@ Triangles conflicted_cycles(Ctx& ctx, const DCoo& A, int max_len, float factor, 
         return T;
     }
     Csr G;
-    if (ctx.last_cycles_empty) {
-        // the last search found nothing, so this dual pass likely leaves A
-        // as it is: build A's full symmetric CSR once (the matching reuses
-        // it) and take the attractive part out of it
+    if (A.sym || ctx.last_cycles_empty) {
+        // the contraction that made A left its full symmetric CSR, or the
+        // last search found nothing, so this dual pass likely leaves A as it
+        // is: build that CSR once (the matching reuses it) and take the
+        // attractive part out of it
         if (!A.sym) A.sym = std::make_shared<Csr>(symmetric_csr(ctx, A.row.get(), A.col.get(), W, A.nnz, N, false, 0.0f));
         G = csr_filter(ctx, *A.sym, -1.0f * thr, 2 * npos);
     } else {

@@ -148,20 +148,20 @@ __global__ __launch_bounds__(kBlock) void k_contract_reduce(const u64* __restric
 }
 
 // Statistics of the sorted contraction keys (grid-stride over the n sorted
-// edges): out[1] = largest col, out[0] = largest col of an off-diagonal
-// edge, out[2] = row of the last key; each self-loop run (the diagonal) is
-// summed in sorted order and rounded to float, a negative one raises out[3]
-// (bad contraction), and their total goes to *diag.  Each block stores its
-// results in `part` (4 ints) and `dpart`; a one-block launch combines them
-// in block order (no atomics: same-address atomics from every block
-// serialise).
+// keys; a key whose row exceeds its col is a mirrored copy, left out):
+// largest col, largest col of an off-diagonal edge, largest row; each
+// self-loop run (the diagonal) is summed in sorted order and rounded to
+// float, a negative one raises the bad-contraction flag, and their total
+// goes to *diag.  Each block stores its results in `part` (4 ints) and
+// `dpart`; a one-block launch combines them in block order (no atomics:
+// same-address atomics from every block serialise).
 __global__ __launch_bounds__(kBlock) void k_contract_stats(const u64* __restrict__ K, const u32* __restrict__ V,
                                                           i64 n, int kb, int* __restrict__ part,
                                                           double* __restrict__ dpart) {
     __shared__ int ws[kWaves];
     __shared__ double wd[kWaves];
     const u64 mask = ((u64)1 << kb) - 1;
-    int all = -1, kept = -1, bad = 0;
+    int all = -1, kept = -1, bad = 0, last = -1;
     double dsum = 0.0;
     const i64 stride = (i64)gridDim.x * kBlock;
     for (i64 e0 = (i64)blockIdx.x * kBlock + threadIdx.x; e0 < n; e0 += 4 * stride) {
@@ -178,8 +178,12 @@ __global__ __launch_bounds__(kBlock) void k_contract_stats(const u64* __restrict
             const i64 e = e0 + u * stride;
             if (e >= n) break;
             const u64 k = kk[u];
-            const int r = (int)(k >> kb), c = (int)(k & mask);
+            const u64 hi = k >> kb;
+            const int c = (int)(k & mask);
+            if (hi > (u64)c) continue;  // mirrored copy or its padding
+            const int r = (int)hi;
             all = c > all ? c : all;
+            last = r > last ? r : last;
             if (r != c) {
                 kept = c > kept ? c : kept;
             } else if (pk[u] != k) {
@@ -194,36 +198,42 @@ __global__ __launch_bounds__(kBlock) void k_contract_stats(const u64* __restrict
     all = block_reduce(all, OpMax(), ws);
     kept = block_reduce(kept, OpMax(), ws);
     bad = block_reduce(bad, OpMax(), ws);
+    last = block_reduce(last, OpMax(), ws);
     dsum = block_reduce(dsum, OpSum(), wd);
     if (threadIdx.x == 0) {
         part[4 * blockIdx.x] = all;
         part[4 * blockIdx.x + 1] = kept;
         part[4 * blockIdx.x + 2] = bad;
+        part[4 * blockIdx.x + 3] = last;
         dpart[blockIdx.x] = dsum;
     }
 }
 
-__global__ __launch_bounds__(kBlock) void k_contract_stats_combine(const u64* __restrict__ K, i64 n, int kb,
-                                                                  const int* __restrict__ part,
+// `merged`, when set, is the merge scan's packed total (forward edges << 32
+// | symmetric entries): its forward count becomes sc[1].
+__global__ __launch_bounds__(kBlock) void k_contract_stats_combine(const int* __restrict__ part,
                                                                   const double* __restrict__ dpart, unsigned nb,
-                                                                  int* sc, double* diag, int* mirror) {
+                                                                  const i64* merged, int* sc, double* diag,
+                                                                  int* mirror) {
     __shared__ int ws[kWaves];
     __shared__ double wd[kWaves];
-    int all = -1, kept = -1, bad = 0;
+    int all = -1, kept = -1, bad = 0, last = -1;
     double d = 0.0;
     for (unsigned b = threadIdx.x; b < nb; b += kBlock) {
-        const int x = part[4 * b], y = part[4 * b + 1], z = part[4 * b + 2];
+        const int x = part[4 * b], y = part[4 * b + 1], z = part[4 * b + 2], w = part[4 * b + 3];
         all = x > all ? x : all;
         kept = y > kept ? y : kept;
         bad = z > bad ? z : bad;
+        last = w > last ? w : last;
         d += dpart[b];
     }
     all = block_reduce(all, OpMax(), ws);
     kept = block_reduce(kept, OpMax(), ws);
     bad = block_reduce(bad, OpMax(), ws);
+    last = block_reduce(last, OpMax(), ws);
     d = block_reduce(d, OpSum(), wd);
     if (threadIdx.x == 0) {
-        const int last = (int)(K[n - 1] >> kb);
+        if (merged) sc[1] = (int)(*merged >> 32);
         sc[2] = kept;
         sc[3] = all;
         sc[4] = last;
@@ -242,6 +252,20 @@ __global__ __launch_bounds__(kBlock) void k_contract_stats_combine(const u64* __
 // dCOO::contract_cuda + has_bad_contractions + remove_diagonal
 // (src/dCOO.cu:57-95,97-142; src/rama_cuda.cu:18-30,136-140).
 // Counts stay on the device until one read-back at the end.
+//
+// Lists of at most kSymContractMax edges also yield the symmetric CSR of the
+// contracted graph (B.sym) from the same sort: each edge enters twice, as
+// (lo, hi) and mirrored as (hi, lo) (a self-loop's mirror as a padding key
+// above every real one), so the sorted runs of keys are the rows of the
+// symmetric matrix in order.  Both copies of an edge sit in the same order
+// within their runs, so a mirrored run sums to the same float as its
+// forward run.  The next cycle search filters this CSR instead of sorting
+// the transpose again.  (Long lists would double the largest sort of the
+// round to save a smaller one.)
+constexpr i64 kSymContractMax = (i64)1 << 19;
+
+static void sym_row_offsets(Ctx& ctx, const int* rows, i64 m, int n_nodes, int* off);
+
 DCoo contract(Ctx& ctx, const DCoo& A, const int* node_map, int max_label, bool* bad, double* diag_sum) {
     const i64 n = A.nnz;
     DCoo B;
@@ -251,8 +275,11 @@ DCoo contract(Ctx& ctx, const DCoo& A, const int* node_map, int max_label, bool*
         return B;
     }
     const int kb = bits_for(max_label < 1 ? 1 : max_label);
-    DBuf<u64> keys = ctx.buf<u64>(n);
-    DBuf<u32> vals = ctx.buf<u32>(n);
+    const bool dbl = n <= kSymContractMax;
+    const i64 nk = dbl ? 2 * n : n;
+    const u64 pad = (u64)1 << (2 * kb);
+    DBuf<u64> keys = ctx.buf<u64>(nk);
+    DBuf<u32> vals = ctx.buf<u32>(nk);
     u64* K = keys.get();
     u32* V = vals.get();
     const int* R = A.row.get();
@@ -269,13 +296,17 @@ DCoo contract(Ctx& ctx, const DCoo& A, const int* node_map, int max_label, bool*
         const u32 lo = (u32)(a < b ? a : b), hi = (u32)(a < b ? b : a);
         K[e] = ((u64)lo << kb) | hi;
         V[e] = f2u(W[e]);
+        if (dbl) {
+            K[n + e] = lo != hi ? (((u64)hi << kb) | lo) : pad;
+            V[n + e] = f2u(W[e]);
+        }
         if (e == 0) {
             for (int k = 0; k < 8; ++k) SC[k] = 0;
             *DG = 0.0;
         }
     });
     prof::Site site_("contract");
-    radix_sort_pairs(ctx, keys, vals, n, 2 * kb);
+    radix_sort_pairs(ctx, keys, vals, nk, dbl ? 2 * kb + 1 : 2 * kb);
     K = keys.get();  // an odd pass count hands back the scratch buffers
     V = vals.get();
 
@@ -286,7 +317,52 @@ DCoo contract(Ctx& ctx, const DCoo& A, const int* node_map, int max_label, bool*
     int* BR = B.row.get();
     int* BC = B.col.get();
     float* BW = B.val.get();
-    {
+    DBuf<int> srow, sadj;
+    DBuf<float> sw;
+    DBuf<i64> merged;
+    if (dbl) {
+        // the head of each run of equal off-diagonal keys sums its run and
+        // writes one symmetric entry; a forward head (row < col) also writes
+        // the merged edge.  One scan numbers both: forward heads count in
+        // the high half, all heads in the low half.
+        prof::Site site2_("contract_merge");
+        srow = ctx.buf<int>(nk);
+        sadj = ctx.buf<int>(nk);
+        sw = ctx.buf<float>(nk);
+        merged = ctx.buf<i64>(1);
+        int* SR = srow.get();
+        int* SA = sadj.get();
+        float* SW = sw.get();
+        scan_apply<i64>(
+            ctx,
+            [=] __device__(i64 e) -> i64 {
+                const u64 k = K[e];
+                if (e > 0 && K[e - 1] == k) return 0;
+                const u64 r = k >> kb, c = k & mask;
+                if (r == c || r > mask) return 0;  // diagonal, padding
+                return r < c ? ((i64)1 << 32) | 1 : 1;
+            },
+            nk,
+            [=] __device__(i64 e, i64 pre, i64 flag) {
+                if (!flag) return;
+                const u64 k = K[e];
+                double acc = 0.0;
+                for (i64 x = e; x < nk && K[x] == k; ++x) acc += (double)__uint_as_float(V[x]);
+                const int r = (int)(k >> kb), c = (int)(k & mask);
+                const float f = (float)acc;
+                const int p = (int)(u32)pre;
+                SR[p] = r;
+                SA[p] = c;
+                SW[p] = f;
+                if (flag >> 32) {
+                    const int q = (int)(pre >> 32);
+                    BR[q] = r;
+                    BC[q] = c;
+                    BW[q] = f;
+                }
+            },
+            merged.get());
+    } else {
         // reduce_by_key + remove_diagonal in one pass: the head of each run
         // of equal keys whose ends differ sums the run's costs in sorted
         // order (double, rounded once) and writes the merged edge
@@ -311,14 +387,15 @@ DCoo contract(Ctx& ctx, const DCoo& A, const int* node_map, int max_label, bool*
     // (src/dCOO.cu:91-92), the largest endpoint among the kept edges, and
     // the self-loop check
     {
-        const unsigned nb = (unsigned)std::min<i64>(std::max<i64>((n + kBlock - 1) / kBlock, 1), kReduceMaxBlocks);
+        const unsigned nb = (unsigned)std::min<i64>(std::max<i64>((nk + kBlock - 1) / kBlock, 1), kReduceMaxBlocks);
         char* scratch = static_cast<char*>(ctx.reduce_scratch((size_t)kReduceMaxBlocks * 24));
         int* part = reinterpret_cast<int*>(scratch);
         double* dpart = reinterpret_cast<double*>(scratch + (size_t)kReduceMaxBlocks * 16);
-        prof::Scope ps(ctx, "contract_stats", 12.0 * n);
-        k_contract_stats<<<nb, kBlock, 0, ctx.stream>>>(K, V, n, kb, part, dpart);
+        prof::Scope ps(ctx, "contract_stats", 12.0 * nk);
+        k_contract_stats<<<nb, kBlock, 0, ctx.stream>>>(K, V, nk, kb, part, dpart);
         RAMA_LAUNCHED();
-        k_contract_stats_combine<<<1, kBlock, 0, ctx.stream>>>(K, n, kb, part, dpart, nb, SC, DG, SM);
+        k_contract_stats_combine<<<1, kBlock, 0, ctx.stream>>>(part, dpart, nb, dbl ? merged.get() : nullptr, SC,
+                                                               DG, SM);
         RAMA_LAUNCHED();
     }
     int h[8];
@@ -329,6 +406,17 @@ DCoo contract(Ctx& ctx, const DCoo& A, const int* node_map, int max_label, bool*
     B.max_node = h[2];
     *bad = h[5] != 0;
     if (diag_sum) *diag_sum = ctx.read1(dg.get());
+    if (dbl && B.nnz > 0) {
+        // rows 0 .. max_dim() - 1, as the cycle search sizes its CSR
+        auto G = std::make_shared<Csr>();
+        G->n = B.max_dim();
+        G->nnz = 2 * B.nnz;
+        G->off = ctx.buf<int>((i64)G->n + 1);
+        sym_row_offsets(ctx, srow.get(), G->nnz, G->n, G->off.get());
+        G->adj = std::move(sadj);
+        G->w = std::move(sw);
+        B.sym = std::move(G);
+    }
     return B;
 }
 
@@ -398,6 +486,20 @@ static void offsets_of(Ctx& ctx, Ids ids, i64 n, int max_value, int* out) {
 
 void compute_offsets(Ctx& ctx, const int* sorted_ids, i64 n, int max_value, int* out) {
     offsets_of(ctx, IntIds{sorted_ids}, n, max_value, out);
+}
+
+// Row offsets off[0..n_nodes] of m entries with ascending rows: each entry
+// fills the rows from its predecessor's to its own, or, with far more rows
+// than entries (long runs of empty rows), one thread per row.
+static void sym_row_offsets(Ctx& ctx, const int* rows, i64 m, int n_nodes, int* off) {
+    if ((i64)n_nodes > 4 * m) {
+        offsets_of(ctx, IntIds{rows}, m, n_nodes - 1, off);
+        return;
+    }
+    for_each(ctx, m + 1, [=] __device__(i64 i) {
+        const int pr = i > 0 ? rows[i - 1] : -1, cr = i < m ? rows[i] : n_nodes;
+        for (int r = pr + 1; r <= cr && r <= n_nodes; ++r) off[r] = (int)i;
+    });
 }
 
 // to_undirected + dCOO(symmetric) + compute_row_offsets
