@@ -262,7 +262,7 @@ __global__ __launch_bounds__(kBlock) void k_contract_stats_combine(const int* __
 // forward run.  The next cycle search filters this CSR instead of sorting
 // the transpose again.  (Long lists would double the largest sort of the
 // round to save a smaller one.)
-constexpr i64 kSymContractMax = (i64)1 << 19;
+constexpr i64 kSymContractMax = (i64)1 << 18;
 
 static void sym_row_offsets(Ctx& ctx, const int* rows, i64 m, int n_nodes, int* off);
 
