@@ -154,6 +154,18 @@ int rama_stage_contract(const int32_t* row, const int32_t* col, const float* val
                         int32_t* out_row, int32_t* out_col, float* out_val, int64_t* out_m,
                         int32_t* out_dims, int32_t* out_bad, double* out_diag_sum);
 
+/* dCOO::contract_cuda + remove_diagonal, then to_undirected + dCOO(symmetric)
+ * + compute_row_offsets of the result (src/dCOO.cu:57-95,144-147,
+ * src/conflicted_cycles_cuda.cu:208-217): the contracted graph's symmetric
+ * adjacency as the next cycle search receives it, with out_n = max(rows,
+ * cols) rows (out_off holds out_n + 1 offsets, up to off_cap) and each row's
+ * neighbours ascending.  Short edge lists get it from the contraction's own
+ * sort of both edge directions; it equals the CSR that
+ * rama_stage_symmetric_csr (mode 0) builds from rama_stage_contract's output. */
+int rama_stage_contract_symmetric(const int32_t* row, const int32_t* col, const float* val, int64_t m,
+                                  const int32_t* node_map, int64_t map_size, int32_t* out_n, int32_t* out_off,
+                                  int64_t off_cap, int32_t* out_adj, float* out_w, int64_t cap, int64_t* out_nnz);
+
 /* compute_offsets(sorted ids, max_value) (include/rama_utils.h:403-413); out has max_value+2 entries. */
 int rama_stage_compute_offsets(const int32_t* ids, int64_t n, int32_t max_value, int32_t* out);
 

@@ -62,6 +62,7 @@ SIGNATURES = {
     "rama_objective": ([I32P, I64, I32P, I32P, F32P, I64], F64),
     "rama_stage_dcoo_init": ([I32P, I32P, F32P, I64, I32P, I32P, F32P, I32P], I32),
     "rama_stage_contract": ([I32P, I32P, F32P, I64, I32P, I64, I32P, I32P, F32P, I64P, I32P, I32P, F64P], I32),
+    "rama_stage_contract_symmetric": ([I32P, I32P, F32P, I64, I32P, I64, I32P, I32P, I64, I32P, F32P, I64, I64P], I32),
     "rama_stage_compute_offsets": ([I32P, I64, I32, I32P], I32),
     "rama_stage_symmetric_csr": ([I32P, I32P, F32P, I64, I32, I32, F32, I32P, I32P, F32P, I64, I64P], I32),
     "rama_stage_compress_labels": ([I32P, I64, I32, I32P], I32),
@@ -243,6 +244,25 @@ def contract(row, col, val, node_map):
                                      ctypes.byref(bad), ctypes.byref(ds)))
     k = mo.value
     return r[:k].copy(), co[:k].copy(), v[:k].copy(), (int(dims[0]), int(dims[1])), bool(bad.value), ds.value
+
+
+def contract_symmetric(row, col, val, node_map):
+    """Symmetric CSR (offsets, neighbours, costs) of the contracted graph,
+    max(rows, cols) rows, as the next cycle search receives it."""
+    row, col, val, node_map = _i32(row), _i32(col), _f32(val), _i32(node_map)
+    m = len(val)
+    off_cap = (int(node_map.max()) if len(node_map) else 0) + 2
+    off = np.zeros(off_cap, np.int32)
+    cap = 2 * m + 1
+    adj = np.zeros(cap, np.int32)
+    w = np.zeros(cap, np.float32)
+    n = ctypes.c_int32()
+    nnz = ctypes.c_int64()
+    _check(lib().rama_stage_contract_symmetric(_p(row, I32P), _p(col, I32P), _p(val, F32P), m, _p(node_map, I32P),
+                                               len(node_map), ctypes.byref(n), _p(off, I32P), off_cap, _p(adj, I32P),
+                                               _p(w, F32P), cap, ctypes.byref(nnz)))
+    k = nnz.value
+    return off[: n.value + 1].copy(), adj[:k].copy(), w[:k].copy()
 
 
 def compute_offsets(ids, max_value):

@@ -333,6 +333,31 @@ int rama_stage_contract(const int32_t* row, const int32_t* col, const float* val
     });
 }
 
+int rama_stage_contract_symmetric(const int32_t* row, const int32_t* col, const float* val, int64_t m,
+                                  const int32_t* node_map, int64_t map_size, int32_t* out_n, int32_t* out_off,
+                                  int64_t off_cap, int32_t* out_adj, float* out_w, int64_t cap, int64_t* out_nnz) {
+    return guarded([&] {
+        if (m < 0 || map_size < 0) throw Error(RAMA_ERR_INVALID, "bad size");
+        Ctx ctx(pick_device(), nullptr, false);
+        DCoo A = upload_dcoo(ctx, row, col, val, m, 0, 0);
+        DBuf<int> mp = upload(ctx, node_map, map_size);
+        int mx = 0;
+        for (int64_t k = 0; k < map_size; ++k) mx = std::max(mx, node_map[k]);
+        bool bad = false;
+        DCoo B = contract(ctx, A, mp.get(), mx, &bad, nullptr);
+        const int n = B.max_dim();
+        // the CSR the contraction built, else the one the cycle search builds
+        Csr G = B.sym ? std::move(*B.sym)
+                      : symmetric_csr(ctx, B.row.get(), B.col.get(), B.val.get(), B.nnz, n, false, 0.0f);
+        *out_n = n;
+        *out_nnz = G.nnz;
+        if ((int64_t)n + 1 > off_cap || G.nnz > cap) throw Error(RAMA_ERR_INVALID, "output buffer too small");
+        download(ctx, out_off, G.off.get(), (i64)n + 1);
+        download(ctx, out_adj, G.adj.get(), G.nnz);
+        download(ctx, out_w, G.w.get(), G.nnz);
+    });
+}
+
 int rama_stage_compute_offsets(const int32_t* ids, int64_t n, int32_t max_value, int32_t* out) {
     return guarded([&] {
         Ctx ctx(pick_device(), nullptr, false);

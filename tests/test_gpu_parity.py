@@ -79,6 +79,57 @@ def test_contract(gpu, oracle, name, groups):
     assert abs(g[5] - o[5]) <= 1e-9 * max(1.0, abs(o[5]))
 
 
+def _doubled_csr(r, c, w, n):
+    """Symmetric CSR by a stable sort of the doubled edge list: row r holds
+    r' of the edges (r', r) and c of the edges (r, c), ascending."""
+    rows = np.concatenate([r, c])
+    cols = np.concatenate([c, r])
+    ws = np.concatenate([w, w])
+    o = np.lexsort((cols, rows))
+    off = np.searchsorted(rows[o], np.arange(n + 1), side="left").astype(np.int32)
+    return off, cols[o].astype(np.int32), ws[o]
+
+
+def _check_contract_symmetric(gpu, oracle, r, c, v, mp):
+    r2, c2, v2, dims2, _, _ = oracle.contract(r, c, v, mp)
+    ref = _doubled_csr(r2, c2, v2, max(dims2))
+    got = gpu.contract_symmetric(r, c, v, mp)
+    assert np.array_equal(got[0], ref[0])
+    assert np.array_equal(got[1], ref[1])
+    assert np.array_equal(got[2].view(np.uint32), ref[2].view(np.uint32))
+
+
+@pytest.mark.parametrize("name", NAMES + ["random_dup"])
+@pytest.mark.parametrize("groups", [2, 7, 50])
+def test_contract_symmetric(gpu, oracle, name, groups):
+    # the CSR that short contractions build from their sort of both edge
+    # directions (merged costs, self-loops dropped) against the oracle's
+    # contraction symmetrised by a stable sort
+    r, c, v, dims = _dcoo(oracle, name)
+    n = max(dims)
+    rng = np.random.Generator(np.random.PCG64(groups + 100))
+    mp = rng.integers(0, max(1, n // groups), size=n).astype(np.int32)
+    _check_contract_symmetric(gpu, oracle, r, c, v, mp)
+
+
+@pytest.mark.parametrize("shape", [(150, 200), (200, 300)])
+def test_contract_symmetric_sizes(gpu, oracle, shape):
+    # 2^18 edges is where contractions stop sorting both directions: one
+    # input below (about 170k edges) and one above (about 342k), each with
+    # a sparse and a merging map; an all-self-loop map leaves no edges
+    from rama_amd import instances
+
+    i, j, c, _ = instances.grid2d_voronoi(H=shape[0], W=shape[1], seed=3)
+    r, co, v, dims = oracle.dcoo_init(i, j, c)
+    n = max(dims)
+    rng = np.random.Generator(np.random.PCG64(shape[0]))
+    for groups in (2, 40):
+        mp = rng.integers(0, max(1, n // groups), size=n).astype(np.int32)
+        _check_contract_symmetric(gpu, oracle, r, co, v, mp)
+    off, adj, w = gpu.contract_symmetric(r, co, v, np.zeros(n, np.int32))
+    assert len(adj) == 0 and len(w) == 0 and np.array_equal(off, np.zeros(len(off), np.int32))
+
+
 def test_compute_offsets_and_compress(gpu, oracle):
     rng = np.random.Generator(np.random.PCG64(1))
     ids = np.sort(rng.integers(0, 5000, size=20000)).astype(np.int32)
