@@ -147,6 +147,23 @@ __global__ __launch_bounds__(kBlock) void k_contract_reduce(const u64* __restric
     }
 }
 
+// Sum, in order and in double, of the costs of the run of keys equal to k
+// that starts at its head e (the caller rounds it once).  The head's cost
+// and the next key are loaded together, so a run of one edge, the common
+// case, costs one round of loads; the additions keep their order.
+__device__ __forceinline__ double head_run_sum(const u64* __restrict__ K, const u32* __restrict__ V, u64 k, i64 e,
+                                               i64 n) {
+    const u32 v0 = V[e];
+    const u64 k1 = e + 1 < n ? K[e + 1] : ~k;
+    double acc = 0.0;
+    acc += (double)__uint_as_float(v0);
+    if (k1 == k) {
+        acc += (double)__uint_as_float(V[e + 1]);
+        for (i64 x = e + 2; x < n && K[x] == k; ++x) acc += (double)__uint_as_float(V[x]);
+    }
+    return acc;
+}
+
 // Statistics of the sorted contraction keys (grid-stride over the n sorted
 // keys; a key whose row exceeds its col is a mirrored copy, left out):
 // largest col, largest col of an off-diagonal edge, largest row; each
@@ -346,10 +363,8 @@ DCoo contract(Ctx& ctx, const DCoo& A, const int* node_map, int max_label, bool*
             [=] __device__(i64 e, i64 pre, i64 flag) {
                 if (!flag) return;
                 const u64 k = K[e];
-                double acc = 0.0;
-                for (i64 x = e; x < nk && K[x] == k; ++x) acc += (double)__uint_as_float(V[x]);
                 const int r = (int)(k >> kb), c = (int)(k & mask);
-                const float f = (float)acc;
+                const float f = (float)head_run_sum(K, V, k, e, nk);
                 const int p = (int)(u32)pre;
                 SR[p] = r;
                 SA[p] = c;
@@ -375,11 +390,9 @@ DCoo contract(Ctx& ctx, const DCoo& A, const int* node_map, int max_label, bool*
             },
             [=] __device__(i64 e, int q) {
                 const u64 k = K[e];
-                double acc = 0.0;
-                for (i64 x = e; x < n && K[x] == k; ++x) acc += (double)__uint_as_float(V[x]);
                 BR[q] = (int)(k >> kb);
                 BC[q] = (int)(k & mask);
-                BW[q] = (float)acc;
+                BW[q] = (float)head_run_sum(K, V, k, e, n);
             },
             SC + 1);
     }
