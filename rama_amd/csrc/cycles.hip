@@ -1678,11 +1678,12 @@ Triangles conflicted_cycles(Ctx& ctx, const DCoo& A, int max_len, float factor, 
         int* X2 = rv2.get();
         const int* AR = A.row.get();
         const int* AC = A.col.get();
-        select_into(
+        select_into_loaded(
             ctx, A.nnz, [=] __device__(i64 e) { return W[e] < thr; },
-            [=] __device__(i64 e, int q) {
-                X1[q] = AR[e];
-                X2[q] = AC[e];
+            [=] __device__(i64 e) { return make_int2(AR[e], AC[e]); },
+            [=] __device__(i64, int q, int2 rc) {
+                X1[q] = rc.x;
+                X2[q] = rc.y;
             },
             reinterpret_cast<int*>(&SZ->pad));
     }

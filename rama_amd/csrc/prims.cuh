@@ -161,6 +161,19 @@ __global__ __launch_bounds__(kBlock) void k_scan_up(F f, i64 n, i64 chunk, T* __
     if (threadIdx.x == 0) partials[blockIdx.x] = r;
 }
 
+// An Out may also load what it will need per item before the scan
+// (`load(i, value)`, returning a payload that `emit(i, prefix, value,
+// payload)` then receives): every item's loads are then issued together
+// instead of one item's loads waiting on the previous item's stores.
+template <class O, class = void>
+struct OutLoads : std::false_type {
+    using type = int;
+};
+template <class O>
+struct OutLoads<O, std::void_t<decltype(std::declval<const O&>().load((i64)0, 0))>> : std::true_type {
+    using type = decltype(std::declval<const O&>().load((i64)0, 0));
+};
+
 // out(i, exclusive_prefix, value) is called for every i < n; total (if not
 // null) receives the sum of all values.
 template <class T, class F, class Out>
@@ -168,6 +181,8 @@ __global__ __launch_bounds__(kBlock) void k_scan_down(F f, i64 n, i64 chunk, Out
                                                       const T* __restrict__ partials, T* total, T* mirror) {
     __shared__ T ws[kWaves];
     __shared__ T sv[kBlock * kScanPad];
+    constexpr bool loads = OutLoads<Out>::value;
+    using Payload = typename OutLoads<Out>::type;
     T p = T(0);
     for (unsigned b = threadIdx.x; b < blockIdx.x; b += kBlock) p += partials[b];
     T run = block_reduce(p, OpSum(), ws);
@@ -175,12 +190,20 @@ __global__ __launch_bounds__(kBlock) void k_scan_down(F f, i64 n, i64 chunk, Out
     const i64 c1 = c0 + chunk < n ? c0 + chunk : n;
     for (i64 tile0 = c0; tile0 < c1; tile0 += kScanTile) {
         T v[kScanItems];
+        Payload pl[kScanItems];
 #pragma unroll
         for (int k = 0; k < kScanItems; ++k) {
             const int j = k * kBlock + threadIdx.x;
             const i64 i = tile0 + j;
             v[k] = i < c1 ? (T)f(i) : T(0);
             sv[scan_slot(j)] = v[k];
+        }
+        if constexpr (loads) {
+#pragma unroll
+            for (int k = 0; k < kScanItems; ++k) {
+                const i64 i = tile0 + k * kBlock + threadIdx.x;
+                pl[k] = out.load(i < c1 ? i : -1, v[k]);
+            }
         }
         __syncthreads();
         T s = T(0);
@@ -204,7 +227,11 @@ __global__ __launch_bounds__(kBlock) void k_scan_down(F f, i64 n, i64 chunk, Out
         for (int k = 0; k < kScanItems; ++k) {
             const int j = k * kBlock + threadIdx.x;
             const i64 i = tile0 + j;
-            if (i < c1) out(i, sv[scan_slot(j)], v[k]);
+            if constexpr (loads) {
+                if (i < c1) out.emit(i, sv[scan_slot(j)], v[k], pl[k]);
+            } else {
+                if (i < c1) out(i, sv[scan_slot(j)], v[k]);
+            }
         }
         run += agg;
         __syncthreads();  // the next tile overwrites sv
@@ -621,6 +648,28 @@ struct SelectOut {
 template <class P, class Emit>
 void select_into(Ctx& ctx, i64 n, P pred, Emit emit, int* d_count, int* mirror = nullptr) {
     scan_apply<int>(ctx, FlagFn<P>{pred}, n, SelectOut<Emit>{emit}, d_count, mirror);
+}
+
+template <class Load, class Emit>
+struct SelectLoadOut {
+    Load ld;
+    Emit emit_;
+    // i < 0: past the end; unselected items load nothing
+    __device__ __forceinline__ auto load(i64 i, int flag) const {
+        using R = decltype(ld((i64)0));
+        return i >= 0 && flag ? ld(i) : R{};
+    }
+    template <class R>
+    __device__ __forceinline__ void emit(i64 i, int pos, int flag, const R& r) const {
+        if (flag) emit_(i, pos, r);
+    }
+};
+
+// select_into whose emit(i, position, load(i)) receives what load(i) read
+// for the selected items, all loaded before the block's scan.
+template <class P, class Load, class Emit>
+void select_into_loaded(Ctx& ctx, i64 n, P pred, Load load, Emit emit, int* d_count, int* mirror = nullptr) {
+    scan_apply<int>(ctx, FlagFn<P>{pred}, n, SelectLoadOut<Load, Emit>{load, emit}, d_count, mirror);
 }
 
 // ------------------------------------------------------------ radix sort
