@@ -563,13 +563,13 @@ static Csr symmetric_csr_impl(Ctx& ctx, const int* row_in, const int* col_in, co
         fc = ctx.buf<int>(P);
         int* FC = fc.get();
         DBuf<int> cnt = ctx.buf<int>(1);
-        select_into(
+        select_into_loaded(
             ctx, m, [=] __device__(i64 e) { return keep(e); },
-            [=] __device__(i64 e, int q) {
-                const int c = col_in[e];
-                RV[q] = ((u64)__float_as_uint(val_in[e]) << 32) | (u64)(u32)row_in[e];
-                FC[q] = c;
-                TK[q] = (u32)c;
+            [=] __device__(i64 e) { return make_int3(row_in[e], col_in[e], __float_as_int(val_in[e])); },
+            [=] __device__(i64, int q, int3 rcw) {
+                RV[q] = ((u64)(u32)rcw.z << 32) | (u64)(u32)rcw.x;
+                FC[q] = rcw.y;
+                TK[q] = (u32)rcw.y;
                 TV[q] = (u32)q;
             },
             cnt.get());
