@@ -140,16 +140,22 @@ def cpu_baseline():
                      % (len(c), dt),
            "host_cpus": os.cpu_count(),
            "grid64": {"edges": len(c0), "ms_per_solve": dt0 * 1e3, "edges_per_s": len(c0) / dt0}}
-    # the oracle's full-size solves, timed once when their fixture was written
-    # (too long to repeat here); not measured in this run
-    path = os.path.join(ROOT, "tests", "golden", "full_size_oracle.json")
-    if os.path.exists(path):
-        with open(path) as f:
-            rec = json.load(f)
-        out["full_size_recorded"] = {
-            k: {"edges": v["edges"], "seconds": v["oracle_seconds"], "edges_per_s": v["edges"] / v["oracle_seconds"]}
-            for k, v in rec.items() if "oracle_seconds" in v}
+    rec = full_size_recorded()
+    if rec:
+        out["full_size_recorded"] = rec
     return out
+
+
+def full_size_recorded():
+    """The oracle's full-size solves, timed once when their fixture was
+    written (too long to repeat in a bench run; not measured in this run)."""
+    path = os.path.join(ROOT, "tests", "golden", "full_size_oracle.json")
+    if not os.path.exists(path):
+        return {}
+    with open(path) as f:
+        rec = json.load(f)
+    return {k: {"edges": v["edges"], "seconds": v["oracle_seconds"], "edges_per_s": v["edges"] / v["oracle_seconds"]}
+            for k, v in rec.items() if "oracle_seconds" in v}
 
 
 def copy_peak(torch, nbytes=2 << 30, reps=5):
