@@ -155,11 +155,11 @@ int rama_stage_contract(const int32_t* row, const int32_t* col, const float* val
                         int32_t* out_dims, int32_t* out_bad, double* out_diag_sum);
 
 /* dCOO::contract_cuda + remove_diagonal, then to_undirected + dCOO(symmetric)
- * + compute_row_offsets of the result (src/dCOO.cu:57-95,144-147,
- * src/conflicted_cycles_cuda.cu:208-217): the contracted graph's symmetric
- * adjacency as the next cycle search receives it, with out_n = max(rows,
- * cols) rows (out_off holds out_n + 1 offsets, up to off_cap) and each row's
- * neighbours ascending.  Short edge lists get it from the contraction's own
+ * + compute_row_offsets of the result (src/dCOO.cu:57-95,97-142,144-147,
+ * include/rama_utils.h:107-131, src/conflicted_cycles_cuda.cu:208-217): the
+ * contracted graph's symmetric adjacency as the next cycle search receives
+ * it, with out_n = max(rows, cols) rows (out_off holds out_n + 1 offsets, up
+ * to off_cap) and each row's neighbours ascending.  Short edge lists get it from the contraction's own
  * sort of both edge directions; it equals the CSR that
  * rama_stage_symmetric_csr (mode 0) builds from rama_stage_contract's output. */
 int rama_stage_contract_symmetric(const int32_t* row, const int32_t* col, const float* val, int64_t m,
